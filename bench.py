@@ -1,0 +1,200 @@
+"""Benchmark of the 2-D TM hot path on one MI355X.
+
+    python bench.py [--steps K] [--warmup W] [--size N] [--depth D] [--rows H] [--cols C]
+
+Workload (the 2-D TM configuration the metric is quoted on): N x N FP32 grid
+(default 4096), npml=8 Sullivan PML, one lossy cylinder of radius 400 cells
+(eps_r=4, sigma=0.02) at (N/2-1, N/2-1), hard Gaussian source (t0=20,
+spread 6) at (N/2, N/2), four probe traces recorded every step.  All fields
+start at zero and live in HBM before timing.  A "step" is one leapfrog time
+step of the whole grid; K timed steps run through fd2d_tm_run (temporally
+blocked launches) after W untimed warm-up steps, bracketed by
+torch.cuda.synchronize().
+
+The JSON line adds:
+  roofline     for the dominant kernel (tm_tb_kernel): algorithmic bytes per
+               launch (60 B per cell-update x cells x fused steps) over its
+               average launch duration (HIP events on the launch stream over
+               the timed region / number of launches), against 8 TB/s;
+               traffic = HBM bytes per launch from rocprofv3 FETCH_SIZE +
+               WRITE_SIZE when a summary for this workload is committed under
+               profiles/, else null.
+  cpu_baseline the reference's own FP32 C kernels (fd2d/clang/test_3_4.c,
+               OpenMP, built into oracle/_ref/) -- or this project's C
+               restatement when that build is absent -- timed on the host for a
+               bounded number of steps of the same grid.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+B_ALG = 60          # bytes per cell-update, lossy PML form
+HBM_PEAK = 8000.0   # GB/s, MI355X
+METRIC = "2D TM Mcell-updates/s + HBM roofline frac; learned-BC vs PML at equal reflection"
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=40)
+    ap.add_argument("--size", type=int, default=4096)
+    ap.add_argument("--depth", type=int, default=0)
+    ap.add_argument("--rows", type=int, default=0)
+    ap.add_argument("--cols", type=int, default=0)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0,
+                    help="budget of the CPU baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    return ap.parse_args()
+
+
+def cpu_baseline(n: int, budget_s: float) -> dict:
+    """Time the reference C FP32 step (dfield, source, efield, hfield) on the host."""
+    import numpy as np
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    ref = os.path.join(ROOT, "oracle", "_ref", "libref_test_3_4.so")
+    f = lambda: np.zeros((n, n), np.float32)  # noqa: E731
+    dz, iz, ez, hx, hy, ihx, ihy = (f() for _ in range(7))
+    naz = np.ones((n, n), np.float32)
+    nbz = np.zeros((n, n), np.float32)
+    P = lambda a: a.ctypes.data_as(C.POINTER(C.c_float))  # noqa: E731
+    prof = {k: (np.zeros(n, np.float32) if k.endswith("1") else np.ones(n, np.float32))
+            for k in ("fx1", "fx2", "fx3", "fy1", "fy2", "fy3", "gx2", "gx3", "gy2", "gy3")}
+    if os.path.exists(ref):
+        kind = "reference"
+        L = C.CDLL(ref)
+
+        class Pml(C.Structure):
+            _fields_ = [(k, C.POINTER(C.c_float)) for k in prof]
+
+        class Med(C.Structure):
+            _fields_ = [("naz", C.POINTER(C.c_float)), ("nbz", C.POINTER(C.c_float))]
+        pml = Pml(*[P(prof[k]) for k in prof])
+        md = Med(P(naz), P(nbz))
+        ezi = np.zeros(n, np.float32)
+
+        def step(t):
+            L.dfield(t, n, n, C.byref(pml), P(ezi), P(dz), P(hx), P(hy))
+            dz[n // 2, n // 2] = 1.0
+            L.efield(n, n, C.byref(md), P(dz), P(iz), P(ez))
+            L.hfield(n, n, C.byref(pml), P(ez), P(ihx), P(ihy), P(hx), P(hy))
+    else:
+        kind = "port"
+        from oracle import oracle as O
+        st = O.TMState(n, n, np.float32)
+        w = np.ones(1, np.float32)
+
+        def step(t):
+            O.fd2d_tm_run(st, t - 1, 1, prof, naz, nbz, [(n // 2, n // 2)], w[None, :])
+    step(1)                                   # warm-up (page faults, thread pool)
+    t0 = time.perf_counter()
+    steps = 0
+    while True:
+        step(steps + 2)
+        steps += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s or steps >= 200:
+            break
+    rate = n * n * steps / el / 1e6
+    return {"value": round(rate, 2), "unit": "Mcell-updates/s", "cores": threads, "kind": kind,
+            "sample": f"{steps} steps of the {n}x{n} FP32 TM step (dfield+source+efield+hfield), "
+                      f"{el:.1f} s, OMP_NUM_THREADS={threads}"}
+
+
+def traffic_for(workload: str):
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        data = json.load(open(path))
+    except ValueError:
+        return None
+    rec = data.get(workload)
+    return None if rec is None else rec.get("bytes_per_launch")
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus N>1 is not supported "
+                                   "(the multi-GPU slab decomposition is a design, not code)"}))
+        sys.exit(2)
+    import torch
+    from simulation_amd import fd2d
+
+    dev = torch.device("cuda", 0)
+    n, K, W = a.size, a.steps, a.warmup
+    cfg = fd2d.c2_config(n, W + K)
+    cfg.depth, cfg.rows, cfg.cols = a.depth, a.rows, a.cols
+    s = fd2d.build_solver(cfg, dev)
+    plan = s.plan(K)
+    s.run(W)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e0.record(stream)
+    s.run(K)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ev_ms = e0.elapsed_time(e1)
+    cells = n * n
+    value = cells * K / wall / 1e6
+    launch_ms = ev_ms / max(plan["launches"], 1)
+    bytes_per_launch = B_ALG * cells * K / max(plan["launches"], 1)
+    achieved = bytes_per_launch / (launch_ms * 1e-3) / 1e9
+    workload = f"fd2d_tm_{n}x{n}_npml8_cyl400_2000steps"
+    rec = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "Mcell-updates/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": round(wall * 1e3 / K, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (fields start at zero; Gaussian point source; seeded geometry)",
+        "config": {
+            "workload": f"2-D TM {n}x{n} FP32, npml=8 Sullivan PML, lossy cylinder r=400 "
+                        f"eps_r=4 sigma=0.02, hard Gaussian t0=20 spread=6 at centre, 4 probes",
+            "grid": [n, n],
+            "depth": plan["depth"], "rows": plan["rows"], "cols": plan["cols"],
+            "launches": plan["launches"],
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK, 4),
+            "traffic": traffic_for(workload),
+            "kernel": "tm_tb_kernel",
+            "avg_launch_ms": round(launch_ms, 4),
+            "alg_bytes_per_launch": int(bytes_per_launch),
+        },
+        "cpu_baseline": None,
+    }
+    if not a.no_cpu:
+        try:
+            rec["cpu_baseline"] = cpu_baseline(n, a.cpu_seconds)
+        except Exception as e:   # report, never fake
+            rec["cpu_baseline"] = {"error": repr(e)}
+    print(json.dumps(rec), flush=True)
+
+
+if __name__ == "__main__":
+    main()

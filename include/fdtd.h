@@ -1,0 +1,226 @@
+/*
+ * fdtd.h -- C-ABI of libfdtd_hip.so, the MI355X (gfx950) FDTD hot path.
+ *
+ * Drop-in boundary for the PyCUDA programs of dsarvan/simulation
+ * (fd1d/pycuda/*.py, fd2d/pycuda/*.py).  There, the "operator API" is
+ * `SourceModule(kernel).get_function(name)` and a call
+ * `f(*args, grid=, block=)` on gpuarrays (fd2d/pycuda/fd2d_3_2.py:333-341).
+ * Here each such kernel is an extern "C" entry point with the same argument
+ * order and meaning, taking raw device pointers (e.g. torch.Tensor.data_ptr())
+ * and a HIP stream (hipStream_t passed as void*, NULL = the null stream).
+ *
+ * Conventions
+ *   - Return value: 0 on success; FDTD_EINVAL (< 0) when an argument is
+ *     rejected before any device work is queued; a positive hipError_t when
+ *     the HIP runtime reports an error.  fdtd_last_error() gives a message.
+ *     The reference checks no error at all (its CUDA errors surface at the
+ *     next drv.Context.synchronize(), fd2d/pycuda/fd2d_3_2.py:343).
+ *   - Launches are asynchronous on `stream`, in order, like PyCUDA's calls
+ *     on the null stream.  Entry points that return a host result say so.
+ *   - No entry point allocates device memory: every buffer, including the
+ *     ping-pong partner set and the reduction work area, is the caller's.
+ *   - 2-D arrays are row-major nx*ny with n = i*ny + j, j contiguous
+ *     (fd2d/pycuda/fd2d_3_1.py:63, :106-111).
+ *   - Field arithmetic is IEEE FP32 without contraction, in the expression
+ *     order of the reference kernels, so results equal the reference's FP32
+ *     C programs bit for bit.  FP64 is used for coefficient set-up and
+ *     energy accounting only.
+ *   - Sources are evaluated by the caller (host) and passed as values; the
+ *     cell that owns a source applies it, so the reference's source/boundary
+ *     races (a thread writing a cell another thread updates) cannot occur.
+ */
+#ifndef FDTD_H
+#define FDTD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FDTD_ABI_VERSION 1
+
+#define FDTD_OK 0
+#define FDTD_EINVAL (-1)
+
+#define FDTD_MAX_SOURCES 16
+#define FDTD_MAX_PROBES 16
+#define FDTD_MAX_CYLINDERS 64
+#define FDTD_ENERGY_WORK 1025   /* doubles in the energy work area */
+
+/* Source modes of the 1-D E update. */
+#define FDTD_SRC_NONE 0
+#define FDTD_SRC_HARD 1        /* ex[pos] = value   (fd1d/pycuda/fd1d_1_2.py:49) */
+#define FDTD_SRC_ADD 2         /* ex[pos] += value  (fd1d/pycuda/fd1d_1_5.py:56) */
+
+/* The PML profile set.  Replaces the device struct `pmlayer` and its device
+ * pointer table (fd2d/pycuda/fd2d_3_2.py:204-209, :306-317): here a host
+ * struct of device pointers, copied into kernel arguments.  fx*, gx* have nx
+ * elements, fy*, gy* have ny.                                               */
+typedef struct fdtd_pml {
+    float *fx1, *fx2, *fx3;
+    float *fy1, *fy2, *fy3;
+    float *gx2, *gx3;
+    float *gy2, *gy3;
+} fdtd_pml;
+
+/* Replaces the device struct `medium` (fd2d/pycuda/fd2d_3_4.py:96-98, :322-324). */
+typedef struct fdtd_medium {
+    float *naz, *nbz;
+} fdtd_medium;
+
+/* ------------------------------------------------------------ library */
+
+int fdtd_abi_version(void);
+/* Message of the last failed call on this host thread ("" if none). */
+const char *fdtd_last_error(void);
+
+/* ---------------------------------------------------- host waveforms
+ * Evaluated on the host with the host libm, in the reference's expression
+ * order, so values equal its C programs':
+ *   gaussian   fd2d/pycuda/fd2d_3_1.py:52-55, fd1d/clang/test_1_2.c:13-15
+ *   sinusoidal fd2d/pycuda/fd2d_3_2.py:212-216, fd2d/clang/test_3_2.c:22-25 */
+float fdtd_gaussian_f32(int t, int t0, float sigma);
+double fdtd_gaussian_f64(int t, int t0, double sigma);
+float fdtd_sinusoidal_f32(int t, float ds, float freq);
+double fdtd_sinusoidal_f64(int t, double ds, double freq);
+
+/* ------------------------------------------------------------- 1-D */
+
+/* E update for i in [1,nx) then the source.  Replaces `exfield` of
+ * fd1d/pycuda/fd1d_1_5.py:49-56 (ca, cb), fd1d/pycuda/fd1d_1_3.py:46-53
+ * (ca NULL: ca = 1) and fd1d/pycuda/fd1d_1_2.py:43-50 (ca, cb NULL: cb = 0.5).
+ * src_mode FDTD_SRC_*, src_val evaluated by the caller for step t.          */
+int fd1d_exfield(int t, int nx, const float *ca, const float *cb,
+                 float *ex, const float *hy,
+                 int src_mode, int src_pos, float src_val, void *stream);
+int fd1d_exfield_f64(int t, int nx, const double *ca, const double *cb,
+                     double *ex, const double *hy,
+                     int src_mode, int src_pos, double src_val, void *stream);
+
+/* Two-step absorbing boundary (bc[4] state) then H update for i in [0,nx-1).
+ * Replaces `hyfield` of fd1d/pycuda/fd1d_1_2.py:53-61; bc NULL gives the
+ * boundary-free form of fd1d/pycuda/fd1d_1_1.py:53-58.  nx >= 3.            */
+int fd1d_hyfield(int nx, float *ex, float *hy, float *bc, void *stream);
+int fd1d_hyfield_f64(int nx, double *ex, double *hy, double *bc, void *stream);
+
+/* nsteps steps of exfield+hyfield in ONE launch (single workgroup, fields in
+ * LDS), for t = t0+1 .. t0+nsteps.  Replaces the driver loop of
+ * fd1d/pycuda/fd1d_1_2.py:87-89 (2 launches per step there).
+ * src_wave: device, src_wave[s] = source value of step t0+1+s (NULL: none).
+ * probe_k: host array of nprobe cell indices; probe_out: device,
+ * probe_out[p*nsteps + s] = ex[probe_k[p]] after step t0+1+s.
+ * Requires nx <= fd1d_run_max_nx(sizeof element).                           */
+int fd1d_run(int nx, int t0, int nsteps, const float *ca, const float *cb,
+             float *ex, float *hy, float *bc,
+             int src_mode, int src_pos, const float *src_wave,
+             int nprobe, const int *probe_k, float *probe_out, void *stream);
+int fd1d_run_f64(int nx, int t0, int nsteps, const double *ca, const double *cb,
+                 double *ex, double *hy, double *bc,
+                 int src_mode, int src_pos, const double *src_wave,
+                 int nprobe, const int *probe_k, double *probe_out, void *stream);
+int fd1d_run_max_nx(int elem_size);
+
+/* ca, cb of a lossy half space i >= nx/2, computed on the device in FP64 and
+ * stored FP32, as fd1d/pycuda/fd1d_1_5.py:71-78 (host FP64 there).          */
+int fd1d_dielectric(int nx, double dt, double epsr, double sigma,
+                    float *ca, float *cb, void *stream);
+/* The same coefficients stored FP64, for the FP64 runs of fd1d/clang/fd1d_1_5.c:44-55. */
+int fd1d_dielectric_f64(int nx, double dt, double epsr, double sigma,
+                        double *ca, double *cb, void *stream);
+
+/* ------------------------------------------------------------- 2-D */
+
+/* D update for i in [1,nx), j in [1,ny), then hard sources.  Replaces
+ * `dfield(int t,int nx,int ny,pmlayer *pml,float *dz,float *hx,float *hy)`
+ * of fd2d/pycuda/fd2d_3_2.py:219-233; pml NULL gives the free-space form of
+ * fd2d/pycuda/fd2d_3_1.py:58-70.  Sources: nsrc <= FDTD_MAX_SOURCES host
+ * arrays of cell (src_i, src_j) and value; dz[cell] = value after the update
+ * (later sources win), applied by the thread that owns the cell.            */
+int fd2d_dfield(int t, int nx, int ny, const fdtd_pml *pml,
+                float *dz, const float *hx, const float *hy,
+                int nsrc, const int *src_i, const int *src_j, const float *src_val,
+                void *stream);
+
+/* E from D over all cells.  Replaces `efield(int nx,int ny,medium *md,float
+ * *dz,float *iz,float *ez)` of fd2d/pycuda/fd2d_3_4.py:175-185; md->nbz NULL
+ * gives `ez = naz*dz` of fd2d/pycuda/fd2d_3_1.py:73-82 (iz unused).          */
+int fd2d_efield(int nx, int ny, const fdtd_medium *md,
+                const float *dz, float *iz, float *ez, void *stream);
+
+/* H update for i in [0,nx-1), j in [0,ny-1).  Replaces `hfield(int nx,int
+ * ny,pmlayer *pml,float *ez,float *ihx,float *ihy,float *hx,float *hy)` of
+ * fd2d/pycuda/fd2d_3_2.py:248-260; pml NULL gives the free-space form of
+ * fd2d/pycuda/fd2d_3_1.py:85-95 (ihx, ihy unused).                          */
+int fd2d_hfield(int nx, int ny, const fdtd_pml *pml, const float *ez,
+                float *ihx, float *ihy, float *hx, float *hy, void *stream);
+
+/* Sullivan PML profiles computed on the device in FP64, stored FP32, over the
+ * default fills (f*1 = 0, others 1).  Replaces `pmlparam` of
+ * fd2d/pycuda/fd2d_3_2.py:264-273 (host FP64, 20*npml scalar copies there)
+ * and the fills of :293-304.  Requires 0 <= npml <= min(nx,ny) - 2.        */
+int fd2d_pmlparam(int nx, int ny, int npml, const fdtd_pml *pml, void *stream);
+
+/* Lossy dielectric cylinders by 3x3 sub-cell sampling, on the device in FP64,
+ * stored FP32.  Replaces the Numba `dielectric` of
+ * fd2d/pycuda/fd2d_3_4.py:233-254 (one cylinder centred at (nx/2-1, ny/2-1):
+ * cyl = {nx/2.0-1, ny/2.0-1, rgrid, epsr, sigma}).  cyl: host array of ncyl
+ * records {cx, cy, radius, epsr, sigma}; a sub-sample takes the first
+ * cylinder that contains it; cells outside [npml, n-npml) get naz=1, nbz=0.
+ * int_sub != 0 selects the integer sub-sampling of fd2d/clang/fd2d_3_4.c:
+ * 149-176 (cx, cy ignored).                                                 */
+int fd2d_dielectric(int nx, int ny, int npml, double dt,
+                    int ncyl, const double *cyl, int int_sub,
+                    float *naz, float *nbz, void *stream);
+
+/* Sum over the window [i0,i1) x [j0,j1) of ez^2 + hx^2 + hy^2, accumulated
+ * in FP64 in a fixed order (deterministic).  work: device, FDTD_ENERGY_WORK
+ * doubles; the result is left in work[0].  If result != NULL the call
+ * synchronises `stream` and stores the value there.  Not a reference
+ * function: the reference has no energy or instability accounting.         */
+int fd2d_energy(int nx, int ny, int i0, int i1, int j0, int j1,
+                const float *ez, const float *hx, const float *hy,
+                double *work, double *result, void *stream);
+
+/* Fused, temporally blocked TM time stepping.  One call advances the state
+ * by nsteps leapfrog steps t = t0+1 .. t0+nsteps with the serial semantics
+ * dfield -> hard sources -> efield -> hfield of fd2d/pycuda/fd2d_3_4.py:
+ * 364-373 (config-2 form, fd2d_3_2.py dfield/hfield + fd2d_3_4.py efield),
+ * i.e. the same values as that many fd2d_dfield/efield/hfield calls.
+ * Each launch fuses `depth` steps; launches alternate between the caller's
+ * state set and the partner set (dz2 ...), and the call always ends with the
+ * result in the caller's set.  ez is written for the final step only.
+ *   src_wave: device [nsrc][wave_len]; the value of step t for source k is
+ *             src_wave[k*wave_len + t-1] (t0+nsteps <= wave_len).
+ *   probe_out: device [nprobe][probe_len]; ez at probe p after step t goes
+ *             to probe_out[p*probe_len + t-1] (t0+nsteps <= probe_len).
+ *   md.nbz NULL: lossless (ez = naz*dz, iz untouched); pml all NULL: free
+ *   space (ihx, ihy untouched).
+ *   depth/rows/cols: temporal depth, rows per block and columns per lane;
+ *   0 selects the built-in defaults (see fd2d_tm_plan).                    */
+typedef struct fd2d_tm_desc {
+    int nx, ny;
+    float *dz, *iz, *ez, *hx, *hy, *ihx, *ihy;
+    float *dz2, *iz2, *hx2, *hy2, *ihx2, *ihy2;
+    fdtd_medium md;
+    fdtd_pml pml;
+    int nsrc;
+    int src_i[FDTD_MAX_SOURCES], src_j[FDTD_MAX_SOURCES];
+    const float *src_wave;
+    int wave_len;
+    int nprobe;
+    int probe_i[FDTD_MAX_PROBES], probe_j[FDTD_MAX_PROBES];
+    float *probe_out;
+    int probe_len;
+    int depth, rows, cols;
+} fd2d_tm_desc;
+
+int fd2d_tm_run(const fd2d_tm_desc *d, int t0, int nsteps, void *stream);
+
+/* The launch plan fd2d_tm_run would use for this descriptor and nsteps:
+ * number of launches and the chosen depth/rows/cols.  Host only.            */
+int fd2d_tm_plan(const fd2d_tm_desc *d, int nsteps,
+                 int *launches, int *depth, int *rows, int *cols);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FDTD_H */

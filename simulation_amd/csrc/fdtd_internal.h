@@ -1,0 +1,106 @@
+// fdtd_internal.h -- shared declarations of libfdtd_hip.so (not installed).
+//
+// Kernel argument structs, the per-thread error message and the launch
+// helpers used by the kernel translation units and the C-ABI layer.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+
+#include "../../include/fdtd.h"
+
+namespace fdtd {
+
+// ------------------------------------------------------------ errors
+// Records a message for fdtd_last_error() and returns FDTD_EINVAL.
+int fail(const char *fmt, ...) __attribute__((format(printf, 1, 2)));
+// Checks a HIP status; on failure records "<what>: <hip message>" and
+// returns the positive hipError_t.
+int hip_check(hipError_t e, const char *what);
+// Clears the per-thread message (called at the start of every entry point).
+void clear_error();
+
+// ------------------------------------------------------ kernel arguments
+
+struct PmlPtrs {
+    const float *fx1, *fx2, *fx3;
+    const float *fy1, *fy2, *fy3;
+    const float *gx2, *gx3;
+    const float *gy2, *gy3;
+};
+
+struct SrcList {                   // hard sources of one step
+    int n;
+    int i[FDTD_MAX_SOURCES];
+    int j[FDTD_MAX_SOURCES];
+    float v[FDTD_MAX_SOURCES];     // values, used when wave is NULL
+    const float *wave;             // device [n][wave_len]; value = wave[k*wave_len + t-1]
+    int wave_len;
+    int t;
+};
+
+struct CylList {
+    int n;
+    double cx[FDTD_MAX_CYLINDERS], cy[FDTD_MAX_CYLINDERS], rad[FDTD_MAX_CYLINDERS];
+    double epsr[FDTD_MAX_CYLINDERS], sigma[FDTD_MAX_CYLINDERS];
+};
+
+// Arguments of one temporally blocked launch (fd2d_tb.hip).
+struct TbParams {
+    int nx, ny;
+    const float *dz_i, *iz_i, *hx_i, *hy_i, *ihx_i, *ihy_i;
+    float *dz_o, *iz_o, *hx_o, *hy_o, *ihx_o, *ihy_o, *ez_o;
+    const float *naz, *nbz;
+    PmlPtrs pml;
+    int halo;                      // column halo of the strip geometry (>= depth)
+    int rows;                      // rows per block
+    int t_first;                   // time index of the first fused step
+    int nsrc;
+    int src_i[FDTD_MAX_SOURCES], src_j[FDTD_MAX_SOURCES];
+    const float *src_wave;
+    int wave_len;
+    int nprobe;
+    int probe_i[FDTD_MAX_PROBES], probe_j[FDTD_MAX_PROBES];
+    float *probe_out;
+    int probe_len;
+};
+
+// ----------------------------------------------------- launch wrappers
+// (defined in the kernel translation units; return 0 or a hip status)
+
+template <typename T>
+int launch_fd1d_exfield(int nx, const T *ca, const T *cb, T *ex, const T *hy,
+                        int src_mode, int src_pos, T src_val, hipStream_t s);
+template <typename T>
+int launch_fd1d_hyfield(int nx, T *ex, T *hy, T *bc, hipStream_t s);
+template <typename T>
+int launch_fd1d_run(int nx, int nsteps, const T *ca, const T *cb, T *ex, T *hy, T *bc,
+                    int src_mode, int src_pos, const T *src_wave,
+                    int nprobe, const int *probe_k, T *probe_out, hipStream_t s);
+int fd1d_run_lds_limit();          // bytes of LDS the single-workgroup run may use
+template <typename T>
+int launch_fd1d_dielectric(int nx, double dt, double epsr, double sigma, T *ca, T *cb,
+                           hipStream_t s);
+
+int launch_fd2d_dfield(int nx, int ny, const PmlPtrs *pml, float *dz, const float *hx,
+                       const float *hy, const SrcList &src, hipStream_t s);
+int launch_fd2d_efield(int nx, int ny, const float *naz, const float *nbz,
+                       const float *dz, float *iz, float *ez, hipStream_t s);
+int launch_fd2d_hfield(int nx, int ny, const PmlPtrs *pml, const float *ez,
+                       float *ihx, float *ihy, float *hx, float *hy, hipStream_t s);
+int launch_fd2d_probe(int ny, const float *ez, int nprobe, const int *pi, const int *pj,
+                      float *out, int stride, int col, hipStream_t s);
+int launch_fd2d_pmlparam(int nx, int ny, int npml, const fdtd_pml &pml, hipStream_t s);
+int launch_fd2d_dielectric(int nx, int ny, int npml, double dt, const CylList &cyl,
+                           int int_sub, float *naz, float *nbz, hipStream_t s);
+int launch_fd2d_energy(int nx, int ny, int i0, int i1, int j0, int j1, const float *ez,
+                       const float *hx, const float *hy, double *work, hipStream_t s);
+
+// Temporally blocked TM launch: depth in [1, tb_max_depth()], cols in {1,2,4}.
+int tb_max_depth();
+bool tb_supported(int depth, int cols);
+int launch_fd2d_tb(const TbParams &p, int depth, int cols, bool lossy, bool pml,
+                   hipStream_t s);
+
+}  // namespace fdtd

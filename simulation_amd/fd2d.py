@@ -1,0 +1,292 @@
+"""2-D TM FDTD on MI355X -- host mirror of the PyCUDA 2-D programs.
+
+Names, argument order and data layout follow fd2d/pycuda/fd2d_3_x.py:
+``pmlayer`` / ``medium`` namedtuples of device arrays, ``pmlparam`` and
+``dielectric`` for the coefficients, ``dfield``/``efield``/``hfield`` for one
+step, and row-major ``nx*ny`` fields with ``n = i*ny + j``.  Every function
+calls a HIP kernel of libfdtd_hip.so through ctypes; PyTorch only owns the
+device buffers.
+
+``TMSolver`` is the fused path: it advances the state by many steps per call
+through ``fd2d_tm_run`` (temporally blocked kernels, several leapfrog steps
+per launch), with hard point sources, probe traces and snapshots, and gives
+the same values as calling dfield -> source -> efield -> hfield per step.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from collections import namedtuple
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._dev import fptr, ptr, require_gpu, stream
+from ._lib import call
+
+pmlayer = namedtuple("pmlayer", ("fx1", "fx2", "fx3", "fy1", "fy2", "fy3",
+                                 "gx2", "gx3", "gy2", "gy3"))
+medium = namedtuple("medium", ("naz", "nbz"))
+
+__all__ = ["pmlayer", "medium", "new_pmlayer", "pmlparam", "dielectric", "dfield", "efield",
+           "hfield", "energy", "TMSolver", "TMConfig", "simulate", "c2_config", "time_step"]
+
+
+def time_step(ds: float = 0.01) -> float:
+    """dt = np.float32(ds/6e8) with ds = np.float32(0.01) (fd2d/pycuda/fd2d_3_4.py:313-314)."""
+    return float(np.float32(np.float32(ds) / np.float32(6e8)))
+
+
+def _pml_struct(pml: pmlayer | None):
+    if pml is None:
+        return None
+    return C.byref(_lib.FdtdPml(*[fptr(getattr(pml, k)) for k in pmlayer._fields]))
+
+
+def new_pmlayer(nx: int, ny: int, device=None) -> pmlayer:
+    """The default fills of fd2d/pycuda/fd2d_3_2.py:293-304 (f*1 = 0, the rest 1)."""
+    dev = device or require_gpu()
+    mk = {"x": nx, "y": ny}
+    return pmlayer(**{k: (torch.zeros if k[2] == "1" else torch.ones)(
+        mk[k[1]], dtype=torch.float32, device=dev) for k in pmlayer._fields})
+
+
+def pmlparam(nx: int, ny: int, npml: int, pml: pmlayer) -> None:
+    """Sullivan PML profiles (fd2d/pycuda/fd2d_3_2.py:264-273) on the device in
+    FP64, stored FP32; also writes the default values outside the layer."""
+    call("fd2d_pmlparam", int(nx), int(ny), int(npml), _pml_struct(pml), stream())
+
+
+def dielectric(nx: int, ny: int, npml: int, rgrid: float, dt: float, epsr: float,
+               sigma: float, device=None, cylinders=None, int_sub: bool = False) -> medium:
+    """naz, nbz of a lossy cylinder (fd2d/pycuda/fd2d_3_4.py:233-254), on the
+    device in FP64, stored FP32.  Default: one cylinder of radius rgrid at
+    (nx/2-1, ny/2-1); ``cylinders`` = [(cx, cy, radius, epsr, sigma), ...]
+    overrides it."""
+    dev = device or require_gpu()
+    if cylinders is None:
+        cylinders = [(nx / 2 - 1, ny / 2 - 1, rgrid, epsr, sigma)]
+    cyl = np.ascontiguousarray(np.asarray(cylinders, dtype=np.float64).reshape(-1, 5))
+    if len(cyl) > _lib.MAX_CYLINDERS:
+        raise ValueError(f"at most {_lib.MAX_CYLINDERS} cylinders")
+    naz = torch.empty((nx, ny), dtype=torch.float32, device=dev)
+    nbz = torch.empty((nx, ny), dtype=torch.float32, device=dev)
+    call("fd2d_dielectric", int(nx), int(ny), int(npml), float(dt), len(cyl),
+         cyl.ctypes.data_as(_lib._DP), 1 if int_sub else 0, ptr(naz), ptr(nbz), stream())
+    return medium(naz, nbz)
+
+
+def dfield(t: int, nx: int, ny: int, pml: pmlayer | None, dz, hx, hy, sources=()) -> None:
+    """D update + hard sources [(i, j, value)] (fd2d/pycuda/fd2d_3_2.py:219-233;
+    pml None = free space, fd2d/pycuda/fd2d_3_1.py:58-70)."""
+    n = len(sources)
+    si = (C.c_int * max(n, 1))(*[int(s[0]) for s in sources])
+    sj = (C.c_int * max(n, 1))(*[int(s[1]) for s in sources])
+    sv = (C.c_float * max(n, 1))(*[float(s[2]) for s in sources])
+    call("fd2d_dfield", int(t), int(nx), int(ny), _pml_struct(pml), ptr(dz), ptr(hx), ptr(hy),
+         n, si, sj, sv, stream())
+
+
+def efield(nx: int, ny: int, md: medium, dz, iz, ez) -> None:
+    """E from D (fd2d/pycuda/fd2d_3_4.py:175-185; md.nbz None = fd2d_3_1.py:73-82)."""
+    m = _lib.FdtdMedium(fptr(md.naz), fptr(md.nbz))
+    call("fd2d_efield", int(nx), int(ny), C.byref(m), ptr(dz), ptr(iz), ptr(ez), stream())
+
+
+def hfield(nx: int, ny: int, pml: pmlayer | None, ez, ihx, ihy, hx, hy) -> None:
+    """H update (fd2d/pycuda/fd2d_3_2.py:248-260; pml None = fd2d_3_1.py:85-95)."""
+    call("fd2d_hfield", int(nx), int(ny), _pml_struct(pml), ptr(ez), ptr(ihx), ptr(ihy),
+         ptr(hx), ptr(hy), stream())
+
+
+def energy(ez, hx, hy, window=None, work: torch.Tensor | None = None) -> float:
+    """FP64 sum of ez^2 + hx^2 + hy^2 over window (i0, i1, j0, j1) (default all)."""
+    nx, ny = ez.shape
+    i0, i1, j0, j1 = window if window is not None else (0, nx, 0, ny)
+    if work is None:
+        work = torch.empty(_lib.ENERGY_WORK, dtype=torch.float64, device=ez.device)
+    out = C.c_double(0.0)
+    call("fd2d_energy", nx, ny, int(i0), int(i1), int(j0), int(j1), ptr(ez), ptr(hx), ptr(hy),
+         ptr(work, torch.float64), C.byref(out), stream())
+    return out.value
+
+
+STATE = ("dz", "iz", "ez", "hx", "hy", "ihx", "ihy")
+
+
+class TMSolver:
+    """Fused 2-D TM time stepping on one grid (fd2d_tm_run).
+
+    pml: pmlayer (or None for free space); md: medium (md.nbz None = lossless);
+    sources: [(i, j, wave)] with wave a host array of the hard-source values of
+    steps 1..wave_len; probes: [(i, j)] recorded every step up to probe_len.
+    The state arrays follow fd2d/pycuda/fd2d_3_4.py:279-286; a partner set of
+    the same shapes is allocated for the ping-pong launches.
+    """
+
+    def __init__(self, nx, ny, pml, md, sources=(), probes=(), nsteps_max=None,
+                 depth=0, rows=0, cols=0, device=None):
+        dev = device or require_gpu()
+        self.nx, self.ny, self.device = int(nx), int(ny), dev
+        self.pml, self.md = pml, md
+        self.lossy = md.nbz is not None
+        z = lambda: torch.zeros((nx, ny), dtype=torch.float32, device=dev)  # noqa: E731
+        self.dz, self.ez, self.hx, self.hy = z(), z(), z(), z()
+        self.iz = z() if self.lossy else None
+        self.ihx = z() if pml is not None else None
+        self.ihy = z() if pml is not None else None
+        self.dz2, self.hx2, self.hy2 = z(), z(), z()
+        self.iz2 = z() if self.lossy else None
+        self.ihx2 = z() if pml is not None else None
+        self.ihy2 = z() if pml is not None else None
+        if len(sources) > _lib.MAX_SOURCES or len(probes) > _lib.MAX_PROBES:
+            raise ValueError("too many sources or probes")
+        self.sources = [(int(i), int(j)) for i, j, _ in sources]
+        if sources:
+            wl = min(len(w) for _, _, w in sources)
+            if nsteps_max is not None:
+                wl = min(wl, nsteps_max)
+            waves = np.stack([np.asarray(w[:wl], dtype=np.float32) for _, _, w in sources])
+            self.src_wave = torch.from_numpy(np.ascontiguousarray(waves)).to(dev)
+            self.wave_len = wl
+        else:
+            self.src_wave, self.wave_len = None, 0
+        self.probes = [(int(i), int(j)) for i, j in probes]
+        self.probe_len = int(nsteps_max or self.wave_len or 0)
+        self.probe_out = (torch.zeros((len(probes), self.probe_len), dtype=torch.float32, device=dev)
+                          if probes else None)
+        self.depth, self.rows, self.cols = depth, rows, cols
+        self.t = 0
+        self._work = torch.empty(_lib.ENERGY_WORK, dtype=torch.float64, device=dev)
+
+    def desc(self) -> _lib.Fd2dTmDesc:
+        d = _lib.Fd2dTmDesc()
+        d.nx, d.ny = self.nx, self.ny
+        for k in STATE:
+            setattr(d, k, fptr(getattr(self, k)))
+        for k in ("dz2", "iz2", "hx2", "hy2", "ihx2", "ihy2"):
+            setattr(d, k, fptr(getattr(self, k)))
+        d.md = _lib.FdtdMedium(fptr(self.md.naz), fptr(self.md.nbz))
+        if self.pml is not None:
+            d.pml = _lib.FdtdPml(*[fptr(getattr(self.pml, k)) for k in pmlayer._fields])
+        d.nsrc = len(self.sources)
+        for k, (i, j) in enumerate(self.sources):
+            d.src_i[k], d.src_j[k] = i, j
+        d.src_wave = fptr(self.src_wave)
+        d.wave_len = self.wave_len
+        d.nprobe = len(self.probes)
+        for k, (i, j) in enumerate(self.probes):
+            d.probe_i[k], d.probe_j[k] = i, j
+        d.probe_out = fptr(self.probe_out)
+        d.probe_len = self.probe_len
+        d.depth, d.rows, d.cols = self.depth, self.rows, self.cols
+        return d
+
+    def plan(self, nsteps: int) -> dict:
+        d = self.desc()
+        out = [C.c_int() for _ in range(4)]
+        call("fd2d_tm_plan", C.byref(d), int(nsteps), *[C.byref(o) for o in out])
+        return dict(launches=out[0].value, depth=out[1].value, rows=out[2].value,
+                    cols=out[3].value)
+
+    def run(self, nsteps: int) -> None:
+        """Advance by nsteps (asynchronous on the current stream)."""
+        d = self.desc()
+        call("fd2d_tm_run", C.byref(d), int(self.t), int(nsteps), stream())
+        self.t += int(nsteps)
+
+    def step_unfused(self) -> None:
+        """One step through the per-step kernels (reference launch order)."""
+        t = self.t + 1
+        src = [(i, j, 0.0) for i, j in self.sources]
+        if self.sources:
+            vals = self.src_wave[:, t - 1].cpu().numpy()
+            src = [(i, j, float(v)) for (i, j), v in zip(self.sources, vals)]
+        dfield(t, self.nx, self.ny, self.pml, self.dz, self.hx, self.hy, src)
+        efield(self.nx, self.ny, self.md, self.dz, self.iz, self.ez)
+        hfield(self.nx, self.ny, self.pml, self.ez, self.ihx, self.ihy, self.hx, self.hy)
+        for p, (i, j) in enumerate(self.probes):
+            self.probe_out[p, t - 1] = self.ez[i, j]
+        self.t = t
+
+    def energy(self, window=None) -> float:
+        return energy(self.ez, self.hx, self.hy, window, self._work)
+
+    def state(self) -> dict:
+        torch.cuda.synchronize()
+        return {k: (None if getattr(self, k) is None else getattr(self, k).cpu().numpy())
+                for k in STATE}
+
+    def probe_traces(self) -> np.ndarray | None:
+        if self.probe_out is None:
+            return None
+        torch.cuda.synchronize()
+        return self.probe_out[:, : self.t].cpu().numpy()
+
+
+@dataclass
+class TMConfig:
+    """A 2-D TM run.  cylinders: [(cx, cy, radius, epsr, sigma)];
+    sources: [(i, j, t0, spread)] hard Gaussian point sources."""
+    nx: int
+    ny: int
+    ns: int
+    npml: int = 8
+    cylinders: list = field(default_factory=list)
+    sources: list = field(default_factory=list)
+    probes: list = field(default_factory=list)
+    snap_every: int = 0
+    lossy: bool = True
+    depth: int = 0
+    rows: int = 0
+    cols: int = 0
+
+
+def c2_config(n: int = 4096, ns: int = 2000) -> TMConfig:
+    """The n x n TM configuration: npml=8 Sullivan PML, one cylinder of radius
+    400 cells (eps_r=4, sigma=0.02) at (n/2-1, n/2-1) as
+    fd2d/pycuda/fd2d_3_4.py:319, hard Gaussian (t0=20, spread 6) at (n/2, n/2)
+    as fd2d/pycuda/fd2d_3_1.py:69, snapshots every 500 steps, four probes."""
+    c = n // 2
+    r = 400 if n >= 1024 else max(1, n // 10)
+    return TMConfig(nx=n, ny=n, ns=ns, npml=8,
+                    cylinders=[(n / 2 - 1, n / 2 - 1, r, 4.0, 0.02)],
+                    sources=[(c, c, 20, 6.0)],
+                    probes=[(c, c + n // 20), (c, c + n // 7), (c + n // 7, c), (12, c)],
+                    snap_every=500 if ns >= 500 else 0)
+
+
+def build_solver(cfg: TMConfig, device=None) -> TMSolver:
+    dev = device or require_gpu()
+    pml = None
+    if cfg.npml >= 0:
+        pml = new_pmlayer(cfg.nx, cfg.ny, dev)
+        pmlparam(cfg.nx, cfg.ny, cfg.npml, pml)
+    dt = time_step()
+    md = dielectric(cfg.nx, cfg.ny, max(cfg.npml, 0), 0, dt, 1.0, 0.0, dev,
+                    cylinders=cfg.cylinders or np.zeros((0, 5)))
+    if not cfg.lossy:
+        md = medium(md.naz, None)
+    from .fd1d import waveform
+    srcs = [(i, j, waveform("gaussian", cfg.ns, 0, np.float32, center=t0, spread=sp))
+            for i, j, t0, sp in cfg.sources]
+    return TMSolver(cfg.nx, cfg.ny, pml, md, srcs, cfg.probes, nsteps_max=cfg.ns,
+                    depth=cfg.depth, rows=cfg.rows, cols=cfg.cols, device=dev)
+
+
+def simulate(cfg: TMConfig, device=None, snapshot_fields=("ez", "hy")) -> dict:
+    """Run cfg; returns final state, probe traces and the snapshots."""
+    s = build_solver(cfg, device)
+    snaps = {k: [] for k in snapshot_fields}
+    snap_t = []
+    seg = cfg.snap_every or cfg.ns
+    while s.t < cfg.ns:
+        s.run(min(seg, cfg.ns - s.t))
+        if cfg.snap_every and s.t % cfg.snap_every == 0:
+            for k in snapshot_fields:
+                snaps[k].append(getattr(s, k).cpu().numpy())
+            snap_t.append(s.t)
+    out = dict(state=s.state(), probe=s.probe_traces(), snap_t=snap_t,
+               snapshots={k: np.stack(v) for k, v in snaps.items() if v})
+    return out

@@ -1,0 +1,119 @@
+"""The C-ABI library on the CPU: it loads, exports every symbol include/fdtd.h
+declares, and rejects bad arguments on the host before any device work."""
+import ctypes as C
+import os
+import subprocess
+
+import pytest
+
+from conftest import ROOT
+
+
+@pytest.fixture(scope="module")
+def L():
+    from simulation_amd import _lib
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.fail("libfdtd_hip.so not built (python -m simulation_amd.build)")
+    return _lib
+
+
+def test_header_symbols_are_exported(L):
+    syms = L.header_symbols()
+    assert len(syms) >= 20
+    out = subprocess.run(["nm", "-D", "--defined-only", L.LIB_PATH], capture_output=True,
+                         text=True, check=True).stdout
+    exported = {ln.split()[-1] for ln in out.splitlines() if " T " in ln}
+    missing = [s for s in syms if s not in exported]
+    assert not missing, missing
+    assert set(syms) == set(L.SIGNATURES), set(syms) ^ set(L.SIGNATURES)
+
+
+def test_abi_version_and_host_waveforms(L, oracle):
+    lib = L.lib()
+    assert lib.fdtd_abi_version() == 1
+    for t in (1, 7, 20, 40, 333):
+        assert lib.fdtd_gaussian_f32(t, 20, 6.0) == oracle.gaussian(t, 20, 6.0)
+        assert lib.fdtd_gaussian_f64(t, 40, 12.0) == oracle.gaussian(t, 40, 12.0, "float64")
+        assert lib.fdtd_sinusoidal_f32(t, 0.01, 700e6) == oracle.sinusoidal(t, 0.01, 700e6)
+        assert lib.fdtd_sinusoidal_f64(t, 0.01, 1500e6) == oracle.sinusoidal(t, 0.01, 1500e6, "float64")
+
+
+def _err(L, name, *args):
+    rc = getattr(L.lib(), name)(*args)
+    return rc, L.lib().fdtd_last_error().decode()
+
+
+def test_argument_validation_is_host_side(L):
+    fake = 0x1000  # never dereferenced: every call below is rejected first
+    rc, msg = _err(L, "fd1d_exfield", 1, 1, None, None, fake, fake, 0, 0, 0.0, None)
+    assert rc == L.FDTD_EINVAL and "nx" in msg
+    rc, msg = _err(L, "fd1d_exfield", 1, 10, fake, None, fake, fake, 0, 0, 0.0, None)
+    assert rc == L.FDTD_EINVAL and "cb" in msg
+    rc, msg = _err(L, "fd1d_exfield", 1, 10, None, None, fake, fake, 1, 10, 0.0, None)
+    assert rc == L.FDTD_EINVAL and "src_pos" in msg
+    rc, msg = _err(L, "fd1d_hyfield", 2, fake, fake, None, None)
+    assert rc == L.FDTD_EINVAL
+    big = L.lib().fd1d_run_max_nx(4) + 1
+    rc, msg = _err(L, "fd1d_run", big, 0, 1, None, None, fake, fake, None, 0, 0, None, 0, None,
+                   None, None)
+    assert rc == L.FDTD_EINVAL and "limit" in msg
+    rc, msg = _err(L, "fd2d_dfield", 1, 1, 5, None, fake, fake, fake, 0, None, None, None, None)
+    assert rc == L.FDTD_EINVAL
+    pml = L.FdtdPml()
+    pml.fx1 = C.cast(fake, C.POINTER(C.c_float))
+    rc, msg = _err(L, "fd2d_hfield", 8, 8, C.byref(pml), fake, fake, fake, fake, fake, None)
+    assert rc == L.FDTD_EINVAL and "complete" in msg
+    rc, msg = _err(L, "fd2d_pmlparam", 8, 8, 7, None, None)
+    assert rc == L.FDTD_EINVAL
+    rc, msg = _err(L, "fd2d_tm_run", None, 0, 1, None)
+    assert rc == L.FDTD_EINVAL and "NULL" in msg
+    d = L.Fd2dTmDesc()
+    d.nx, d.ny = 64, 64
+    for k in ("dz", "ez", "hx", "hy", "iz"):
+        setattr(d, k, C.cast(fake, C.POINTER(C.c_float)))
+    d.md.naz = C.cast(fake, C.POINTER(C.c_float))
+    d.nsrc = 1
+    d.src_i[0], d.src_j[0] = 70, 3
+    d.src_wave = C.cast(fake, C.POINTER(C.c_float))
+    d.wave_len = 10
+    rc, msg = _err(L, "fd2d_tm_run", C.byref(d), 0, 5, None)
+    assert rc == L.FDTD_EINVAL and "outside" in msg
+    d.src_i[0] = 3
+    rc, msg = _err(L, "fd2d_tm_run", C.byref(d), 0, 11, None)
+    assert rc == L.FDTD_EINVAL and "wave_len" in msg
+    d.depth = 99
+    rc, msg = _err(L, "fd2d_tm_run", C.byref(d), 0, 5, None)
+    assert rc == L.FDTD_EINVAL and "depth" in msg
+    rc, msg = _err(L, "fd2d_energy", 8, 8, 0, 9, 0, 8, fake, fake, fake, fake, None, None)
+    assert rc == L.FDTD_EINVAL and "window" in msg
+
+
+def test_tm_plan_even_launch_counts(L):
+    """Launch plans always end in the caller's state set: an even number of
+    ping-pong launches, plus one in-place step when that is impossible."""
+    fake = C.cast(0x1000, C.POINTER(C.c_float))
+    d = L.Fd2dTmDesc()
+    d.nx, d.ny = 4096, 4096
+    for k in ("dz", "ez", "hx", "hy"):
+        setattr(d, k, fake)
+    d.md.naz = fake
+    out = [C.c_int() for _ in range(4)]
+    for depth in (1, 2, 5, 8):
+        d.depth = depth
+        for ns in (1, 2, 3, 7, 8, 9, 17, 500, 2000):
+            rc = L.lib().fd2d_tm_plan(C.byref(d), ns, *[C.byref(o) for o in out])
+            assert rc == 0, L.lib().fdtd_last_error()
+            launches = out[0].value
+            assert out[1].value == depth
+            assert launches >= (ns + depth - 1) // depth
+
+
+def test_product_path_has_no_oracle_or_cpu_fallback():
+    """simulation_amd never imports or loads the oracle, and every compute
+    entry point goes through libfdtd_hip.so."""
+    pkg = os.path.join(ROOT, "simulation_amd")
+    for dirpath, _, files in os.walk(pkg):
+        for f in files:
+            if f.endswith((".py", ".hip", ".h", ".cpp")):
+                txt = open(os.path.join(dirpath, f)).read()
+                assert "oracle" not in txt.lower().replace("the oracle", ""), f

@@ -1,0 +1,165 @@
+"""HIP 2-D TM path against the reference's golden fixtures and the oracle.
+
+Bar: bit-exact (np.array_equal) for every state array and probe trace in FP32:
+the kernels evaluate the reference's expressions operand for operand without
+contraction, and the fused/temporally blocked path only reorders which cells
+are computed when.
+"""
+import numpy as np
+import pytest
+import torch
+
+from cases import FD2D_CASES, PML_KEYS, fd2d_case
+
+pytestmark = pytest.mark.gpu
+
+F32_CASES = [c for c in FD2D_CASES if c.endswith("f32")]
+
+
+def _wave(kind, ns, dt, **kw):
+    from simulation_amd import fd1d
+    return fd1d.waveform(kind, ns, 0, dt, **kw)
+
+
+def _t(a, dev):
+    return None if a is None else torch.from_numpy(np.ascontiguousarray(a, dtype=np.float32)).to(dev)
+
+
+def _setup(c, dev):
+    from simulation_amd import fd2d
+    pml = None
+    if c["pml"] is not None:
+        pml = fd2d.pmlayer(**{k: _t(c["pml"][k], dev) for k in PML_KEYS})
+    md = fd2d.medium(_t(c["naz"], dev), _t(c["nbz"], dev))
+    return pml, md
+
+
+def _check(c, state, probe):
+    e = c["exp"]
+    for k in ("dz", "ez", "hx", "hy", "iz", "ihx", "ihy"):
+        if "out_" + k in e and state.get(k) is not None:
+            assert np.array_equal(state[k], e["out_" + k]), k
+    assert np.array_equal(probe, e["out_probe"])
+
+
+@pytest.mark.parametrize("name", F32_CASES)
+def test_fd2d_step_kernels_bit_exact(gpu, name):
+    from simulation_amd import fd2d
+    c = fd2d_case(name, _wave)
+    print(f"[fd2d step] {name} {c['nx']}x{c['ny']} ns={c['ns']}", flush=True)
+    pml, md = _setup(c, gpu)
+    s = fd2d.TMSolver(c["nx"], c["ny"], pml, md, [(*c["sources"][0], c["wave"])], c["probes"],
+                      nsteps_max=c["ns"], device=gpu)
+    for _ in range(c["ns"]):
+        s.step_unfused()
+    _check(c, s.state(), s.probe_traces())
+
+
+FUSED = [(1, 1, 0), (2, 2, 0), (3, 1, 7), (5, 2, 16), (8, 2, 0), (8, 1, 32), (4, 2, 200)]
+
+
+@pytest.mark.parametrize("name", F32_CASES)
+@pytest.mark.parametrize("depth,cols,rows", FUSED)
+def test_fd2d_fused_bit_exact(gpu, name, depth, cols, rows):
+    from simulation_amd import fd2d
+    c = fd2d_case(name, _wave)
+    print(f"[fd2d fused] {name} depth={depth} cols={cols} rows={rows}", flush=True)
+    pml, md = _setup(c, gpu)
+    s = fd2d.TMSolver(c["nx"], c["ny"], pml, md, [(*c["sources"][0], c["wave"])], c["probes"],
+                      nsteps_max=c["ns"], depth=depth, rows=rows, cols=cols, device=gpu)
+    # uneven chunks exercise odd launch counts and the unfused tail step
+    for n in (1, 2, 13, c["ns"] - 16):
+        s.run(n)
+    assert s.t == c["ns"]
+    _check(c, s.state(), s.probe_traces())
+
+
+@pytest.mark.parametrize("nx,ny,npml", [(200, 150, 8), (97, 131, 8), (300, 300, 32),
+                                        (256, 512, 64), (20, 30, 9)])
+def test_pmlparam_device_matches_oracle(gpu, oracle, nx, ny, npml):
+    from simulation_amd import fd2d
+    pml = fd2d.new_pmlayer(nx, ny, gpu)
+    fd2d.pmlparam(nx, ny, npml, pml)
+    ref = oracle.fd2d_pmlparam(nx, ny, npml)
+    for k in PML_KEYS:
+        assert np.array_equal(getattr(pml, k).cpu().numpy(), ref[k].astype(np.float32)), k
+
+
+@pytest.mark.parametrize("int_sub", [False, True])
+def test_dielectric_device_matches_oracle(gpu, oracle, int_sub):
+    from simulation_amd import fd2d
+    nx, ny, npml = 203, 177, 8
+    dt = fd2d.time_step()
+    cyl = [(nx / 2 - 1, ny / 2 - 1, 40, 30.0, 0.3), (40.5, 60.0, 12.5, 4.0, 0.02),
+           (150.0, 30.0, 20.0, 7.5, 0.05)]
+    if int_sub:
+        cyl = cyl[:1]
+    md = fd2d.dielectric(nx, ny, npml, 0, dt, 1, 0, gpu, cylinders=cyl, int_sub=int_sub)
+    naz, nbz = oracle.fd2d_dielectric(nx, ny, npml, dt, cyl, int_sub=int_sub)
+    assert np.array_equal(md.naz.cpu().numpy(), naz.astype(np.float32))
+    assert np.array_equal(md.nbz.cpu().numpy(), nbz.astype(np.float32))
+
+
+def test_energy_matches_oracle(gpu, oracle):
+    from simulation_amd import fd2d
+    g = torch.Generator().manual_seed(7)
+    ez, hx, hy = (torch.randn(333, 517, generator=g) for _ in range(3))
+    e = fd2d.energy(ez.to(gpu), hx.to(gpu), hy.to(gpu))
+    ref = oracle.fd2d_energy(ez.numpy(), hx.numpy(), hy.numpy())
+    assert abs(e - ref) <= 1e-12 * ref
+    w = (10, 300, 5, 400)
+    e = fd2d.energy(ez.to(gpu), hx.to(gpu), hy.to(gpu), window=w)
+    sub = [a.numpy()[10:300, 5:400].copy() for a in (ez, hx, hy)]
+    assert abs(e - oracle.fd2d_energy(*sub)) <= 1e-12 * e
+
+
+@pytest.mark.parametrize("n,ns,depth", [(1024, 300, 8), (640, 257, 6)])
+def test_c2_shape_fused_matches_oracle(gpu, oracle, n, ns, depth):
+    """The n x n configuration-2 shape (PML + cylinder + hard Gaussian + 4
+    probes) through TMSolver, against the C oracle in FP32, bit-exact; and the
+    FP32 error against the FP64 oracle is recorded."""
+    from simulation_amd import fd2d
+    cfg = fd2d.c2_config(n, ns)
+    cfg.depth = depth
+    print(f"[fd2d c2-shape] n={n} ns={ns} depth={depth}", flush=True)
+    s = fd2d.build_solver(cfg, gpu)
+    s.run(ns)
+    st = s.state()
+    pml = {k: getattr(s.pml, k).cpu().numpy() for k in PML_KEYS}
+    naz, nbz = s.md.naz.cpu().numpy(), s.md.nbz.cpu().numpy()
+    w = oracle.waveform("gaussian", 0, ns, np.float32, center=20, spread=6.0)
+    o = oracle.TMState(n, n, np.float32)
+    pr = oracle.fd2d_tm_run(o, 0, ns, pml, naz, nbz, [cfg.sources[0][:2]], w[None, :], cfg.probes)
+    for k in ("dz", "iz", "ez", "hx", "hy", "ihx", "ihy"):
+        assert np.array_equal(st[k], getattr(o, k)), k
+    assert np.array_equal(s.probe_traces(), pr)
+    # FP64 oracle on FP64 coefficients (the PyCUDA set-up before narrowing)
+    p64 = oracle.fd2d_pmlparam(n, n, cfg.npml)
+    n64, b64 = oracle.fd2d_dielectric(n, n, cfg.npml, fd2d.time_step(), cfg.cylinders)
+    o64 = oracle.TMState(n, n, np.float64)
+    w64 = oracle.waveform("gaussian", 0, ns, np.float64, center=20, spread=6.0)
+    pr64 = oracle.fd2d_tm_run(o64, 0, ns, p64, n64, b64, [cfg.sources[0][:2]], w64[None, :],
+                              cfg.probes)
+    rel = np.linalg.norm(st["ez"] - o64.ez) / np.linalg.norm(o64.ez)
+    relp = np.linalg.norm(s.probe_traces() - pr64) / np.linalg.norm(pr64)
+    print(f"[fd2d c2-shape] n={n}: rel-L2 vs FP64 ez={rel:.3e} probes={relp:.3e}")
+    assert rel < 1e-4 and relp < 1e-4
+
+
+def test_fused_is_deterministic_and_matches_unfused(gpu):
+    """Two fused runs agree bit for bit, and with the per-step kernels."""
+    from simulation_amd import fd2d
+    cfg = fd2d.c2_config(512, 120)
+    outs = []
+    for depth in (8, 8, 3):
+        cfg.depth = depth
+        s = fd2d.build_solver(cfg, gpu)
+        s.run(120)
+        outs.append(s.state())
+    s = fd2d.build_solver(cfg, gpu)
+    for _ in range(120):
+        s.step_unfused()
+    outs.append(s.state())
+    for o in outs[1:]:
+        for k in ("dz", "ez", "hx", "hy", "iz", "ihx", "ihy"):
+            assert np.array_equal(o[k], outs[0][k]), k

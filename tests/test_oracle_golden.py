@@ -1,0 +1,88 @@
+"""The CPU oracle against the golden fixtures written by the reference's own C
+programs (tests/golden/make_golden.py): bit-exact, FP32 and FP64."""
+import numpy as np
+import pytest
+
+from cases import FD1D_CASES, FD2D_CASES, fd1d_case, fd2d_case
+
+
+def _wave(O):
+    return lambda kind, ns, dt, **kw: O.waveform(kind, 0, ns, dt, **kw)
+
+
+@pytest.mark.parametrize("name", FD1D_CASES)
+def test_fd1d_oracle_matches_reference(oracle, name):
+    c = fd1d_case(name, _wave(oracle))
+    dt, nx = c["dtype"], c["nx"]
+    ex = np.zeros(nx, dt)
+    hy = np.zeros(nx, dt)
+    bc = np.zeros(4, dt) if c["abc"] else None
+    ca = None if c["ca"] is None else np.ascontiguousarray(c["ca"])
+    cb = None if c["cb"] is None else np.ascontiguousarray(c["cb"])
+    snaps_ex, snaps_hy = [], []
+    probes = []
+    t = 0
+    cuts = c["snap_t"] or [c["ns"]]
+    for cut in cuts:
+        n = cut - t
+        probes.append(oracle.fd1d_run(nx, t, n, ex, hy, bc, ca, cb, c["src_mode"], c["src_pos"],
+                                      np.ascontiguousarray(c["wave"][t:cut]), c["probes"]))
+        t = cut
+        snaps_ex.append(ex.copy())
+        snaps_hy.append(hy.copy())
+    e = c["exp"]
+    assert np.array_equal(ex, e["out_ex"])
+    assert np.array_equal(hy, e["out_hy"])
+    assert np.array_equal(np.concatenate(probes, axis=1), e["out_probe"])
+    if c["abc"]:
+        assert np.array_equal(bc, e["out_bc"])
+    if c["snap_t"]:
+        assert np.array_equal(np.array(snaps_ex), e["out_snap_ex"])
+        assert np.array_equal(np.array(snaps_hy), e["out_snap_hy"])
+    # the case is not trivially zero: the pulse is on the grid or passed the probe
+    assert max(np.abs(e["out_ex"]).max(), np.abs(e["out_probe"]).max()) > 0.1
+
+
+@pytest.mark.parametrize("name", FD2D_CASES)
+def test_fd2d_oracle_matches_reference(oracle, name):
+    c = fd2d_case(name, _wave(oracle))
+    st = oracle.TMState(c["nx"], c["ny"], c["dtype"])
+    pr = oracle.fd2d_tm_run(st, 0, c["ns"], c["pml"], c["naz"], c["nbz"], c["sources"],
+                            c["wave"][None, :], c["probes"])
+    e = c["exp"]
+    for k in ("dz", "ez", "hx", "hy", "iz", "ihx", "ihy"):
+        if "out_" + k in e:
+            assert np.array_equal(getattr(st, k), e["out_" + k]), k
+    assert np.array_equal(pr, e["out_probe"])
+    assert np.abs(st.ez).max() > 1e-4
+
+
+def test_dielectric_integer_subsampling_matches_reference(oracle):
+    """The accumulation and naz/nbz formulas, pinned bit-exactly by the C
+    program's dielectric (fd2d/clang/fd2d_3_4.c:149-176, integer m/3)."""
+    from conftest import golden
+    m, a = golden("fd2d_dielectric_c")
+    cyl = [(m["nx"] / 2 - 1, m["ny"] / 2 - 1, m["rgrid"], m["epsr"], m["sigma"])]
+    naz, nbz = oracle.fd2d_dielectric(m["nx"], m["ny"], m["npml"], m["dt"], cyl, int_sub=True)
+    assert np.array_equal(naz, a["out_naz"])
+    assert np.array_equal(nbz, a["out_nbz"])
+    assert (a["out_nbz"] > 0).sum() > 100
+
+
+@pytest.mark.parametrize("npml", [8, 32, 64])
+def test_pmlparam_matches_reference_after_fp32_storage(oracle, npml):
+    """The PyCUDA profiles (FP64 pow, stored FP32) equal the C program's FP64
+    profiles (fd2d/clang/fd2d_3_2.c:70-81, a different association of the
+    cube) once both are stored as float32, and differ from them in FP64 by at
+    most 2 ulp; the FP32 program's own float arithmetic (test_3_2.c:70-81)
+    agrees within 2e-7 relative."""
+    from conftest import golden
+    m, a = golden("fd2d_pmlparam_c")
+    p = oracle.fd2d_pmlparam(m["nx"], m["ny"], npml)
+    for k in oracle.PML_FIELDS:
+        ref64 = a[f"out_fd2d_3_2_{npml}_{k}"]
+        ref32 = a[f"out_test_3_2_{npml}_{k}"]
+        assert np.array_equal(p[k].astype(np.float32), ref64.astype(np.float32)), k
+        ulp = np.spacing(np.abs(ref64))
+        assert np.all(np.abs(p[k] - ref64) <= 2 * ulp), k
+        assert np.allclose(p[k].astype(np.float32), ref32, rtol=2e-7, atol=0), k
