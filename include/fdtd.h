@@ -195,7 +195,14 @@ int fd2d_energy(int nx, int ny, int i0, int i1, int j0, int j1,
  *   md.nbz NULL: lossless (ez = naz*dz, iz untouched); pml all NULL: free
  *   space (ihx, ihy untouched).
  *   depth/rows/cols: temporal depth, rows per block and columns per lane;
- *   0 selects the built-in defaults (see fd2d_tm_plan).                    */
+ *   0 selects the built-in defaults (see fd2d_tm_plan).
+ *   band_rows/band_cols: the caller's promise that row indices
+ *   [band_rows[0], band_rows[1]) hold the default profile values
+ *   (fx1 = 0, fx2 = fx3 = gx2 = gx3 = 1) and column indices
+ *   [band_cols[0], band_cols[1]) likewise for fy*, gy* (as fd2d_pmlparam
+ *   leaves [npml, n-1-npml)).  Blocks of the grid wholly inside both bands
+ *   run a body with those values as constants (same results, fewer loads
+ *   and multiplies by one).  An empty range promises nothing.              */
 typedef struct fd2d_tm_desc {
     int nx, ny;
     float *dz, *iz, *ez, *hx, *hy, *ihx, *ihy;
@@ -211,6 +218,7 @@ typedef struct fd2d_tm_desc {
     float *probe_out;
     int probe_len;
     int depth, rows, cols;
+    int band_rows[2], band_cols[2];
 } fd2d_tm_desc;
 
 int fd2d_tm_run(const fd2d_tm_desc *d, int t0, int nsteps, void *stream);

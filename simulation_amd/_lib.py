@@ -59,6 +59,7 @@ class Fd2dTmDesc(C.Structure):
         ("probe_i", C.c_int * MAX_PROBES), ("probe_j", C.c_int * MAX_PROBES),
         ("probe_out", _f), ("probe_len", C.c_int),
         ("depth", C.c_int), ("rows", C.c_int), ("cols", C.c_int),
+        ("band_rows", C.c_int * 2), ("band_cols", C.c_int * 2),
     ]
 
 

@@ -1,0 +1,381 @@
+// fd2d_tb_impl.h -- fused, temporally blocked 2-D TM time stepping for gfx950
+// (template body; instantiated per depth by fd2d_tb_k<depth>.hip).
+//
+// One launch advances the whole grid by K leapfrog steps (K = template
+// depth).  Semantics per step are those of the reference's per-step kernels
+// dfield -> hard source -> efield -> hfield (fd2d/pycuda/fd2d_3_2.py:219-260,
+// fd2d/pycuda/fd2d_3_4.py:175-185), operand for operand, so the result equals
+// K rounds of fd2d_dfield/efield/hfield bit for bit.
+//
+// Decomposition (one wavefront per workgroup, no barriers, no LDS):
+//   * Columns: a wave owns a strip of S = 64*C columns, lane l holding the C
+//     contiguous columns js + l*C .. js + l*C + C-1.  The column left of a
+//     lane's first and right of its last come from lanes l-1 / l+1 through
+//     DPP wave shifts (one VALU op each).  Each fused step widens the
+//     dependency cone by one column on each side, so a strip writes only its
+//     central S - 2*halo columns; neighbouring strips overlap by the halo
+//     (recomputed, never written twice).
+//   * Rows: the wave sweeps down a block of `rows` rows.  Time level s of the
+//     K fused steps trails level s-1 by one row: at sweep position p, level s
+//     computes the E update of row p-s and the H update of row p-s-1, fed by
+//     the H row level s-1 produced in the same iteration and the D row it
+//     produced one iteration earlier.  What a level needs from earlier rows is
+//     carried in registers, so HBM is read once (level 0, row p) and written
+//     once (level K-1) per launch.  A block starts K rows above its first row
+//     (warm-up) and ends K rows below its last (drain).
+//   * Waves whose rows (columns) all lie inside [1, n-2] and inside the band
+//     the caller promises to hold the default PML profile values run a body
+//     in which those values are compile-time constants and boundary tests are
+//     absent; the few boundary waves run the general body.  Multiplying by a
+//     constant 1 is exact, so the compiler may drop it without changing any
+//     result; products with a constant 0 are kept.
+//   * Launches read one state set and write the other (ping-pong), so waves
+//     never read a cell another wave has already advanced.
+//
+// HBM traffic per cell per launch (lossy PML form): reads dz iz hx hy ihx ihy
+// naz nbz (32 B, x halo/warm-up overlap), writes dz iz hx hy ihx ihy (24 B,
+// + 4 B ez on the last launch of a run), for K cell-updates.
+#pragma once
+
+#include "fdtd_internal.h"
+
+namespace fdtd {
+namespace tb {
+
+// lane l receives lane l-1's value (lane 0 receives 0)
+__device__ __forceinline__ float from_left(float v)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v),
+                                                                 0x138 /* wave_shr:1 */, 0xf, 0xf, false));
+}
+// lane l receives lane l+1's value (lane 63 receives 0)
+__device__ __forceinline__ float from_right(float v)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v),
+                                                                 0x130 /* wave_shl:1 */, 0xf, 0xf, false));
+}
+
+template <int C> struct HRow { float hx[C], hy[C], ix[C], iy[C]; };
+template <int C> struct DRow { float dz[C], iz[C], na[C], nb[C]; };
+
+// Registers one time level carries from one sweep iteration to the next.
+template <int C>
+struct Carry {
+    HRow<C> h;        // H input of this level at its E row (next iteration: the row above)
+    float e[C];       // ez of this level at its E row
+    float er;         // the same, column right of the lane's last
+    DRow<C> d;        // D/E state of this level at its E row (input of the next level)
+};
+
+struct Geom {
+    int j0;           // first column of the lane
+    int i0, i1;       // owned rows [i0, i1)
+    int vj0, vj1;     // owned columns [vj0, vj1)
+};
+
+// Load the lane's C columns of a row.  VEC: one vector load (every lane's
+// columns are wholly inside or outside the grid); CLAMP: columns outside the
+// grid read an in-bounds neighbour (their values are never used).
+template <int C, bool VEC, bool CLAMP>
+__device__ __forceinline__ void ld(const float *__restrict__ row, int j0, int ny, float (&v)[C])
+{
+    if (VEC) {
+        const int jc = CLAMP ? min(max(j0, 0), ny - C) : j0;
+        if constexpr (C == 2) {
+            const float2 q = *reinterpret_cast<const float2 *>(row + jc);
+            v[0] = q.x; v[1] = q.y;
+        } else {
+#pragma unroll
+            for (int c = 0; c < C; ++c) v[c] = row[jc + c];
+        }
+    } else {
+#pragma unroll
+        for (int c = 0; c < C; ++c) v[c] = row[CLAMP ? min(max(j0 + c, 0), ny - 1) : j0 + c];
+    }
+}
+
+template <int C, bool VEC>
+__device__ __forceinline__ void st(float *__restrict__ row, int j0, int vj0, int vj1,
+                                   const float (&v)[C])
+{
+    if (VEC) {
+        if (j0 >= vj0 && j0 + C <= vj1) {
+            if constexpr (C == 2) {
+                *reinterpret_cast<float2 *>(row + j0) = make_float2(v[0], v[1]);
+            } else {
+#pragma unroll
+                for (int c = 0; c < C; ++c) row[j0 + c] = v[c];
+            }
+        }
+    } else {
+#pragma unroll
+        for (int c = 0; c < C; ++c)
+            if (j0 + c >= vj0 && j0 + c < vj1) row[j0 + c] = v[c];
+    }
+}
+
+template <int C, bool LOSSY, bool PML, bool VEC, bool CLAMP>
+__device__ __forceinline__ void load_row(const TbParams &P, int p, int j0, HRow<C> &H, DRow<C> &D)
+{
+    const int pc = min(max(p, 0), P.nx - 1);     // rows outside the grid: any in-bounds row
+    const size_t base = (size_t)pc * P.ny;
+    ld<C, VEC, CLAMP>(P.dz_i + base, j0, P.ny, D.dz);
+    ld<C, VEC, CLAMP>(P.naz + base, j0, P.ny, D.na);
+    ld<C, VEC, CLAMP>(P.hx_i + base, j0, P.ny, H.hx);
+    ld<C, VEC, CLAMP>(P.hy_i + base, j0, P.ny, H.hy);
+    if (LOSSY) {
+        ld<C, VEC, CLAMP>(P.iz_i + base, j0, P.ny, D.iz);
+        ld<C, VEC, CLAMP>(P.nbz + base, j0, P.ny, D.nb);
+    } else {
+#pragma unroll
+        for (int c = 0; c < C; ++c) { D.iz[c] = 0.f; D.nb[c] = 0.f; }
+    }
+    if (PML) {
+        ld<C, VEC, CLAMP>(P.ihx_i + base, j0, P.ny, H.ix);
+        ld<C, VEC, CLAMP>(P.ihy_i + base, j0, P.ny, H.iy);
+    } else {
+#pragma unroll
+        for (int c = 0; c < C; ++c) { H.ix[c] = 0.f; H.iy[c] = 0.f; }
+    }
+}
+
+// Column profile values of the lane (constant along the sweep).  In interior
+// strips they are the defaults, as compile-time constants.
+template <int C>
+struct ColCoef { float gy2[C], gy3[C], fy1[C], fy2[C], fy3[C]; };
+
+template <int C, bool PML, bool BCOL>
+__device__ __forceinline__ void col_coef(const TbParams &P, int j0, ColCoef<C> &cc)
+{
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+        cc.gy2[c] = 1.f; cc.gy3[c] = 1.f; cc.fy1[c] = 0.f; cc.fy2[c] = 1.f; cc.fy3[c] = 1.f;
+        if (PML && BCOL) {
+            const int j = min(max(j0 + c, 0), P.ny - 1);
+            cc.gy2[c] = P.pml.gy2[j]; cc.gy3[c] = P.pml.gy3[j];
+            cc.fy1[c] = P.pml.fy1[j]; cc.fy2[c] = P.pml.fy2[j]; cc.fy3[c] = P.pml.fy3[j];
+        }
+    }
+}
+
+// One sweep iteration at position p: level s computes the E update of row
+// p-s and the H update of row p-s-1.  BROW: rows may lie outside [1, nx-2] or
+// outside the default band; BCOL: the same for the strip's columns.
+template <int K, int C, bool LOSSY, bool PML, bool VEC, bool BROW, bool BCOL>
+__device__ __forceinline__ void iterate(const TbParams &P, const Geom &g, const ColCoef<C> &cc,
+                                        int p, const HRow<C> &Hload, const DRow<C> &Dload,
+                                        Carry<C> (&St)[K])
+{
+    const int nx = P.nx, ny = P.ny;
+    HRow<C> Hin = Hload;
+    DRow<C> Dpend;                        // this iteration's D output of the previous level
+#pragma unroll
+    for (int s = 0; s < K; ++s) {
+        const int r = p - s;              // E row of this level
+        const int q = r - 1;              // H row
+        const int t = P.t_first + s;      // time index of this level
+        // D input: the previous level's output of the last iteration, which
+        // is then replaced by its output of this iteration
+        DRow<C> Din;
+        if (s == 0) {
+            Din = Dload;
+        } else {
+            Din = St[s - 1].d;
+            St[s - 1].d = Dpend;
+        }
+        const HRow<C> Hp = St[s].h;
+        float gx2r = 1.f, gx3r = 1.f, fx1q = 0.f, fx2q = 1.f, fx3q = 1.f;
+        if (PML && BROW) {
+            const int rc = min(max(r, 0), nx - 1), qc = min(max(q, 0), nx - 1);
+            gx2r = P.pml.gx2[rc]; gx3r = P.pml.gx3[rc];
+            fx1q = P.pml.fx1[qc]; fx2q = P.pml.fx2[qc]; fx3q = P.pml.fx3[qc];
+        }
+        // ------------------------------------------------ E update, row r
+        const float hxl = from_left(Hin.hx[C - 1]);
+        DRow<C> Dn;
+        float En[C];
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+            const float hL = (c == 0) ? hxl : Hin.hx[c - 1];
+            const float curl = Hin.hy[c] - Hp.hy[c] - Hin.hx[c] + hL;
+            float dz;
+            if (PML) dz = gx3r * cc.gy3[c] * Din.dz[c] + gx2r * cc.gy2[c] * 0.5f * curl;
+            else dz = Din.dz[c] + 0.5f * curl;
+            if (BCOL) dz = (g.j0 + c >= 1) ? dz : Din.dz[c];     // column 0: no update
+            if (BROW) dz = (r >= 1) ? dz : Din.dz[c];            // row 0: no update
+            Dn.dz[c] = dz;
+        }
+        for (int k = 0; k < P.nsrc; ++k) {
+            if (r == P.src_i[k]) {
+                const float v = P.src_wave[(size_t)k * P.wave_len + (t - 1)];
+#pragma unroll
+                for (int c = 0; c < C; ++c)
+                    if (g.j0 + c == P.src_j[k]) Dn.dz[c] = v;
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+            const float dz = Dn.dz[c];
+            float iz = Din.iz[c], ez;
+            if (LOSSY) {
+                ez = Din.na[c] * (dz - iz);
+                iz = iz + Din.nb[c] * ez;
+            } else {
+                ez = Din.na[c] * dz;
+            }
+            Dn.iz[c] = iz;
+            Dn.na[c] = Din.na[c];
+            Dn.nb[c] = Din.nb[c];
+            En[c] = ez;
+        }
+        const float enr = from_right(En[0]);
+        if (P.nprobe && r >= g.i0 && r < g.i1) {
+            for (int pr = 0; pr < P.nprobe; ++pr)
+                if (r == P.probe_i[pr]) {
+#pragma unroll
+                    for (int c = 0; c < C; ++c) {
+                        const int j = g.j0 + c;
+                        if (j == P.probe_j[pr] && j >= g.vj0 && j < g.vj1)
+                            P.probe_out[(size_t)pr * P.probe_len + (t - 1)] = En[c];
+                    }
+                }
+        }
+        // ------------------------------------------------ H update, row q
+        HRow<C> Hout;
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+            const float e = St[s].e[c];
+            const float eR = (c + 1 < C) ? St[s].e[c + 1] : St[s].er;
+            const float eD = En[c];
+            float hx, hy, ax, ay;
+            if (PML) {
+                ax = Hp.ix[c] + (e - eR);
+                ay = Hp.iy[c] + (e - eD);
+                hx = cc.fy3[c] * Hp.hx[c] + cc.fy2[c] * (0.5f * e - 0.5f * eR + fx1q * ax);
+                hy = fx3q * Hp.hy[c] - fx2q * (0.5f * e - 0.5f * eD + cc.fy1[c] * ay);
+            } else {
+                ax = Hp.ix[c];
+                ay = Hp.iy[c];
+                hx = Hp.hx[c] + 0.5f * (e - eR);
+                hy = Hp.hy[c] - 0.5f * (e - eD);
+            }
+            if (BCOL || BROW) {
+                bool upd = true;
+                if (BCOL) upd = (g.j0 + c <= ny - 2);           // column ny-1: no update
+                if (BROW) upd = upd && (q <= nx - 2);            // row nx-1: no update
+                hx = upd ? hx : Hp.hx[c];
+                hy = upd ? hy : Hp.hy[c];
+                ax = upd ? ax : Hp.ix[c];
+                ay = upd ? ay : Hp.iy[c];
+            }
+            Hout.hx[c] = hx; Hout.hy[c] = hy; Hout.ix[c] = ax; Hout.iy[c] = ay;
+        }
+        // carry this level's rows to the next iteration
+        St[s].h = Hin;
+#pragma unroll
+        for (int c = 0; c < C; ++c) St[s].e[c] = En[c];
+        St[s].er = enr;
+        if (s + 1 < K) {
+            Hin = Hout;          // input of the next level
+            Dpend = Dn;
+        } else {
+            // last level: D/E of row r and H of row q are final for this launch
+            if (r >= g.i0 && r < g.i1) {
+                const size_t base = (size_t)r * ny;
+                st<C, VEC>(P.dz_o + base, g.j0, g.vj0, g.vj1, Dn.dz);
+                if (LOSSY) st<C, VEC>(P.iz_o + base, g.j0, g.vj0, g.vj1, Dn.iz);
+                if (P.ez_o) st<C, VEC>(P.ez_o + base, g.j0, g.vj0, g.vj1, En);
+            }
+            if (q >= g.i0 && q < g.i1) {
+                const size_t base = (size_t)q * ny;
+                st<C, VEC>(P.hx_o + base, g.j0, g.vj0, g.vj1, Hout.hx);
+                st<C, VEC>(P.hy_o + base, g.j0, g.vj0, g.vj1, Hout.hy);
+                if (PML) {
+                    st<C, VEC>(P.ihx_o + base, g.j0, g.vj0, g.vj1, Hout.ix);
+                    st<C, VEC>(P.ihy_o + base, g.j0, g.vj0, g.vj1, Hout.iy);
+                }
+            }
+        }
+    }
+}
+
+template <int K, int C, bool LOSSY, bool PML, bool VEC, bool BROW, bool BCOL>
+__device__ __forceinline__ void sweep(const TbParams &P, const Geom &g)
+{
+    ColCoef<C> cc;
+    col_coef<C, PML, BCOL>(P, g.j0, cc);
+    Carry<C> St[K];
+#pragma unroll
+    for (int s = 0; s < K; ++s) {
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+            St[s].h.hx[c] = 0.f; St[s].h.hy[c] = 0.f; St[s].h.ix[c] = 0.f; St[s].h.iy[c] = 0.f;
+            St[s].e[c] = 0.f;
+            St[s].d.dz[c] = 0.f; St[s].d.iz[c] = 0.f; St[s].d.na[c] = 0.f; St[s].d.nb[c] = 0.f;
+        }
+        St[s].er = 0.f;
+    }
+    const int pbeg = max(g.i0 - K, 0);
+    const int pend = g.i1 - 1 + K;
+    // the load of row p+1 is in flight while row p is computed
+    HRow<C> H0, H1;
+    DRow<C> D0, D1;
+    load_row<C, LOSSY, PML, VEC, BCOL>(P, pbeg, g.j0, H0, D0);
+    for (int p = pbeg; p <= pend; ++p) {
+        load_row<C, LOSSY, PML, VEC, BCOL>(P, p + 1, g.j0, H1, D1);
+        iterate<K, C, LOSSY, PML, VEC, BROW, BCOL>(P, g, cc, p, H0, D0, St);
+        H0 = H1;
+        D0 = D1;
+    }
+}
+
+template <int K, int C, bool LOSSY, bool PML, bool VEC>
+__global__ void __launch_bounds__(64) tm_tb_kernel(const TbParams P)
+{
+    constexpr int S = 64 * C;
+    const int js = (int)blockIdx.x * (S - 2 * P.halo) - P.halo;   // first column of the strip
+    Geom g;
+    g.j0 = js + (int)threadIdx.x * C;
+    g.i0 = (int)blockIdx.y * P.rows;
+    g.i1 = min(g.i0 + P.rows, P.nx);
+    g.vj0 = js + P.halo;
+    g.vj1 = min(js + S - P.halo, P.ny);
+    // every row the sweep touches, [i0-2K, i1-1+K], inside the interior band?
+    const bool brow = !(g.i0 - 2 * K >= P.rlo && g.i1 - 1 + K < P.rhi);
+    // every column of the strip inside the interior band?
+    const bool bcol = !(js >= P.clo && js + S <= P.chi);
+    if (!brow && !bcol) sweep<K, C, LOSSY, PML, VEC, false, false>(P, g);
+    else if (!brow) sweep<K, C, LOSSY, PML, VEC, false, true>(P, g);
+    else if (!bcol) sweep<K, C, LOSSY, PML, VEC, true, false>(P, g);
+    else sweep<K, C, LOSSY, PML, VEC, true, true>(P, g);
+}
+
+template <int K, int C, bool VEC>
+int launch_kv(const TbParams &p, bool lossy, bool pml, hipStream_t s)
+{
+    constexpr int S = 64 * C;
+    const int w = S - 2 * p.halo;
+    const dim3 grid((p.ny + w - 1) / w, (p.nx + p.rows - 1) / p.rows);
+    if (lossy && pml) hipLaunchKernelGGL((tm_tb_kernel<K, C, true, true, VEC>), grid, dim3(64), 0, s, p);
+    else if (lossy) hipLaunchKernelGGL((tm_tb_kernel<K, C, true, false, VEC>), grid, dim3(64), 0, s, p);
+    else if (pml) hipLaunchKernelGGL((tm_tb_kernel<K, C, false, true, VEC>), grid, dim3(64), 0, s, p);
+    else hipLaunchKernelGGL((tm_tb_kernel<K, C, false, false, VEC>), grid, dim3(64), 0, s, p);
+    return hip_check(hipGetLastError(), "fd2d_tm_run launch");
+}
+
+}  // namespace tb
+
+template <int K>
+int launch_fd2d_tb_depth(const TbParams &p, int cols, bool lossy, bool pml, hipStream_t s)
+{
+    // vector loads need every lane's columns wholly inside or outside the grid
+    // (the host rounds the halo up to a multiple of the columns per lane)
+    if (cols == 2) {
+        if (p.ny % 2 == 0) return tb::launch_kv<K, 2, true>(p, lossy, pml, s);
+        return tb::launch_kv<K, 2, false>(p, lossy, pml, s);
+    }
+    if (cols == 1) return tb::launch_kv<K, 1, false>(p, lossy, pml, s);
+    return fail("columns per lane %d not built", cols);
+}
+
+}  // namespace fdtd

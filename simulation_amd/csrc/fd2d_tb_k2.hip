@@ -1,0 +1,6 @@
+// Instantiation of the temporally blocked TM kernels for depth 2 (fd2d_tb_impl.h).
+#include "fd2d_tb_impl.h"
+
+namespace fdtd {
+template int launch_fd2d_tb_depth<2>(const TbParams &, int, bool, bool, hipStream_t);
+}  // namespace fdtd

@@ -115,7 +115,8 @@ int plan_tm(const fd2d_tm_desc *d, int nsteps, Plan *pl)
         H = (int)h;
     }
     if (H < 1) return fail("fd2d_tm: rows=%d", H);
-    pl->depth = D; pl->cols = C; pl->rows = H; pl->halo = D;
+    // strip halo: the deepest launch's cone, rounded up to whole lanes
+    pl->depth = D; pl->cols = C; pl->rows = H; pl->halo = ((D + C - 1) / C) * C;
     pl->launches = 0; pl->tail_unfused = 0;
     if (nsteps <= 0) return 0;
     if (nsteps == 1) { pl->tail_unfused = 1; return 0; }
@@ -409,6 +410,17 @@ int fd2d_tm_run(const fd2d_tm_desc *d, int t0, int nsteps, void *stream)
     p.naz = d->md.naz; p.nbz = d->md.nbz;
     if (pml) p.pml = to_ptrs(d->pml);
     p.halo = pl.halo; p.rows = pl.rows;
+    // interior band: inside [1, n-2] and, with a PML, inside the promised
+    // default-profile band (empty promise: every wave runs the general body)
+    if (pml) {
+        const int *br = d->band_rows, *bc = d->band_cols;
+        p.rlo = br[1] > br[0] ? (br[0] > 1 ? br[0] : 1) : 0;
+        p.rhi = br[1] > br[0] ? (br[1] < d->nx - 1 ? br[1] : d->nx - 1) : 0;
+        p.clo = bc[1] > bc[0] ? (bc[0] > 1 ? bc[0] : 1) : 0;
+        p.chi = bc[1] > bc[0] ? (bc[1] < d->ny - 1 ? bc[1] : d->ny - 1) : 0;
+    } else {
+        p.rlo = 1; p.rhi = d->nx - 1; p.clo = 1; p.chi = d->ny - 1;
+    }
     p.nsrc = d->nsrc;
     for (int k = 0; k < d->nsrc; ++k) { p.src_i[k] = d->src_i[k]; p.src_j[k] = d->src_j[k]; }
     p.src_wave = d->src_wave; p.wave_len = d->wave_len;

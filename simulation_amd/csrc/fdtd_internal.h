@@ -55,6 +55,9 @@ struct TbParams {
     PmlPtrs pml;
     int halo;                      // column halo of the strip geometry (>= depth)
     int rows;                      // rows per block
+    // interior band: rows [rlo, rhi) and columns [clo, chi) lie inside
+    // [1, n-2] and hold the default profile values (empty when unknown)
+    int rlo, rhi, clo, chi;
     int t_first;                   // time index of the first fused step
     int nsrc;
     int src_i[FDTD_MAX_SOURCES], src_j[FDTD_MAX_SOURCES];
@@ -97,10 +100,13 @@ int launch_fd2d_dielectric(int nx, int ny, int npml, double dt, const CylList &c
 int launch_fd2d_energy(int nx, int ny, int i0, int i1, int j0, int j1, const float *ez,
                        const float *hx, const float *hy, double *work, hipStream_t s);
 
-// Temporally blocked TM launch: depth in [1, tb_max_depth()], cols in {1,2,4}.
+// Temporally blocked TM launch: depth in [1, tb_max_depth()], cols in {1,2}.
 int tb_max_depth();
 bool tb_supported(int depth, int cols);
 int launch_fd2d_tb(const TbParams &p, int depth, int cols, bool lossy, bool pml,
                    hipStream_t s);
+// one instantiation per depth (fd2d_tb_k<depth>.hip)
+template <int K>
+int launch_fd2d_tb_depth(const TbParams &p, int cols, bool lossy, bool pml, hipStream_t s);
 
 }  // namespace fdtd

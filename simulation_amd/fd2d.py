@@ -157,8 +157,33 @@ class TMSolver:
         self.probe_out = (torch.zeros((len(probes), self.probe_len), dtype=torch.float32, device=dev)
                           if probes else None)
         self.depth, self.rows, self.cols = depth, rows, cols
+        self.band_rows, self.band_cols = self.default_band()
         self.t = 0
         self._work = torch.empty(_lib.ENERGY_WORK, dtype=torch.float64, device=dev)
+
+    def default_band(self):
+        """Rows and columns around the grid centre whose profile values are the
+        defaults (f*1 = 0, the rest 1): the band promised to fd2d_tm_run.
+        Call again after changing the pmlayer arrays."""
+        if self.pml is None:
+            return (0, 0), (0, 0)
+
+        def run(mask):
+            n = len(mask)
+            c = n // 2
+            if not mask[c]:
+                return (0, 0)
+            lo = c
+            while lo > 0 and mask[lo - 1]:
+                lo -= 1
+            hi = c + 1
+            while hi < n and mask[hi]:
+                hi += 1
+            return (lo, hi)
+        pm = self.pml
+        rows = ((pm.fx1 == 0) & (pm.fx2 == 1) & (pm.fx3 == 1) & (pm.gx2 == 1) & (pm.gx3 == 1))
+        cols = ((pm.fy1 == 0) & (pm.fy2 == 1) & (pm.fy3 == 1) & (pm.gy2 == 1) & (pm.gy3 == 1))
+        return run(rows.cpu().numpy()), run(cols.cpu().numpy())
 
     def desc(self) -> _lib.Fd2dTmDesc:
         d = _lib.Fd2dTmDesc()
@@ -181,6 +206,8 @@ class TMSolver:
         d.probe_out = fptr(self.probe_out)
         d.probe_len = self.probe_len
         d.depth, d.rows, d.cols = self.depth, self.rows, self.cols
+        d.band_rows[0], d.band_rows[1] = self.band_rows
+        d.band_cols[0], d.band_cols[1] = self.band_cols
         return d
 
     def plan(self, nsteps: int) -> dict:
