@@ -26,9 +26,15 @@
 //   * Waves whose rows (columns) all lie inside [1, n-2] and inside the band
 //     the caller promises to hold the default PML profile values run a body
 //     in which those values are compile-time constants and boundary tests are
-//     absent; the few boundary waves run the general body.  Multiplying by a
-//     constant 1 is exact, so the compiler may drop it without changing any
-//     result; products with a constant 0 are kept.
+//     absent; the few waves at the grid edges or holding a source or probe
+//     run the general body.  Multiplying by a constant 1 is exact, so the
+//     compiler may drop it without changing any result; products with a
+//     constant 0 are kept.
+//   * The interior and the other waves are two kernels (two launches over
+//     the same grid, each wave returning at once if it belongs to the other
+//     launch), so the common interior body is not limited by the register
+//     needs of the general one.  The host runs the second on an auxiliary
+//     stream, concurrently with the first.
 //   * Launches read one state set and write the other (ping-pong), so waves
 //     never read a cell another wave has already advanced.
 //
@@ -73,44 +79,61 @@ struct Geom {
     int vj0, vj1;     // owned columns [vj0, vj1)
 };
 
-// Load the lane's C columns of a row.  VEC: one vector load (every lane's
-// columns are wholly inside or outside the grid); CLAMP: columns outside the
-// grid read an in-bounds neighbour (their values are never used).
+// Element offsets below are 32-bit (the host rejects grids of 2^30 cells or
+// more), so every access is a uniform 64-bit array base plus a 32-bit
+// per-lane offset.
+__device__ __forceinline__ const float *at(const float *base, unsigned off)
+{
+    const unsigned bytes = off << 2;                  // < 2^32: zero-extended 32-bit offset
+    return reinterpret_cast<const float *>(reinterpret_cast<const char *>(base) + bytes);
+}
+__device__ __forceinline__ float *at(float *base, unsigned off)
+{
+    const unsigned bytes = off << 2;
+    return reinterpret_cast<float *>(reinterpret_cast<char *>(base) + bytes);
+}
+
+// Load the lane's C columns of a row (element offset `row` of the row start).
+// VEC: one vector load (every lane's columns are wholly inside or outside the
+// grid); CLAMP: columns outside the grid read an in-bounds neighbour (their
+// values are never used).
 template <int C, bool VEC, bool CLAMP>
-__device__ __forceinline__ void ld(const float *__restrict__ row, int j0, int ny, float (&v)[C])
+__device__ __forceinline__ void ld(const float *__restrict__ base, unsigned row, int j0, int ny,
+                                   float (&v)[C])
 {
     if (VEC) {
         const int jc = CLAMP ? min(max(j0, 0), ny - C) : j0;
         if constexpr (C == 2) {
-            const float2 q = *reinterpret_cast<const float2 *>(row + jc);
+            const float2 q = *reinterpret_cast<const float2 *>(at(base, row + (unsigned)jc));
             v[0] = q.x; v[1] = q.y;
         } else {
 #pragma unroll
-            for (int c = 0; c < C; ++c) v[c] = row[jc + c];
+            for (int c = 0; c < C; ++c) v[c] = *at(base, row + (unsigned)(jc + c));
         }
     } else {
 #pragma unroll
-        for (int c = 0; c < C; ++c) v[c] = row[CLAMP ? min(max(j0 + c, 0), ny - 1) : j0 + c];
+        for (int c = 0; c < C; ++c)
+            v[c] = *at(base, row + (unsigned)(CLAMP ? min(max(j0 + c, 0), ny - 1) : j0 + c));
     }
 }
 
 template <int C, bool VEC>
-__device__ __forceinline__ void st(float *__restrict__ row, int j0, int vj0, int vj1,
+__device__ __forceinline__ void st(float *__restrict__ base, unsigned row, int j0, int vj0, int vj1,
                                    const float (&v)[C])
 {
     if (VEC) {
         if (j0 >= vj0 && j0 + C <= vj1) {
             if constexpr (C == 2) {
-                *reinterpret_cast<float2 *>(row + j0) = make_float2(v[0], v[1]);
+                *reinterpret_cast<float2 *>(at(base, row + (unsigned)j0)) = make_float2(v[0], v[1]);
             } else {
 #pragma unroll
-                for (int c = 0; c < C; ++c) row[j0 + c] = v[c];
+                for (int c = 0; c < C; ++c) *at(base, row + (unsigned)(j0 + c)) = v[c];
             }
         }
     } else {
 #pragma unroll
         for (int c = 0; c < C; ++c)
-            if (j0 + c >= vj0 && j0 + c < vj1) row[j0 + c] = v[c];
+            if (j0 + c >= vj0 && j0 + c < vj1) *at(base, row + (unsigned)(j0 + c)) = v[c];
     }
 }
 
@@ -118,21 +141,21 @@ template <int C, bool LOSSY, bool PML, bool VEC, bool CLAMP>
 __device__ __forceinline__ void load_row(const TbParams &P, int p, int j0, HRow<C> &H, DRow<C> &D)
 {
     const int pc = min(max(p, 0), P.nx - 1);     // rows outside the grid: any in-bounds row
-    const size_t base = (size_t)pc * P.ny;
-    ld<C, VEC, CLAMP>(P.dz_i + base, j0, P.ny, D.dz);
-    ld<C, VEC, CLAMP>(P.naz + base, j0, P.ny, D.na);
-    ld<C, VEC, CLAMP>(P.hx_i + base, j0, P.ny, H.hx);
-    ld<C, VEC, CLAMP>(P.hy_i + base, j0, P.ny, H.hy);
+    const unsigned row = (unsigned)pc * (unsigned)P.ny;
+    ld<C, VEC, CLAMP>(P.dz_i, row, j0, P.ny, D.dz);
+    ld<C, VEC, CLAMP>(P.naz, row, j0, P.ny, D.na);
+    ld<C, VEC, CLAMP>(P.hx_i, row, j0, P.ny, H.hx);
+    ld<C, VEC, CLAMP>(P.hy_i, row, j0, P.ny, H.hy);
     if (LOSSY) {
-        ld<C, VEC, CLAMP>(P.iz_i + base, j0, P.ny, D.iz);
-        ld<C, VEC, CLAMP>(P.nbz + base, j0, P.ny, D.nb);
+        ld<C, VEC, CLAMP>(P.iz_i, row, j0, P.ny, D.iz);
+        ld<C, VEC, CLAMP>(P.nbz, row, j0, P.ny, D.nb);
     } else {
 #pragma unroll
         for (int c = 0; c < C; ++c) { D.iz[c] = 0.f; D.nb[c] = 0.f; }
     }
     if (PML) {
-        ld<C, VEC, CLAMP>(P.ihx_i + base, j0, P.ny, H.ix);
-        ld<C, VEC, CLAMP>(P.ihy_i + base, j0, P.ny, H.iy);
+        ld<C, VEC, CLAMP>(P.ihx_i, row, j0, P.ny, H.ix);
+        ld<C, VEC, CLAMP>(P.ihy_i, row, j0, P.ny, H.iy);
     } else {
 #pragma unroll
         for (int c = 0; c < C; ++c) { H.ix[c] = 0.f; H.iy[c] = 0.f; }
@@ -160,8 +183,9 @@ __device__ __forceinline__ void col_coef(const TbParams &P, int j0, ColCoef<C> &
 
 // One sweep iteration at position p: level s computes the E update of row
 // p-s and the H update of row p-s-1.  BROW: rows may lie outside [1, nx-2] or
-// outside the default band; BCOL: the same for the strip's columns.
-template <int K, int C, bool LOSSY, bool PML, bool VEC, bool BROW, bool BCOL>
+// outside the default band; BCOL: the same for the strip's columns; SP: a
+// source or probe lies in the rows and columns the wave computes.
+template <int K, int C, bool LOSSY, bool PML, bool VEC, bool BROW, bool BCOL, bool SP>
 __device__ __forceinline__ void iterate(const TbParams &P, const Geom &g, const ColCoef<C> &cc,
                                         int p, const HRow<C> &Hload, const DRow<C> &Dload,
                                         Carry<C> (&St)[K])
@@ -205,7 +229,7 @@ __device__ __forceinline__ void iterate(const TbParams &P, const Geom &g, const 
             if (BROW) dz = (r >= 1) ? dz : Din.dz[c];            // row 0: no update
             Dn.dz[c] = dz;
         }
-        for (int k = 0; k < P.nsrc; ++k) {
+        for (int k = 0; SP && k < P.nsrc; ++k) {
             if (r == P.src_i[k]) {
                 const float v = P.src_wave[(size_t)k * P.wave_len + (t - 1)];
 #pragma unroll
@@ -229,7 +253,7 @@ __device__ __forceinline__ void iterate(const TbParams &P, const Geom &g, const 
             En[c] = ez;
         }
         const float enr = from_right(En[0]);
-        if (P.nprobe && r >= g.i0 && r < g.i1) {
+        if (SP && P.nprobe && r >= g.i0 && r < g.i1) {
             for (int pr = 0; pr < P.nprobe; ++pr)
                 if (r == P.probe_i[pr]) {
 #pragma unroll
@@ -281,25 +305,25 @@ __device__ __forceinline__ void iterate(const TbParams &P, const Geom &g, const 
         } else {
             // last level: D/E of row r and H of row q are final for this launch
             if (r >= g.i0 && r < g.i1) {
-                const size_t base = (size_t)r * ny;
-                st<C, VEC>(P.dz_o + base, g.j0, g.vj0, g.vj1, Dn.dz);
-                if (LOSSY) st<C, VEC>(P.iz_o + base, g.j0, g.vj0, g.vj1, Dn.iz);
-                if (P.ez_o) st<C, VEC>(P.ez_o + base, g.j0, g.vj0, g.vj1, En);
+                const unsigned row = (unsigned)r * (unsigned)ny;
+                st<C, VEC>(P.dz_o, row, g.j0, g.vj0, g.vj1, Dn.dz);
+                if (LOSSY) st<C, VEC>(P.iz_o, row, g.j0, g.vj0, g.vj1, Dn.iz);
+                if (P.ez_o) st<C, VEC>(P.ez_o, row, g.j0, g.vj0, g.vj1, En);
             }
             if (q >= g.i0 && q < g.i1) {
-                const size_t base = (size_t)q * ny;
-                st<C, VEC>(P.hx_o + base, g.j0, g.vj0, g.vj1, Hout.hx);
-                st<C, VEC>(P.hy_o + base, g.j0, g.vj0, g.vj1, Hout.hy);
+                const unsigned row = (unsigned)q * (unsigned)ny;
+                st<C, VEC>(P.hx_o, row, g.j0, g.vj0, g.vj1, Hout.hx);
+                st<C, VEC>(P.hy_o, row, g.j0, g.vj0, g.vj1, Hout.hy);
                 if (PML) {
-                    st<C, VEC>(P.ihx_o + base, g.j0, g.vj0, g.vj1, Hout.ix);
-                    st<C, VEC>(P.ihy_o + base, g.j0, g.vj0, g.vj1, Hout.iy);
+                    st<C, VEC>(P.ihx_o, row, g.j0, g.vj0, g.vj1, Hout.ix);
+                    st<C, VEC>(P.ihy_o, row, g.j0, g.vj0, g.vj1, Hout.iy);
                 }
             }
         }
     }
 }
 
-template <int K, int C, bool LOSSY, bool PML, bool VEC, bool BROW, bool BCOL>
+template <int K, int C, bool LOSSY, bool PML, bool VEC, bool BROW, bool BCOL, bool SP>
 __device__ __forceinline__ void sweep(const TbParams &P, const Geom &g)
 {
     ColCoef<C> cc;
@@ -317,19 +341,24 @@ __device__ __forceinline__ void sweep(const TbParams &P, const Geom &g)
     }
     const int pbeg = max(g.i0 - K, 0);
     const int pend = g.i1 - 1 + K;
-    // the load of row p+1 is in flight while row p is computed
+    // row p+1 is in flight while row p is computed
     HRow<C> H0, H1;
     DRow<C> D0, D1;
     load_row<C, LOSSY, PML, VEC, BCOL>(P, pbeg, g.j0, H0, D0);
     for (int p = pbeg; p <= pend; ++p) {
         load_row<C, LOSSY, PML, VEC, BCOL>(P, p + 1, g.j0, H1, D1);
-        iterate<K, C, LOSSY, PML, VEC, BROW, BCOL>(P, g, cc, p, H0, D0, St);
+        iterate<K, C, LOSSY, PML, VEC, BROW, BCOL, SP>(P, g, cc, p, H0, D0, St);
         H0 = H1;
         D0 = D1;
     }
 }
 
-template <int K, int C, bool LOSSY, bool PML, bool VEC>
+// PART 0: the interior waves; PART 1: every other wave (grid edges, PML
+// bands, sources and probes).  Both launches cover the whole grid and each
+// wave returns at once if it belongs to the other part, so the two kernels
+// write disjoint cells and may run concurrently; each gets the register
+// allocation of its own body.
+template <int K, int C, bool LOSSY, bool PML, bool VEC, int PART>
 __global__ void __launch_bounds__(64) tm_tb_kernel(const TbParams P)
 {
     constexpr int S = 64 * C;
@@ -344,37 +373,54 @@ __global__ void __launch_bounds__(64) tm_tb_kernel(const TbParams P)
     const bool brow = !(g.i0 - 2 * K >= P.rlo && g.i1 - 1 + K < P.rhi);
     // every column of the strip inside the interior band?
     const bool bcol = !(js >= P.clo && js + S <= P.chi);
-    if (!brow && !bcol) sweep<K, C, LOSSY, PML, VEC, false, false>(P, g);
-    else if (!brow) sweep<K, C, LOSSY, PML, VEC, false, true>(P, g);
-    else if (!bcol) sweep<K, C, LOSSY, PML, VEC, true, false>(P, g);
-    else sweep<K, C, LOSSY, PML, VEC, true, true>(P, g);
+    // a source or probe inside the rows and columns this wave computes?
+    bool sp = false;
+    for (int k = 0; k < P.nsrc; ++k)
+        sp |= P.src_i[k] >= g.i0 - 2 * K && P.src_i[k] <= g.i1 + K && P.src_j[k] >= js &&
+              P.src_j[k] < js + S;
+    for (int k = 0; k < P.nprobe; ++k)
+        sp |= P.probe_i[k] >= g.i0 && P.probe_i[k] < g.i1 && P.probe_j[k] >= js &&
+              P.probe_j[k] < js + S;
+    const bool interior = !brow && !bcol && !sp;
+    if (PART == 0) {
+        if (interior) sweep<K, C, LOSSY, PML, VEC, false, false, false>(P, g);
+    } else if (!interior) {
+        if (!brow && !sp) sweep<K, C, LOSSY, PML, VEC, false, true, false>(P, g);
+        else sweep<K, C, LOSSY, PML, VEC, true, true, true>(P, g);
+    }
 }
 
-template <int K, int C, bool VEC>
+template <int K, int C, bool VEC, int PART>
 int launch_kv(const TbParams &p, bool lossy, bool pml, hipStream_t s)
 {
     constexpr int S = 64 * C;
     const int w = S - 2 * p.halo;
     const dim3 grid((p.ny + w - 1) / w, (p.nx + p.rows - 1) / p.rows);
-    if (lossy && pml) hipLaunchKernelGGL((tm_tb_kernel<K, C, true, true, VEC>), grid, dim3(64), 0, s, p);
-    else if (lossy) hipLaunchKernelGGL((tm_tb_kernel<K, C, true, false, VEC>), grid, dim3(64), 0, s, p);
-    else if (pml) hipLaunchKernelGGL((tm_tb_kernel<K, C, false, true, VEC>), grid, dim3(64), 0, s, p);
-    else hipLaunchKernelGGL((tm_tb_kernel<K, C, false, false, VEC>), grid, dim3(64), 0, s, p);
+    if (lossy && pml) hipLaunchKernelGGL((tm_tb_kernel<K, C, true, true, VEC, PART>), grid, dim3(64), 0, s, p);
+    else if (lossy) hipLaunchKernelGGL((tm_tb_kernel<K, C, true, false, VEC, PART>), grid, dim3(64), 0, s, p);
+    else if (pml) hipLaunchKernelGGL((tm_tb_kernel<K, C, false, true, VEC, PART>), grid, dim3(64), 0, s, p);
+    else hipLaunchKernelGGL((tm_tb_kernel<K, C, false, false, VEC, PART>), grid, dim3(64), 0, s, p);
     return hip_check(hipGetLastError(), "fd2d_tm_run launch");
+}
+
+template <int K, int C, int PART>
+int launch_part(const TbParams &p, bool lossy, bool pml, hipStream_t s)
+{
+    // vector loads need every lane's columns wholly inside or outside the grid
+    // (the host rounds the halo up to a multiple of the columns per lane)
+    if (C == 2 && p.ny % 2 == 0) return launch_kv<K, C, true, PART>(p, lossy, pml, s);
+    return launch_kv<K, C, false, PART>(p, lossy, pml, s);
 }
 
 }  // namespace tb
 
 template <int K>
-int launch_fd2d_tb_depth(const TbParams &p, int cols, bool lossy, bool pml, hipStream_t s)
+int launch_fd2d_tb_depth(const TbParams &p, int cols, bool lossy, bool pml, int part, hipStream_t s)
 {
-    // vector loads need every lane's columns wholly inside or outside the grid
-    // (the host rounds the halo up to a multiple of the columns per lane)
-    if (cols == 2) {
-        if (p.ny % 2 == 0) return tb::launch_kv<K, 2, true>(p, lossy, pml, s);
-        return tb::launch_kv<K, 2, false>(p, lossy, pml, s);
-    }
-    if (cols == 1) return tb::launch_kv<K, 1, false>(p, lossy, pml, s);
+    if (cols == 2)
+        return part == 0 ? tb::launch_part<K, 2, 0>(p, lossy, pml, s) : tb::launch_part<K, 2, 1>(p, lossy, pml, s);
+    if (cols == 1)
+        return part == 0 ? tb::launch_part<K, 1, 0>(p, lossy, pml, s) : tb::launch_part<K, 1, 1>(p, lossy, pml, s);
     return fail("columns per lane %d not built", cols);
 }
 

@@ -2,5 +2,5 @@
 #include "fd2d_tb_impl.h"
 
 namespace fdtd {
-template int launch_fd2d_tb_depth<3>(const TbParams &, int, bool, bool, hipStream_t);
+template int launch_fd2d_tb_depth<3>(const TbParams &, int, bool, bool, int, hipStream_t);
 }  // namespace fdtd

@@ -105,13 +105,11 @@ int plan_tm(const fd2d_tm_desc *d, int nsteps, Plan *pl)
     if (64 * C - 2 * D < 8) return fail("fd2d_tm: depth %d too large for %d columns/lane", D, C);
     int H = d->rows;
     if (H == 0) {
-        // default: enough blocks for ~4 waves per SIMD of a 256-CU part, 32..256 rows
-        const int w = 64 * C - 2 * D;
-        const long strips = (d->ny + w - 1) / w;
-        long want = (4096 + strips - 1) / strips;
-        long h = (d->nx + want - 1) / want;
-        if (h < 32) h = 32;
-        if (h > 256) h = 256;
+        // default from measurements on MI355X (4096^2 .. 16384^2): ~110 row
+        // blocks per strip, i.e. several waves per SIMD over the launch
+        long h = d->nx / 110;
+        if (h < 16) h = 16;
+        if (h > 512) h = 512;
         H = (int)h;
     }
     if (H < 1) return fail("fd2d_tm: rows=%d", H);
@@ -126,6 +124,34 @@ int plan_tm(const fd2d_tm_desc *d, int nsteps, Plan *pl)
         else { pl->tail_unfused = 1; L = (nsteps - 1 + D - 1) / D; if (L % 2) L += 1; }
     }
     pl->launches = L;
+    return 0;
+}
+
+// The auxiliary stream on which the boundary part of each temporally blocked
+// launch runs, concurrently with the interior part on the caller's stream,
+// and the two events that order them.  One set per host thread and device,
+// created on first use and kept for the life of the process (no device
+// memory is involved).
+struct Aux {
+    hipStream_t s = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
+};
+
+int aux_for_current_device(Aux **out)
+{
+    static thread_local Aux tab[64];
+    int dev = 0;
+    int rc = hip_check(hipGetDevice(&dev), "hipGetDevice");
+    if (rc) return rc;
+    if (dev < 0 || dev >= 64) return fail("device ordinal %d out of range", dev);
+    Aux &a = tab[dev];
+    if (!a.s) {
+        rc = hip_check(hipStreamCreateWithFlags(&a.s, hipStreamNonBlocking), "aux stream");
+        if (!rc) rc = hip_check(hipEventCreateWithFlags(&a.fork, hipEventDisableTiming), "aux event");
+        if (!rc) rc = hip_check(hipEventCreateWithFlags(&a.join, hipEventDisableTiming), "aux event");
+        if (rc) return rc;
+    }
+    *out = &a;
     return 0;
 }
 
@@ -343,6 +369,8 @@ static int tm_validate(const fd2d_tm_desc *d, int t0, int nsteps)
 {
     if (!d) return fail("fd2d_tm: desc is NULL");
     if (d->nx < 2 || d->ny < 2) return fail("fd2d_tm: grid %dx%d too small", d->nx, d->ny);
+    if ((long long)d->nx * d->ny >= (1LL << 30))
+        return fail("fd2d_tm: grid %dx%d has 2^30 cells or more (32-bit offsets)", d->nx, d->ny);
     if (!d->dz || !d->ez || !d->hx || !d->hy || !d->md.naz)
         return fail("fd2d_tm: dz, ez, hx, hy, md.naz are required");
     const bool lossy = d->md.nbz != nullptr;
@@ -430,6 +458,11 @@ int fd2d_tm_run(const fd2d_tm_desc *d, int t0, int nsteps, void *stream)
     float *A[6] = {d->dz, d->iz, d->hx, d->hy, d->ihx, d->ihy};
     float *B[6] = {d->dz2, d->iz2, d->hx2, d->hy2, d->ihx2, d->ihy2};
     const int L = pl.launches;
+    Aux *ax = nullptr;
+    if (L) {
+        rc = aux_for_current_device(&ax);
+        if (rc) return rc;
+    }
     int done = 0;
     for (int l = 0; l < L; ++l) {
         const int k = nfused / L + (l < nfused % L ? 1 : 0);
@@ -439,9 +472,21 @@ int fd2d_tm_run(const fd2d_tm_desc *d, int t0, int nsteps, void *stream)
         p.dz_o = out[0]; p.iz_o = out[1]; p.hx_o = out[2]; p.hy_o = out[3]; p.ihx_o = out[4]; p.ihy_o = out[5];
         p.ez_o = (l == L - 1 && !pl.tail_unfused) ? d->ez : nullptr;
         p.t_first = t0 + done + 1;
-        rc = launch_fd2d_tb(p, k, pl.cols, lossy, pml, S(stream));
+        // boundary part on the auxiliary stream, after all work queued so far
+        // on the caller's stream (which includes the previous interior part)
+        rc = hip_check(hipEventRecord(ax->fork, S(stream)), "fork event");
+        if (!rc) rc = hip_check(hipStreamWaitEvent(ax->s, ax->fork, 0), "fork wait");
+        if (!rc) rc = launch_fd2d_tb(p, k, pl.cols, lossy, pml, 1, ax->s);
+        // interior part on the caller's stream, after the previous boundary part
+        if (!rc && l > 0) rc = hip_check(hipStreamWaitEvent(S(stream), ax->join, 0), "join wait");
+        if (!rc) rc = launch_fd2d_tb(p, k, pl.cols, lossy, pml, 0, S(stream));
+        if (!rc) rc = hip_check(hipEventRecord(ax->join, ax->s), "join event");
         if (rc) return rc;
         done += k;
+    }
+    if (L) {
+        rc = hip_check(hipStreamWaitEvent(S(stream), ax->join, 0), "join wait");
+        if (rc) return rc;
     }
     if (pl.tail_unfused) {
         const int t = t0 + done + 1;

@@ -100,13 +100,15 @@ int launch_fd2d_dielectric(int nx, int ny, int npml, double dt, const CylList &c
 int launch_fd2d_energy(int nx, int ny, int i0, int i1, int j0, int j1, const float *ez,
                        const float *hx, const float *hy, double *work, hipStream_t s);
 
-// Temporally blocked TM launch: depth in [1, tb_max_depth()], cols in {1,2}.
+// Temporally blocked TM launch: depth in [1, tb_max_depth()], cols in {1,2};
+// part 0 = interior waves, part 1 = all other waves (see fd2d_tb_impl.h).
 int tb_max_depth();
 bool tb_supported(int depth, int cols);
-int launch_fd2d_tb(const TbParams &p, int depth, int cols, bool lossy, bool pml,
+int launch_fd2d_tb(const TbParams &p, int depth, int cols, bool lossy, bool pml, int part,
                    hipStream_t s);
 // one instantiation per depth (fd2d_tb_k<depth>.hip)
 template <int K>
-int launch_fd2d_tb_depth(const TbParams &p, int cols, bool lossy, bool pml, hipStream_t s);
+int launch_fd2d_tb_depth(const TbParams &p, int cols, bool lossy, bool pml, int part,
+                         hipStream_t s);
 
 }  // namespace fdtd
