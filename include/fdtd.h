@@ -228,6 +228,69 @@ int fd2d_tm_run(const fd2d_tm_desc *d, int t0, int nsteps, void *stream);
 int fd2d_tm_plan(const fd2d_tm_desc *d, int nsteps,
                  int *launches, int *depth, int *rows, int *cols);
 
+/* ------------------------------------------------------ batched grids
+ * Many independent TM grids of different sizes advanced together by one
+ * persistent kernel per fused step group.  No reference counterpart: the
+ * reference runs one grid per program (fd2d/pycuda/fd2d_3_4.py:269-389);
+ * each grid here evolves exactly as fd2d_tm_run would evolve it alone (bit
+ * for bit).  All grids of a batch share the form (lossy PML: md.nbz and the
+ * ten profiles given) and advance by the same steps.                        */
+typedef struct fd2d_grid {
+    int nx, ny;
+    float *dz, *iz, *ez, *hx, *hy, *ihx, *ihy;       /* state (ez: final step only) */
+    float *dz2, *iz2, *hx2, *hy2, *ihx2, *ihy2;     /* partner state set */
+    fdtd_medium md;
+    fdtd_pml pml;
+    int band_rows[2], band_cols[2];                 /* as in fd2d_tm_desc */
+    int src_i, src_j;                               /* hard source; src_i < 0: none */
+    const float *src_wave;                          /* value of step t at [t-1] */
+    int wave_len;
+    int probe_i, probe_j;                           /* ez probe; probe_i < 0: none */
+    float *probe_out;                               /* ez after step t at [t-1] */
+    int probe_len;
+} fd2d_grid;
+
+typedef struct fd2d_batch_desc {
+    int ngrids;
+    const fd2d_grid *grids_dev;   /* device copy of the grid records */
+    const int *units_dev;         /* device work list, 4 ints per unit */
+    int nunits;
+    int depth, rows, cols;        /* fixed when the work list is made */
+    int waves;                    /* workgroups of the persistent launch */
+    int vec;                      /* 1: every grid allows vector loads */
+} fd2d_batch_desc;
+
+/* Number of work units (and so ints/4 of units_dev) a batch needs for the
+ * given depth (0 = 8), rows per block (0 = 32) and columns per lane (0 = 1). */
+int fd2d_batch_units(const fd2d_grid *grids, int ngrids, int depth, int rows, int cols,
+                     int *nunits);
+
+/* Validate the grids (host array), write the work list and the grid records
+ * into the caller's device buffers (asynchronously on stream; the host
+ * arrays may be released on return) and fill *out.  units_dev holds
+ * 4*max_units ints, grids_dev ngrids records.  waves = 0 picks the number of
+ * waves the device keeps resident.                                          */
+int fd2d_batch_upload(const fd2d_grid *grids, int ngrids, int depth, int rows, int cols,
+                      int waves, fd2d_grid *grids_dev, int *units_dev, int max_units,
+                      fd2d_batch_desc *out, void *stream);
+
+/* Advance every grid by nsteps (t = t0+1 .. t0+nsteps).  max_wave_len /
+ * max_probe_len: the smallest wave_len / probe_len over the grids (checked
+ * against t0+nsteps; pass INT_MAX when no grid has a source / probe).  The
+ * steps are split into an even number of launches of at most `depth` steps,
+ * so that the result lands in the grids' own state sets: nsteps = 1, or an
+ * odd nsteps with depth 1, is rejected.                                    */
+int fd2d_batch_run(const fd2d_batch_desc *b, int t0, int nsteps, int max_wave_len,
+                   int max_probe_len, void *stream);
+
+/* The same launches captured into one executable HIP graph (returned as an
+ * opaque handle), launched by fdtd_graph_launch and freed by
+ * fdtd_graph_destroy.  stream must not be the null stream.                */
+int fd2d_batch_graph(const fd2d_batch_desc *b, int t0, int nsteps, int max_wave_len,
+                     int max_probe_len, void *stream, void **graph_exec);
+int fdtd_graph_launch(void *graph_exec, void *stream);
+int fdtd_graph_destroy(void *graph_exec);
+
 #ifdef __cplusplus
 }
 #endif

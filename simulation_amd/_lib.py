@@ -63,6 +63,31 @@ class Fd2dTmDesc(C.Structure):
     ]
 
 
+class Fd2dGrid(C.Structure):
+    """fd2d_grid: one grid of a batch (include/fdtd.h)."""
+    _fields_ = [
+        ("nx", C.c_int), ("ny", C.c_int),
+        ("dz", _f), ("iz", _f), ("ez", _f), ("hx", _f), ("hy", _f), ("ihx", _f), ("ihy", _f),
+        ("dz2", _f), ("iz2", _f), ("hx2", _f), ("hy2", _f), ("ihx2", _f), ("ihy2", _f),
+        ("md", FdtdMedium),
+        ("pml", FdtdPml),
+        ("band_rows", C.c_int * 2), ("band_cols", C.c_int * 2),
+        ("src_i", C.c_int), ("src_j", C.c_int),
+        ("src_wave", _f), ("wave_len", C.c_int),
+        ("probe_i", C.c_int), ("probe_j", C.c_int),
+        ("probe_out", _f), ("probe_len", C.c_int),
+    ]
+
+
+class Fd2dBatchDesc(C.Structure):
+    _fields_ = [
+        ("ngrids", C.c_int),
+        ("grids_dev", C.c_void_p), ("units_dev", C.c_void_p),
+        ("nunits", C.c_int), ("depth", C.c_int), ("rows", C.c_int), ("cols", C.c_int),
+        ("waves", C.c_int), ("vec", C.c_int),
+    ]
+
+
 _P = C.c_void_p  # device pointers and streams are passed as integers
 _I = C.c_int
 _D = C.c_double
@@ -96,6 +121,13 @@ SIGNATURES = {
     "fd2d_energy": (_I, [_I, _I, _I, _I, _I, _I, _P, _P, _P, _P, _DP, _P]),
     "fd2d_tm_run": (_I, [C.POINTER(Fd2dTmDesc), _I, _I, _P]),
     "fd2d_tm_plan": (_I, [C.POINTER(Fd2dTmDesc), _I, _IP, _IP, _IP, _IP]),
+    "fd2d_batch_units": (_I, [C.POINTER(Fd2dGrid), _I, _I, _I, _I, _IP]),
+    "fd2d_batch_upload": (_I, [C.POINTER(Fd2dGrid), _I, _I, _I, _I, _I, _P, _P, _I,
+                               C.POINTER(Fd2dBatchDesc), _P]),
+    "fd2d_batch_run": (_I, [C.POINTER(Fd2dBatchDesc), _I, _I, _I, _I, _P]),
+    "fd2d_batch_graph": (_I, [C.POINTER(Fd2dBatchDesc), _I, _I, _I, _I, _P, C.POINTER(C.c_void_p)]),
+    "fdtd_graph_launch": (_I, [_P, _P]),
+    "fdtd_graph_destroy": (_I, [_P]),
 }
 
 _lib = None

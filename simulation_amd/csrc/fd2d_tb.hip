@@ -28,4 +28,35 @@ int launch_fd2d_tb(const TbParams &p, int depth, int cols, bool lossy, bool pml,
     }
 }
 
+int launch_fd2d_batch(const fd2d_grid *grids, const int *units, int nunits, int waves, int depth,
+                      int cols, bool vec, int parity, int t_first, int write_ez, int halo,
+                      int rows, hipStream_t s)
+{
+#define FDTD_BATCH_CASE(k)                                                                   \
+    case k:                                                                                  \
+        return launch_fd2d_batch_depth<k>(grids, units, nunits, waves, cols, vec, parity,    \
+                                          t_first, write_ez, halo, rows, s);
+    switch (depth) {
+        FDTD_BATCH_CASE(1) FDTD_BATCH_CASE(2) FDTD_BATCH_CASE(3) FDTD_BATCH_CASE(4)
+        FDTD_BATCH_CASE(5) FDTD_BATCH_CASE(6) FDTD_BATCH_CASE(7) FDTD_BATCH_CASE(8)
+    default: return fail("temporal depth %d not built", depth);
+    }
+#undef FDTD_BATCH_CASE
+}
+
+int fd2d_batch_resident(int depth, int cols, bool vec, int *waves)
+{
+    switch (depth) {
+    case 1: return fd2d_batch_resident_depth<1>(cols, vec, waves);
+    case 2: return fd2d_batch_resident_depth<2>(cols, vec, waves);
+    case 3: return fd2d_batch_resident_depth<3>(cols, vec, waves);
+    case 4: return fd2d_batch_resident_depth<4>(cols, vec, waves);
+    case 5: return fd2d_batch_resident_depth<5>(cols, vec, waves);
+    case 6: return fd2d_batch_resident_depth<6>(cols, vec, waves);
+    case 7: return fd2d_batch_resident_depth<7>(cols, vec, waves);
+    case 8: return fd2d_batch_resident_depth<8>(cols, vec, waves);
+    default: return fail("temporal depth %d not built", depth);
+    }
+}
+
 }  // namespace fdtd

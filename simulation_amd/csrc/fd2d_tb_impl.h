@@ -412,7 +412,120 @@ int launch_part(const TbParams &p, bool lossy, bool pml, hipStream_t s)
     return launch_kv<K, C, false, PART>(p, lossy, pml, s);
 }
 
+// ----------------------------------------------------------- batched grids
+// A persistent kernel over a work list of (grid, strip, row block) units of
+// many independent grids (fd2d_grid records in device memory); each unit is
+// swept exactly as the single-grid kernels sweep it.  Lossy PML form only.
+template <int K, int C, bool VEC>
+__global__ void __launch_bounds__(64)
+tm_tb_batch_kernel(const fd2d_grid *__restrict__ grids, const int *__restrict__ units, int nunits,
+                   int parity, int t_first, int write_ez, int halo, int rows)
+{
+    constexpr int S = 64 * C;
+    for (int u = blockIdx.x; u < nunits; u += gridDim.x) {
+        const int gi = units[4 * u], sx = units[4 * u + 1], by = units[4 * u + 2];
+        const fd2d_grid &G = grids[gi];
+        TbParams P;
+        P.nx = G.nx;
+        P.ny = G.ny;
+        const bool swap = parity != 0;        // odd launches read the partner set
+        P.dz_i = swap ? G.dz2 : G.dz;   P.dz_o = swap ? G.dz : G.dz2;
+        P.iz_i = swap ? G.iz2 : G.iz;   P.iz_o = swap ? G.iz : G.iz2;
+        P.hx_i = swap ? G.hx2 : G.hx;   P.hx_o = swap ? G.hx : G.hx2;
+        P.hy_i = swap ? G.hy2 : G.hy;   P.hy_o = swap ? G.hy : G.hy2;
+        P.ihx_i = swap ? G.ihx2 : G.ihx; P.ihx_o = swap ? G.ihx : G.ihx2;
+        P.ihy_i = swap ? G.ihy2 : G.ihy; P.ihy_o = swap ? G.ihy : G.ihy2;
+        P.ez_o = write_ez ? G.ez : nullptr;
+        P.naz = G.md.naz;
+        P.nbz = G.md.nbz;
+        P.pml = PmlPtrs{G.pml.fx1, G.pml.fx2, G.pml.fx3, G.pml.fy1, G.pml.fy2, G.pml.fy3,
+                        G.pml.gx2, G.pml.gx3, G.pml.gy2, G.pml.gy3};
+        P.halo = halo;
+        P.rows = rows;
+        const int *br = G.band_rows, *bc = G.band_cols;
+        P.rlo = br[1] > br[0] ? max(br[0], 1) : 0;
+        P.rhi = br[1] > br[0] ? min(br[1], G.nx - 1) : 0;
+        P.clo = bc[1] > bc[0] ? max(bc[0], 1) : 0;
+        P.chi = bc[1] > bc[0] ? min(bc[1], G.ny - 1) : 0;
+        P.t_first = t_first;
+        P.nsrc = G.src_i >= 0 ? 1 : 0;
+        P.src_i[0] = G.src_i;
+        P.src_j[0] = G.src_j;
+        P.src_wave = G.src_wave;
+        P.wave_len = G.wave_len;
+        P.nprobe = G.probe_i >= 0 ? 1 : 0;
+        P.probe_i[0] = G.probe_i;
+        P.probe_j[0] = G.probe_j;
+        P.probe_out = G.probe_out;
+        P.probe_len = G.probe_len;
+        const int js = sx * (S - 2 * halo) - halo;
+        Geom g;
+        g.j0 = js + (int)threadIdx.x * C;
+        g.i0 = by * rows;
+        g.i1 = min(g.i0 + rows, P.nx);
+        g.vj0 = js + halo;
+        g.vj1 = min(js + S - halo, P.ny);
+        const bool brow = !(g.i0 - 2 * K >= P.rlo && g.i1 - 1 + K < P.rhi);
+        const bool bcol = !(js >= P.clo && js + S <= P.chi);
+        bool sp = false;
+        if (P.nsrc)
+            sp |= P.src_i[0] >= g.i0 - 2 * K && P.src_i[0] <= g.i1 + K && P.src_j[0] >= js &&
+                  P.src_j[0] < js + S;
+        if (P.nprobe)
+            sp |= P.probe_i[0] >= g.i0 && P.probe_i[0] < g.i1 && P.probe_j[0] >= js &&
+                  P.probe_j[0] < js + S;
+        if (!brow && !bcol && !sp) sweep<K, C, true, true, VEC, false, false, false>(P, g);
+        else sweep<K, C, true, true, VEC, true, true, true>(P, g);
+    }
+}
+
+template <int K, int C, bool VEC>
+int launch_batch_kv(const fd2d_grid *grids, const int *units, int nunits, int waves, int parity,
+                    int t_first, int write_ez, int halo, int rows, hipStream_t s)
+{
+    hipLaunchKernelGGL((tm_tb_batch_kernel<K, C, VEC>), dim3(waves), dim3(64), 0, s, grids, units,
+                       nunits, parity, t_first, write_ez, halo, rows);
+    return hip_check(hipGetLastError(), "fd2d_batch_run launch");
+}
+
+template <int K, int C, bool VEC>
+int batch_resident_kv(int *waves)
+{
+    int per_cu = 0, dev = 0, ncu = 0;
+    int rc = hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                           &per_cu, (const void *)tm_tb_batch_kernel<K, C, VEC>, 64, 0),
+                       "occupancy query");
+    if (!rc) rc = hip_check(hipGetDevice(&dev), "hipGetDevice");
+    if (!rc) rc = hip_check(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev),
+                            "CU count");
+    if (!rc) *waves = per_cu * ncu;
+    return rc;
+}
+
 }  // namespace tb
+
+template <int K>
+int launch_fd2d_batch_depth(const fd2d_grid *grids, const int *units, int nunits, int waves,
+                            int cols, bool vec, int parity, int t_first, int write_ez, int halo,
+                            int rows, hipStream_t s)
+{
+    if (cols == 2 && vec)
+        return tb::launch_batch_kv<K, 2, true>(grids, units, nunits, waves, parity, t_first, write_ez, halo, rows, s);
+    if (cols == 2)
+        return tb::launch_batch_kv<K, 2, false>(grids, units, nunits, waves, parity, t_first, write_ez, halo, rows, s);
+    if (cols == 1)
+        return tb::launch_batch_kv<K, 1, false>(grids, units, nunits, waves, parity, t_first, write_ez, halo, rows, s);
+    return fail("columns per lane %d not built", cols);
+}
+
+template <int K>
+int fd2d_batch_resident_depth(int cols, bool vec, int *waves)
+{
+    if (cols == 2 && vec) return tb::batch_resident_kv<K, 2, true>(waves);
+    if (cols == 2) return tb::batch_resident_kv<K, 2, false>(waves);
+    if (cols == 1) return tb::batch_resident_kv<K, 1, false>(waves);
+    return fail("columns per lane %d not built", cols);
+}
 
 template <int K>
 int launch_fd2d_tb_depth(const TbParams &p, int cols, bool lossy, bool pml, int part, hipStream_t s)

@@ -1,6 +1,9 @@
-// Instantiation of the temporally blocked TM kernels for depth 6 (fd2d_tb_impl.h).
+// Instantiation of the temporally blocked TM kernels (single grid and batched) for depth 6 (fd2d_tb_impl.h).
 #include "fd2d_tb_impl.h"
 
 namespace fdtd {
 template int launch_fd2d_tb_depth<6>(const TbParams &, int, bool, bool, int, hipStream_t);
+template int launch_fd2d_batch_depth<6>(const fd2d_grid *, const int *, int, int, int, bool, int, int,
+                                        int, int, int, hipStream_t);
+template int fd2d_batch_resident_depth<6>(int, bool, int *);
 }  // namespace fdtd

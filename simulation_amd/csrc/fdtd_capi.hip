@@ -8,6 +8,8 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <algorithm>
+#include <vector>
 
 #include "fdtd_internal.h"
 
@@ -152,6 +154,69 @@ int aux_for_current_device(Aux **out)
         if (rc) return rc;
     }
     *out = &a;
+    return 0;
+}
+
+// ------------------------------------------------------- batched grids
+
+struct BatchGeom {
+    int depth, rows, cols, halo;
+};
+
+int batch_geom(int depth, int rows, int cols, BatchGeom *g)
+{
+    g->depth = depth ? depth : 8;
+    g->cols = cols ? cols : 1;
+    g->rows = rows ? rows : 32;
+    if (!tb_supported(g->depth, g->cols))
+        return fail("fd2d_batch: depth=%d cols=%d not supported", g->depth, g->cols);
+    g->halo = ((g->depth + g->cols - 1) / g->cols) * g->cols;
+    if (64 * g->cols - 2 * g->halo < 8) return fail("fd2d_batch: depth too large for the strip");
+    if (g->rows < 1) return fail("fd2d_batch: rows=%d", g->rows);
+    return 0;
+}
+
+int check_grid(const fd2d_grid &G, int k)
+{
+    if (G.nx < 2 || G.ny < 2) return fail("fd2d_batch: grid %d is %dx%d", k, G.nx, G.ny);
+    if ((long long)G.nx * G.ny >= (1LL << 30)) return fail("fd2d_batch: grid %d too large", k);
+    if (!G.dz || !G.iz || !G.ez || !G.hx || !G.hy || !G.ihx || !G.ihy || !G.dz2 || !G.iz2 ||
+        !G.hx2 || !G.hy2 || !G.ihx2 || !G.ihy2)
+        return fail("fd2d_batch: grid %d: both state sets are required", k);
+    if (!G.md.naz || !G.md.nbz || !pml_all(&G.pml))
+        return fail("fd2d_batch: grid %d: the lossy PML form (naz, nbz, ten profiles) is required", k);
+    if (G.src_i >= 0 && (G.src_i >= G.nx || G.src_j < 0 || G.src_j >= G.ny || !G.src_wave))
+        return fail("fd2d_batch: grid %d: source outside the grid or without a waveform", k);
+    if (G.probe_i >= 0 && (G.probe_i >= G.nx || G.probe_j < 0 || G.probe_j >= G.ny || !G.probe_out))
+        return fail("fd2d_batch: grid %d: probe outside the grid or without output", k);
+    return 0;
+}
+
+// launches of one batched run (shared by fd2d_batch_run and the graph capture)
+int batch_launches(const fd2d_batch_desc *b, int t0, int nsteps, int max_wave_len,
+                   int max_probe_len, hipStream_t s)
+{
+    if (!b || !b->grids_dev || !b->units_dev || b->nunits <= 0 || b->waves <= 0)
+        return fail("fd2d_batch_run: incomplete batch descriptor");
+    if (!tb_supported(b->depth, b->cols)) return fail("fd2d_batch_run: depth/cols");
+    if (t0 < 0 || nsteps < 0) return fail("fd2d_batch_run: t0=%d nsteps=%d", t0, nsteps);
+    if (t0 + nsteps > max_wave_len || t0 + nsteps > max_probe_len)
+        return fail("fd2d_batch_run: steps up to %d exceed a waveform or probe length", t0 + nsteps);
+    if (nsteps == 0) return 0;
+    const int D = b->depth;
+    int L = (nsteps + D - 1) / D;
+    if (L % 2) L += 1;
+    if (L > nsteps)
+        return fail("fd2d_batch_run: %d steps cannot be split into an even number of launches", nsteps);
+    const int halo = ((D + b->cols - 1) / b->cols) * b->cols;
+    int done = 0;
+    for (int l = 0; l < L; ++l) {
+        const int k = nsteps / L + (l < nsteps % L ? 1 : 0);
+        int rc = launch_fd2d_batch(b->grids_dev, b->units_dev, b->nunits, b->waves, k, b->cols,
+                                   b->vec != 0, l % 2, t0 + done + 1, l == L - 1, halo, b->rows, s);
+        if (rc) return rc;
+        done += k;
+    }
     return 0;
 }
 
@@ -509,6 +574,142 @@ int fd2d_tm_run(const fd2d_tm_desc *d, int t0, int nsteps, void *stream)
         }
     }
     return 0;
+}
+
+// ------------------------------------------------------------ batches
+
+int fd2d_batch_units(const fd2d_grid *grids, int ngrids, int depth, int rows, int cols,
+                     int *nunits)
+{
+    clear_error();
+    if (!grids || ngrids <= 0 || !nunits) return fail("fd2d_batch_units: grids/ngrids/nunits");
+    BatchGeom g;
+    int rc = batch_geom(depth, rows, cols, &g);
+    if (rc) return rc;
+    const int w = 64 * g.cols - 2 * g.halo;
+    long long n = 0;
+    for (int k = 0; k < ngrids; ++k) {
+        if (grids[k].nx < 2 || grids[k].ny < 2) return fail("fd2d_batch_units: grid %d size", k);
+        n += (long long)((grids[k].ny + w - 1) / w) * ((grids[k].nx + g.rows - 1) / g.rows);
+    }
+    if (n > (1LL << 30)) return fail("fd2d_batch_units: too many units");
+    *nunits = (int)n;
+    return 0;
+}
+
+int fd2d_batch_upload(const fd2d_grid *grids, int ngrids, int depth, int rows, int cols,
+                      int waves, fd2d_grid *grids_dev, int *units_dev, int max_units,
+                      fd2d_batch_desc *out, void *stream)
+{
+    clear_error();
+    if (!grids || ngrids <= 0 || !grids_dev || !units_dev || !out)
+        return fail("fd2d_batch_upload: grids, grids_dev, units_dev and out are required");
+    BatchGeom g;
+    int rc = batch_geom(depth, rows, cols, &g);
+    if (rc) return rc;
+    bool vec = g.cols == 2;
+    for (int k = 0; k < ngrids; ++k) {
+        rc = check_grid(grids[k], k);
+        if (rc) return rc;
+        vec = vec && (grids[k].ny % 2 == 0);
+    }
+    const int w = 64 * g.cols - 2 * g.halo;
+    // work list: units holding a grid edge, PML band, source or probe first
+    // (their body is the slower one), then the interior units
+    std::vector<int> edge, inner;
+    for (int k = 0; k < ngrids; ++k) {
+        const fd2d_grid &G = grids[k];
+        const int ns = (G.ny + w - 1) / w, nb = (G.nx + g.rows - 1) / g.rows;
+        const int rlo = G.band_rows[1] > G.band_rows[0] ? std::max(G.band_rows[0], 1) : 0;
+        const int rhi = G.band_rows[1] > G.band_rows[0] ? std::min(G.band_rows[1], G.nx - 1) : 0;
+        const int clo = G.band_cols[1] > G.band_cols[0] ? std::max(G.band_cols[0], 1) : 0;
+        const int chi = G.band_cols[1] > G.band_cols[0] ? std::min(G.band_cols[1], G.ny - 1) : 0;
+        for (int by = 0; by < nb; ++by) {
+            for (int sx = 0; sx < ns; ++sx) {
+                const int js = sx * w - g.halo, i0 = by * g.rows;
+                const int i1 = std::min(i0 + g.rows, G.nx);
+                const bool brow = !(i0 - 2 * g.depth >= rlo && i1 - 1 + g.depth < rhi);
+                const bool bcol = !(js >= clo && js + 64 * g.cols <= chi);
+                const bool sp = (G.src_i >= 0 && G.src_i >= i0 - 2 * g.depth && G.src_i <= i1 + g.depth &&
+                                 G.src_j >= js && G.src_j < js + 64 * g.cols) ||
+                                (G.probe_i >= 0 && G.probe_i >= i0 && G.probe_i < i1 &&
+                                 G.probe_j >= js && G.probe_j < js + 64 * g.cols);
+                auto &v = (brow || bcol || sp) ? edge : inner;
+                v.insert(v.end(), {k, sx, by, 0});
+            }
+        }
+    }
+    const int n = (int)((edge.size() + inner.size()) / 4);
+    if (n > max_units) return fail("fd2d_batch_upload: %d units exceed max_units=%d", n, max_units);
+    edge.insert(edge.end(), inner.begin(), inner.end());
+    rc = hip_check(hipMemcpyAsync(units_dev, edge.data(), edge.size() * sizeof(int),
+                                  hipMemcpyHostToDevice, S(stream)), "units upload");
+    if (!rc) rc = hip_check(hipMemcpyAsync(grids_dev, grids, (size_t)ngrids * sizeof(fd2d_grid),
+                                           hipMemcpyHostToDevice, S(stream)), "grids upload");
+    if (!rc) rc = hip_check(hipStreamSynchronize(S(stream)), "upload sync");
+    if (rc) return rc;
+    if (waves <= 0) {
+        rc = fd2d_batch_resident(g.depth, g.cols, vec, &waves);
+        if (rc) return rc;
+    }
+    out->ngrids = ngrids;
+    out->grids_dev = grids_dev;
+    out->units_dev = units_dev;
+    out->nunits = n;
+    out->depth = g.depth;
+    out->rows = g.rows;
+    out->cols = g.cols;
+    out->waves = std::max(1, std::min(waves, n));
+    out->vec = vec ? 1 : 0;
+    return 0;
+}
+
+int fd2d_batch_run(const fd2d_batch_desc *b, int t0, int nsteps, int max_wave_len,
+                   int max_probe_len, void *stream)
+{
+    clear_error();
+    return batch_launches(b, t0, nsteps, max_wave_len, max_probe_len, S(stream));
+}
+
+int fd2d_batch_graph(const fd2d_batch_desc *b, int t0, int nsteps, int max_wave_len,
+                     int max_probe_len, void *stream, void **graph_exec)
+{
+    clear_error();
+    if (!stream) return fail("fd2d_batch_graph: capture needs a stream other than the null stream");
+    if (!graph_exec) return fail("fd2d_batch_graph: graph_exec is required");
+    hipStream_t s = S(stream);
+    int rc = hip_check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin capture");
+    if (rc) return rc;
+    int lrc = batch_launches(b, t0, nsteps, max_wave_len, max_probe_len, s);
+    hipGraph_t graph = nullptr;
+    rc = hip_check(hipStreamEndCapture(s, &graph), "end capture");
+    if (lrc) {
+        if (graph) hipGraphDestroy(graph);
+        return lrc;
+    }
+    if (rc) return rc;
+    hipGraphExec_t exec = nullptr;
+    rc = hip_check(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "graph instantiate");
+    hipGraphDestroy(graph);
+    if (rc) return rc;
+    *graph_exec = exec;
+    return 0;
+}
+
+int fdtd_graph_launch(void *graph_exec, void *stream)
+{
+    clear_error();
+    if (!graph_exec) return fail("fdtd_graph_launch: graph_exec is NULL");
+    return hip_check(hipGraphLaunch(reinterpret_cast<hipGraphExec_t>(graph_exec), S(stream)),
+                     "graph launch");
+}
+
+int fdtd_graph_destroy(void *graph_exec)
+{
+    clear_error();
+    if (!graph_exec) return 0;
+    return hip_check(hipGraphExecDestroy(reinterpret_cast<hipGraphExec_t>(graph_exec)),
+                     "graph destroy");
 }
 
 }  // extern "C"
