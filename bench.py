@@ -53,10 +53,20 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=("tm", "batch"), default="tm",
-                    help="tm: the N x N configuration (default); batch: 256 uneven grids")
+    ap.add_argument("--workload", choices=("tm", "large", "batch"), default="tm",
+                    help="tm: the N x N single-cylinder configuration (default, N=4096); "
+                         "large: 16384^2, 16 cylinders, 4 sources, 5000 steps; "
+                         "batch: 256 uneven grids")
     ap.add_argument("--graph", action="store_true", help="batch: time one captured graph launch")
-    return ap.parse_args()
+    ap.add_argument("--tune", action="store_true",
+                    help="choose depth/rows/cols with the autotuner before timing")
+    a = ap.parse_args()
+    if a.workload == "large":
+        if a.size == 4096:
+            a.size = 16384
+        if a.steps == 2000:
+            a.steps = 5000
+    return a
 
 
 def cpu_baseline(n: int, budget_s: float) -> dict:
@@ -197,9 +207,22 @@ def main():
 
     dev = torch.device("cuda", 0)
     n, K, W = a.size, a.steps, a.warmup
-    cfg = fd2d.c2_config(n, W + K)
+    if a.workload == "large":
+        cfg = fd2d.c5_config(n, W + K)
+        workload = f"fd2d_tm_{n}x{n}_npml8_16cyl_4src"
+        desc = (f"2-D TM {n}x{n} FP32, npml=8 Sullivan PML, 16 seeded lossy cylinders, "
+                f"4 hard Gaussian point sources, 4 probes")
+    else:
+        cfg = fd2d.c2_config(n, W + K)
+        workload = f"fd2d_tm_{n}x{n}_npml8_cyl400"
+        desc = (f"2-D TM {n}x{n} FP32, npml=8 Sullivan PML, lossy cylinder r=400 "
+                f"eps_r=4 sigma=0.02, hard Gaussian t0=20 spread=6 at centre, 4 probes")
     cfg.depth, cfg.rows, cfg.cols = a.depth, a.rows, a.cols
     s = fd2d.build_solver(cfg, dev)
+    tuned = None
+    if a.tune:
+        from simulation_amd import autotune
+        tuned = autotune.tune(s)
     plan = s.plan(K)
     s.run(W)
     torch.cuda.synchronize()
@@ -219,7 +242,6 @@ def main():
     launch_ms = ev_ms / max(plan["launches"], 1)
     bytes_per_launch = B_ALG * cells * K / max(plan["launches"], 1)
     achieved = bytes_per_launch / (launch_ms * 1e-3) / 1e9
-    workload = f"fd2d_tm_{n}x{n}_npml8_cyl400_2000steps"
     rec = {
         "metric": METRIC,
         "value": round(value, 1),
@@ -232,13 +254,13 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic (fields start at zero; Gaussian point source; seeded geometry)",
+        "data": "synthetic (fields start at zero; Gaussian point sources; seeded geometry)",
         "config": {
-            "workload": f"2-D TM {n}x{n} FP32, npml=8 Sullivan PML, lossy cylinder r=400 "
-                        f"eps_r=4 sigma=0.02, hard Gaussian t0=20 spread=6 at centre, 4 probes",
+            "workload": desc,
             "grid": [n, n],
             "depth": plan["depth"], "rows": plan["rows"], "cols": plan["cols"],
             "launches": plan["launches"],
+            "autotuned": tuned,
         },
         "roofline": {
             "bound": "hbm",
@@ -246,7 +268,7 @@ def main():
             "peak": HBM_PEAK,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK, 4),
-            "traffic": traffic_for(workload),
+            "traffic": traffic_for(f"{workload}_d{plan['depth']}_r{plan['rows']}_c{plan['cols']}"),
             "kernel": "tm_tb_kernel",
             "avg_launch_ms": round(launch_ms, 4),
             "alg_bytes_per_launch": int(bytes_per_launch),

@@ -224,7 +224,8 @@ typedef struct fd2d_tm_desc {
 int fd2d_tm_run(const fd2d_tm_desc *d, int t0, int nsteps, void *stream);
 
 /* The launch plan fd2d_tm_run would use for this descriptor and nsteps:
- * number of launches and the chosen depth/rows/cols.  Host only.            */
+ * number of launches and the chosen depth/rows/cols (defaults: depth 8, one
+ * column per lane, rows = nx/220 clamped to [16, 512]).  Host only.         */
 int fd2d_tm_plan(const fd2d_tm_desc *d, int nsteps,
                  int *launches, int *depth, int *rows, int *cols);
 

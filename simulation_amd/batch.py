@@ -42,7 +42,8 @@ class GridSpec:
 def c3_specs(seed: int = 0, count: int = 256, lo: int = 128, hi: int = 1024, npml: int = 8):
     """The batched configuration: `count` square grids with sides uniform in
     [lo, hi], 1-5 lossy cylinders each (eps_r U[1,8], sigma U[0,0.05], radius
-    U[4%, 20%] of the side, centre inside the non-PML region), npml=8, and a
+    U[4%, 20%] of the side, whole disc at least 2 cells clear of the PML),
+    npml=8, and a
     hard Gaussian source of spread U[4,12] (t0 = 20*spread/6, the ratio of
     fd2d/pycuda/fd2d_3_1.py:69) at a random interior point, probed 10 cells
     away.  Deterministic in `seed`."""
@@ -54,8 +55,8 @@ def c3_specs(seed: int = 0, count: int = 256, lo: int = 128, hi: int = 1024, npm
         cyl = []
         for _ in range(int(rng.integers(1, 6))):
             r = float(rng.uniform(0.04, 0.2) * n)
-            cx = float(rng.uniform(npml + r * 0.5, n - npml - r * 0.5))
-            cy = float(rng.uniform(npml + r * 0.5, n - npml - r * 0.5))
+            cx = float(rng.uniform(npml + r + 2, n - npml - r - 2))
+            cy = float(rng.uniform(npml + r + 2, n - npml - r - 2))
             cyl.append((cx, cy, r, float(rng.uniform(1.0, 8.0)), float(rng.uniform(0.0, 0.05))))
         spread = float(rng.uniform(4.0, 12.0))
         si = int(rng.integers(npml + 2, npml + inner - 12))
@@ -113,17 +114,34 @@ class BatchSolver:
                 g.probe_i, g.probe_j, g.probe_len = -1, 0, 0
             st.update(pml=pml, md=md, wave=wave, probe=probe)
             self.state.append(st)
+        self.grids_dev = torch.empty(n * C.sizeof(_lib.Fd2dGrid), dtype=torch.uint8, device=dev)
+        self.wave_len = self.nsteps_max if any(s.source for s in self.specs) else INT_MAX
+        self.probe_len = self.nsteps_max if any(s.probe for s in self.specs) else INT_MAX
+        self._graphs = {}
+        self.plan(depth, rows, cols, waves)
+
+    def plan(self, depth: int = 0, rows: int = 0, cols: int = 0, waves: int = 0) -> None:
+        """(Re)build the work list for a launch shape (0 = library default).
+        Captured graphs of the previous shape are destroyed."""
+        self.close()
+        n = len(self.specs)
         nunits = C.c_int()
         call("fd2d_batch_units", self.grids, n, depth, rows, cols, C.byref(nunits))
-        self.units = torch.empty(4 * nunits.value, dtype=torch.int32, device=dev)
-        self.grids_dev = torch.empty(n * C.sizeof(_lib.Fd2dGrid), dtype=torch.uint8, device=dev)
+        self.units = torch.empty(4 * nunits.value, dtype=torch.int32, device=self.device)
         self.desc = _lib.Fd2dBatchDesc()
         call("fd2d_batch_upload", self.grids, n, depth, rows, cols, waves,
              self.grids_dev.data_ptr(), self.units.data_ptr(), nunits.value,
              C.byref(self.desc), stream())
-        self.wave_len = self.nsteps_max if any(s.source for s in self.specs) else INT_MAX
-        self.probe_len = self.nsteps_max if any(s.probe for s in self.specs) else INT_MAX
-        self._graphs = {}
+
+    def reset(self) -> None:
+        """All fields and probe traces back to zero, step counter to 0."""
+        for st in self.state:
+            for name in ("dz", "iz", "ez", "hx", "hy", "ihx", "ihy",
+                         "dz2", "iz2", "hx2", "hy2", "ihx2", "ihy2"):
+                st[name].zero_()
+            if st["probe"] is not None:
+                st["probe"].zero_()
+        self.t = 0
 
     @property
     def cells(self) -> int:

@@ -102,14 +102,15 @@ struct Plan {
 int plan_tm(const fd2d_tm_desc *d, int nsteps, Plan *pl)
 {
     int D = d->depth ? d->depth : 8;
-    int C = d->cols ? d->cols : 2;
+    int C = d->cols ? d->cols : 1;
     if (!tb_supported(D, C)) return fail("fd2d_tm: depth=%d cols=%d not supported", D, C);
     if (64 * C - 2 * D < 8) return fail("fd2d_tm: depth %d too large for %d columns/lane", D, C);
     int H = d->rows;
     if (H == 0) {
-        // default from measurements on MI355X (4096^2 .. 16384^2): ~110 row
-        // blocks per strip, i.e. several waves per SIMD over the launch
-        long h = d->nx / 110;
+        // default from autotuner runs on MI355X (8192^2: depth 8, one column
+        // per lane, ~220 row blocks per strip -- many short waves, so the
+        // slower edge waves are spread over more of the launch)
+        long h = d->nx / 220;
         if (h < 16) h = 16;
         if (h > 512) h = 512;
         H = (int)h;

@@ -284,6 +284,29 @@ def c2_config(n: int = 4096, ns: int = 2000) -> TMConfig:
                     snap_every=500 if ns >= 500 else 0)
 
 
+def c5_config(n: int = 16384, ns: int = 5000, seed: int = 5) -> TMConfig:
+    """Large single grid: n x n, npml=8, 16 seeded lossy cylinders (radius
+    2-6% of the side, eps_r U[1,8], sigma U[0,0.05]) and 4 seeded hard
+    Gaussian point sources (spread U[4,12], t0 = round(20 spread / 6)), one
+    probe beside each source."""
+    rng = np.random.default_rng(seed)
+    lo, hi = n // 8, n - n // 8
+    cyl = []
+    for _ in range(16):
+        r = float(rng.uniform(0.02, 0.06) * n)
+        cx, cy = (float(v) for v in rng.uniform(lo, hi, 2))
+        cyl.append((round(cx * 2) / 2, round(cy * 2) / 2, round(r * 2) / 2,
+                    float(rng.uniform(1.0, 8.0)), float(rng.uniform(0.0, 0.05))))
+    src, probes = [], []
+    for _ in range(4):
+        i, j = (int(v) for v in rng.integers(lo, hi, 2))
+        spread = float(rng.uniform(4.0, 12.0))
+        src.append((i, j, int(round(20 * spread / 6)), spread))
+        probes.append((i + 10, j))
+    return TMConfig(nx=n, ny=n, ns=ns, npml=8, cylinders=cyl, sources=src, probes=probes,
+                    snap_every=0)
+
+
 def build_solver(cfg: TMConfig, device=None) -> TMSolver:
     dev = device or require_gpu()
     pml = None
