@@ -32,6 +32,9 @@
 #ifndef FDTD_H
 #define FDTD_H
 
+#include <stddef.h>
+#include <stdint.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -291,6 +294,89 @@ int fd2d_batch_graph(const fd2d_batch_desc *b, int t0, int nsteps, int max_wave_
                      int max_probe_len, void *stream, void **graph_exec);
 int fdtd_graph_launch(void *graph_exec, void *stream);
 int fdtd_graph_destroy(void *graph_exec);
+
+/* ------------------------------------------------ learned absorbing boundary
+ * No reference counterpart: the reference absorbs with the two-step 1-D ABC
+ * (fd1d/pycuda/fd1d_1_2.py:53-61) and the Sullivan PML
+ * (fd2d/pycuda/fd2d_3_2.py:126-135); this replaces a thick PML with a small
+ * network applied at the outer three cells of every edge.
+ *
+ * Network: 108 features (the last four canonical boundary samples of a
+ * position, newest first, 27 values each) -> 64 -> 64 -> 3, GELU, bf16
+ * weights and activations, FP32 accumulation (MFMA).  With s = max|x|:
+ *   ez_next[d] = ez_now[d] + s * y[d],  y = net(bf16(x / s)),  d = 0..2.
+ * Parameters: one FP32 vector of lbc_param_count() values, in the order
+ * W1 (64x108, row-major), b1 (64), W2 (64x64), b2, W3 (3x64), b3; the
+ * kernels read bf16 copies from a buffer of lbc_wbuf_elems() uint16 values
+ * written by lbc_pack (and kept current by lbc_adam).
+ *
+ * Canonical sample (27 values, index (f*3 + d)*3 + o + 1 for field f, depth
+ * d = 0..2 from the edge, tangential offset o = -1..1, tangential index
+ * clamped to the edge); f = 0 ez, 1 normal-coupled H, 2 tangential-coupled H:
+ *   top    (i = d):      ez[d,t],      hy[d,t],       hx[d,t]
+ *   bottom (i = nx-1-d): ez[nx-1-d,t], -hy[nx-2-d,t], hx[nx-1-d,t]
+ *   left   (j = d):      ez[t,d],      -hx[t,d],      -hy[t,d]
+ *   right  (j = ny-1-d): ez[t,ny-1-d], hx[t,ny-2-d],  -hy[t,ny-1-d]
+ * with the tangential-coupled value 0 at depth 0 and at offset +1 (it is
+ * staggered half a cell along the edge; offsets -1 and 0 are the two points
+ * beside the cell).  Positions: top then
+ * bottom (t in [0, ny), owning the corners), then left and right (t in
+ * [3, nx-3)): lbc_npos = 2 ny + 2 (nx - 6).                                 */
+int lbc_param_count(void);
+int lbc_wbuf_elems(void);
+size_t lbc_train_smem(void);
+
+/* Canonical samples of an nx x ny window (row stride ld) of ez/hx/hy:
+ * out[lbc_npos][27].                                                        */
+int lbc_sample(int nx, int ny, int ld, const float *ez, const float *hx, const float *hy,
+               float *out, void *stream);
+
+/* bf16 operand buffer from the FP32 parameters.                             */
+int lbc_pack(const float *params, uint16_t *wbuf, void *stream);
+
+/* Rows of features given as samples in device memory: row r's sample k
+ * (k = 0 newest .. 3) starts at src + offs[r] - k*tstride.  Writes the
+ * normalised outputs y[r][3] and (if s is not NULL) the scales s[r].        */
+int lbc_forward(int n, const float *src, const long long *offs, long long tstride,
+                const float *params, const uint16_t *wbuf, float *y, float *s, void *stream);
+
+/* Training gradient of  sum_r w_r |y_r - (T_r - ez_now_r)/s_r|^2 / norm
+ * (w_r = 0 for all-zero rows), targets T_r = tgt[r][3] or, with tgt NULL,
+ * the next sample's ez (src + offs[r] + tstride, depths 0..2, offset 0).
+ * nblocks workgroups each write one partial gradient (lbc_param_count
+ * floats) to part[block] and its loss sum to part_loss[block]; lbc_adam
+ * reduces them in a fixed order (deterministic).                            */
+int lbc_train_grad(int n, const float *src, const long long *offs, long long tstride,
+                   const float *tgt, float norm, const float *params, const uint16_t *wbuf,
+                   int nblocks, float *part, float *part_loss, void *stream);
+
+/* Time stepping with the learned boundary.  The TM descriptor gives the grid
+ * without a PML (pml fields NULL; md.nbz optional: lossy or lossless), the
+ * partner state set, sources and probes (all at least fd2d_lbc_margin()
+ * cells from every edge) and the launch shape of the blocked kernel (depth
+ * at most 8).  Each launch of k <= depth steps is the blocked launch with PEC
+ * edges followed by a frame kernel that recomputes the outer k+3 cells with
+ * the network applied every step (the blocked launch is exact deeper in).
+ * hist, hist2: [lbc_hist_elems(nx, ny)] floats each, the last four boundary
+ * samples of every position (zero for fields at rest); the run leaves them
+ * current in hist.  nsteps >= 2 (an even number of launches).               */
+#define FDTD_LBC_MARGIN 32
+typedef struct fd2d_lbc_desc {
+    const float *params;        /* lbc_param_count() FP32 parameters         */
+    const uint16_t *wbuf;       /* bf16 operands (lbc_pack)                  */
+    float *hist, *hist2;        /* boundary-sample history, ping-pong pair   */
+} fd2d_lbc_desc;
+size_t lbc_hist_elems(int nx, int ny);
+int fd2d_lbc_margin(void);
+int fd2d_lbc_run(const fd2d_tm_desc *d, const fd2d_lbc_desc *b, int t0, int nsteps,
+                 void *stream);
+
+/* Reduces the partials; writes the gradient to grad_out (if not NULL); with
+ * lr != 0 applies one Adam step (bias corrections for `step` >= 1) to params,
+ * m, v and refreshes the bf16 copies in wbuf.                               */
+int lbc_adam(int nblocks, const float *part, float *params, float *m, float *v, int step,
+             float lr, float beta1, float beta2, float eps, uint16_t *wbuf, float *grad_out,
+             void *stream);
 
 #ifdef __cplusplus
 }

@@ -27,6 +27,7 @@ import numpy as np
 from oracle import numpy_oracle as NO
 
 IN, HID, OUT = 108, 64, 3
+S_MIN = np.float32(1e-30)      # rows with max|x| <= S_MIN count as all-zero
 NSAMP = 27
 HIST = 4
 NPARAM = HID * IN + HID + HID * HID + HID + OUT * HID + OUT
@@ -97,11 +98,12 @@ def gelu_grad(z):
 
 def scale(X: np.ndarray):
     """Per-row scale s = max|x| and normalised bf16 inputs x*(1/s) (FP32
-    reciprocal and product, as the kernels compute them); rows with s = 0
-    give zero inputs."""
+    reciprocal and product, as the kernels compute them); rows with
+    s <= S_MIN give zero inputs."""
     X = np.asarray(X, np.float32)
     s = np.max(np.abs(X), axis=1).astype(np.float32)
-    inv = np.where(s > 0, np.float32(1.0) / np.where(s > 0, s, np.float32(1.0)), np.float32(0.0))
+    big = s > S_MIN
+    inv = np.where(big, np.float32(1.0) / np.where(big, s, np.float32(1.0)), np.float32(0.0))
     inv = inv.astype(np.float32)
     xh = bf16((X * inv[:, None]).astype(np.float32))
     return s, inv, xh
@@ -130,12 +132,12 @@ def forward(params, X, rounding: bool = True):
 def targets(X, T):
     """Normalised targets (T - ez_now) / s for raw features X and next-step
     ez T (B,3), FP32 as the kernels compute them, and the sample weights
-    (0 for all-zero rows)."""
+    (0 for rows with s <= S_MIN)."""
     X = np.asarray(X, np.float32)
     s, inv, _ = scale(X)
     ez0 = X[:, [sidx(0, d, 0) for d in range(OUT)]]
     th = ((np.asarray(T, np.float32) - ez0) * inv[:, None]).astype(np.float32)
-    w = (s > 0).astype(np.float32)
+    w = (s > S_MIN).astype(np.float32)
     return th, w
 
 
@@ -172,8 +174,8 @@ def adam(params, m, v, g, step: int, lr: float, b1=0.9, b2=0.999, eps=1e-8):
     """One Adam update in FP32 (the kernel's arithmetic order)."""
     f = np.float32
     g = np.asarray(g, f)
-    m[:] = f(b1) * m + f(1 - b1) * g
-    v[:] = f(b2) * v + f(1 - b2) * g * g
+    m[:] = f(b1) * m + (f(1) - f(b1)) * g
+    v[:] = f(b2) * v + (f(1) - f(b2)) * g * g
     c1 = f(1.0 - b1 ** step)
     c2 = f(1.0 - b2 ** step)
     params[:] = params - f(lr) * (m / c1) / (np.sqrt(v / c2) + f(eps))
@@ -201,8 +203,10 @@ def sample(ez, hx, hy, nx: int, ny: int) -> np.ndarray:
     left:   ez[t,d],      hn = -hx[t,d],        ht = -hy[t,d]
     right:  ez[t,ny-1-d], hn =  hx[t,ny-2-d],   ht = -hy[t,ny-1-d]
     ht at depth 0 is 0 on every edge (never updated on the bottom and right
-    edges, never read by an update on the other two); tangential indices are
-    clamped to the edge."""
+    edges, never read by an update on the other two), and ht is taken only at
+    offsets -1 and 0 -- the two H points beside the cell (ht is staggered by
+    half a cell along the edge), so a prediction reaches one cell to either
+    side; tangential indices are clamped to the edge."""
     pos = positions(nx, ny)
     out = np.zeros((len(pos), NSAMP), np.float32)
     for p, (e, t) in enumerate(pos):
@@ -220,7 +224,7 @@ def sample(ez, hx, hy, nx: int, ny: int) -> np.ndarray:
                     v = (ez[tt, ny - 1 - d], hx[tt, ny - 2 - d], -hy[tt, ny - 1 - d])
                 out[p, sidx(0, d, o)] = v[0]
                 out[p, sidx(1, d, o)] = v[1]
-                out[p, sidx(2, d, o)] = v[2] if d > 0 else 0.0
+                out[p, sidx(2, d, o)] = v[2] if (d > 0 and o < 1) else 0.0
     return out
 
 
@@ -237,7 +241,8 @@ def predict(params, hist):
     ez0 = X[:, [sidx(0, d, 0) for d in range(OUT)]]
     yf = y.astype(np.float32)
     pred = (ez0 + (s[:, None] * yf).astype(np.float32)).astype(np.float32)
-    pred[s == 0] = ez0[s == 0]
+    small = s <= S_MIN
+    pred[small] = ez0[small]
     return pred
 
 

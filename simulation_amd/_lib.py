@@ -79,6 +79,12 @@ class Fd2dGrid(C.Structure):
     ]
 
 
+class Fd2dLbcDesc(C.Structure):
+    """fd2d_lbc_desc: network and boundary-sample history (include/fdtd.h)."""
+    _fields_ = [("params", C.c_void_p), ("wbuf", C.c_void_p), ("hist", C.c_void_p),
+                ("hist2", C.c_void_p)]
+
+
 class Fd2dBatchDesc(C.Structure):
     _fields_ = [
         ("ngrids", C.c_int),
@@ -128,6 +134,17 @@ SIGNATURES = {
     "fd2d_batch_graph": (_I, [C.POINTER(Fd2dBatchDesc), _I, _I, _I, _I, _P, C.POINTER(C.c_void_p)]),
     "fdtd_graph_launch": (_I, [_P, _P]),
     "fdtd_graph_destroy": (_I, [_P]),
+    "lbc_hist_elems": (C.c_size_t, [_I, _I]),
+    "fd2d_lbc_margin": (_I, []),
+    "fd2d_lbc_run": (_I, [C.POINTER(Fd2dTmDesc), C.POINTER(Fd2dLbcDesc), _I, _I, _P]),
+    "lbc_param_count": (_I, []),
+    "lbc_wbuf_elems": (_I, []),
+    "lbc_train_smem": (C.c_size_t, []),
+    "lbc_sample": (_I, [_I, _I, _I, _P, _P, _P, _P, _P]),
+    "lbc_pack": (_I, [_P, _P, _P]),
+    "lbc_forward": (_I, [_I, _P, _P, C.c_longlong, _P, _P, _P, _P, _P]),
+    "lbc_train_grad": (_I, [_I, _P, _P, C.c_longlong, _P, _F, _P, _P, _I, _P, _P, _P]),
+    "lbc_adam": (_I, [_I, _P, _P, _P, _P, _I, _F, _F, _F, _F, _P, _P, _P]),
 }
 
 _lib = None

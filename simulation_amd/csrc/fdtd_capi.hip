@@ -577,6 +577,96 @@ int fd2d_tm_run(const fd2d_tm_desc *d, int t0, int nsteps, void *stream)
     return 0;
 }
 
+// ------------------------------------------------ learned boundary
+
+size_t lbc_hist_elems(int nx, int ny)
+{
+    if (nx < 8 || ny < 8) return 0;
+    return (size_t)(2 * ny + 2 * (nx - 6)) * 4 * 27;
+}
+
+int fd2d_lbc_margin(void) { return FDTD_LBC_MARGIN; }
+
+int fd2d_lbc_run(const fd2d_tm_desc *d, const fd2d_lbc_desc *b, int t0, int nsteps, void *stream)
+{
+    clear_error();
+    int rc = tm_validate(d, t0, nsteps);
+    if (rc) return rc;
+    if (!pml_none(&d->pml)) return fail("fd2d_lbc_run: the learned boundary replaces the pml (pass none)");
+    if (!b || !b->params || !b->wbuf || !b->hist || !b->hist2)
+        return fail("fd2d_lbc_run: params, wbuf, hist and hist2 are required");
+    if (d->nx < 64 || d->ny < 64) return fail("fd2d_lbc_run: grid %dx%d below 64x64", d->nx, d->ny);
+    const int M = FDTD_LBC_MARGIN;
+    for (int k = 0; k < d->nsrc; ++k)
+        if (d->src_i[k] < M || d->src_i[k] >= d->nx - M || d->src_j[k] < M || d->src_j[k] >= d->ny - M)
+            return fail("fd2d_lbc_run: source %d within %d cells of an edge", k, M);
+    for (int k = 0; k < d->nprobe; ++k)
+        if (d->probe_i[k] < M || d->probe_i[k] >= d->nx - M || d->probe_j[k] < M ||
+            d->probe_j[k] >= d->ny - M)
+            return fail("fd2d_lbc_run: probe %d within %d cells of an edge", k, M);
+    if (nsteps == 0) return 0;
+    if (nsteps < 2) return fail("fd2d_lbc_run: nsteps=%d (at least 2: launches come in pairs)", nsteps);
+    Plan pl;
+    rc = plan_tm(d, 2 * 8, &pl);
+    if (rc) return rc;
+    if (pl.depth > 8) return fail("fd2d_lbc_run: depth %d above 8", pl.depth);
+    const bool lossy = d->md.nbz != nullptr;
+    if (!d->dz2 || !d->hx2 || !d->hy2 || (lossy && !d->iz2))
+        return fail("fd2d_lbc_run: the partner state set (dz2, iz2, hx2, hy2) is required");
+    int L = (nsteps + pl.depth - 1) / pl.depth;
+    L += L % 2;
+    TbParams p{};
+    p.nx = d->nx; p.ny = d->ny;
+    p.naz = d->md.naz; p.nbz = d->md.nbz;
+    p.halo = pl.halo; p.rows = pl.rows;
+    p.rlo = 1; p.rhi = d->nx - 1; p.clo = 1; p.chi = d->ny - 1;
+    p.nsrc = d->nsrc;
+    for (int k = 0; k < d->nsrc; ++k) { p.src_i[k] = d->src_i[k]; p.src_j[k] = d->src_j[k]; }
+    p.src_wave = d->src_wave; p.wave_len = d->wave_len;
+    p.nprobe = d->nprobe;
+    for (int k = 0; k < d->nprobe; ++k) { p.probe_i[k] = d->probe_i[k]; p.probe_j[k] = d->probe_j[k]; }
+    p.probe_out = d->probe_out; p.probe_len = d->probe_len;
+    float *A[4] = {d->dz, d->iz, d->hx, d->hy};
+    float *B[4] = {d->dz2, d->iz2, d->hx2, d->hy2};
+    float *H[2] = {b->hist, b->hist2};
+    Aux *ax = nullptr;
+    rc = aux_for_current_device(&ax);
+    if (rc) return rc;
+    int done = 0;
+    for (int l = 0; l < L; ++l) {
+        const int k = nsteps / L + (l < nsteps % L ? 1 : 0);
+        float **in = (l % 2 == 0) ? A : B;
+        float **out = (l % 2 == 0) ? B : A;
+        p.dz_i = in[0]; p.iz_i = in[1]; p.hx_i = in[2]; p.hy_i = in[3];
+        p.ihx_i = p.ihy_i = nullptr;
+        p.dz_o = out[0]; p.iz_o = out[1]; p.hx_o = out[2]; p.hy_o = out[3];
+        p.ihx_o = p.ihy_o = nullptr;
+        p.ez_o = (l == L - 1) ? d->ez : nullptr;
+        p.t_first = t0 + done + 1;
+        rc = hip_check(hipEventRecord(ax->fork, S(stream)), "fork event");
+        if (!rc) rc = hip_check(hipStreamWaitEvent(ax->s, ax->fork, 0), "fork wait");
+        if (!rc) rc = launch_fd2d_tb(p, k, pl.cols, lossy, false, 1, ax->s);
+        if (!rc) rc = launch_fd2d_tb(p, k, pl.cols, lossy, false, 0, S(stream));
+        if (!rc) rc = hip_check(hipEventRecord(ax->join, ax->s), "join event");
+        if (!rc) rc = hip_check(hipStreamWaitEvent(S(stream), ax->join, 0), "join wait");
+        if (rc) return rc;
+        // the frame of depth k+3, recomputed with the network, over the
+        // blocked launch's output
+        LbcFrameParams f{};
+        f.nx = d->nx; f.ny = d->ny; f.K = k;
+        f.dz_i = in[0]; f.iz_i = lossy ? in[1] : nullptr; f.hx_i = in[2]; f.hy_i = in[3];
+        f.dz_o = out[0]; f.iz_o = lossy ? out[1] : nullptr; f.hx_o = out[2]; f.hy_o = out[3];
+        f.ez_o = p.ez_o;
+        f.naz = d->md.naz; f.nbz = d->md.nbz;
+        f.hist_in = H[l % 2]; f.hist_out = H[(l + 1) % 2];
+        f.w = b->wbuf; f.params = b->params;
+        rc = launch_lbc_frame(f, S(stream));
+        if (rc) return rc;
+        done += k;
+    }
+    return 0;
+}
+
 // ------------------------------------------------------------ batches
 
 int fd2d_batch_units(const fd2d_grid *grids, int ngrids, int depth, int rows, int cols,

@@ -69,6 +69,21 @@ struct TbParams {
     int probe_len;
 };
 
+// Arguments of one learned-boundary frame launch (lbc_frame.hip).
+struct LbcFrameParams {
+    int nx, ny, K;                              // grid, steps of the launch
+    const float *dz_i, *iz_i, *hx_i, *hy_i;     // state at the launch start
+    float *dz_o, *iz_o, *hx_o, *hy_o, *ez_o;    // state after K steps (ez_o may be NULL)
+    const float *naz, *nbz;
+    const float *hist_in;                       // [npos][4][27], newest sample first
+    float *hist_out;
+    const uint16_t *w;                          // network: bf16 operands
+    const float *params;                        //          FP32 parameters (biases)
+};
+size_t lbc_frame_smem();
+int lbc_frame_blocks(int nx, int ny, int K);
+int launch_lbc_frame(const LbcFrameParams &p, hipStream_t s);
+
 // ----------------------------------------------------- launch wrappers
 // (defined in the kernel translation units; return 0 or a hip status)
 

@@ -1,0 +1,409 @@
+// Learned absorbing boundary: canonical boundary samples, network inference,
+// training (fused forward + backward with bf16 MFMA, FP32 accumulation,
+// deterministic per-block partial gradients) and the Adam update.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string.h>
+
+#include "fdtd_internal.h"
+#include "lbc_impl.h"
+
+namespace fdtd {
+namespace lbc {
+namespace {
+
+constexpr int MS = 72;     // transposed tiles: 64 rows of a round + 8
+constexpr int DS = 40;     // dy rows (32 + 8)
+
+// Samples of an nx x ny field window (row stride ld) -> out[npos][27].
+__global__ void __launch_bounds__(256) sample_kernel(int nx, int ny, int ld, const float *ez,
+                                                     const float *hx, const float *hy, float *out)
+{
+    const int np = npos(nx, ny);
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    int e, t;
+    pos_of(p, nx, ny, e, t);
+    const int L = e < 2 ? ny : nx;
+    const float *F[3] = {ez, hx, hy};
+    auto acc = [&](int f, int i, int j) { return F[f][(long)i * ld + j]; };
+    float *o = out + (long)p * NS;
+    for (int oo = -1; oo <= 1; ++oo) {
+        int tt = min(max(t + oo, 0), L - 1);
+        for (int f = 0; f < 3; ++f)
+            for (int d = 0; d < 3; ++d) o[sidx(f, d, oo)] = canon(acc, e, f, d, oo, tt, nx, ny);
+    }
+}
+
+// bf16 operand buffer from FP32 parameters (padding written as zero).
+__global__ void __launch_bounds__(256) pack_kernel(const float *p, uint16_t *w)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= WBUF) return;
+    float v = 0.0f;
+    if (q < W_W2) {
+        int j = q / K1, k = q % K1;
+        if (k < IN) v = p[P_W1 + j * IN + k];
+    } else if (q < W_W3) {
+        v = p[P_W2 + (q - W_W2)];
+    } else if (q < W_W2T) {
+        int j = (q - W_W3) / HID, k = (q - W_W3) % HID;
+        if (j < OUT) v = p[P_W3 + j * HID + k];
+    } else if (q < W_W3T) {
+        int k = (q - W_W2T) / HID, j = (q - W_W2T) % HID;      // W2T[k][j] = W2[j][k]
+        v = p[P_W2 + j * HID + k];
+    } else {
+        int k = (q - W_W3T) / 32, j = (q - W_W3T) % 32;        // W3T[k][j] = W3[j][k]
+        if (j < OUT) v = p[P_W3 + j * HID + k];
+    }
+    w[q] = f2bf(v);
+}
+
+// Raw features of row `row` (sample g of 4) from src: sample s of a row at
+// offs[row] - s * tstride.
+__device__ __forceinline__ void load_sample(const float *src, const long long *offs,
+                                            long long tstride, int n, int row, int g,
+                                            float (&v)[NS])
+{
+    if (row < n) {
+        const float *q = src + offs[row] - (long long)g * tstride;
+#pragma unroll
+        for (int i = 0; i < NS; ++i) v[i] = q[i];
+    } else {
+#pragma unroll
+        for (int i = 0; i < NS; ++i) v[i] = 0.0f;
+    }
+}
+
+// Inference: y[row][3] (normalised outputs) and s[row] for n rows.
+__global__ void __launch_bounds__(256) forward_kernel(int n, const float *src,
+                                                      const long long *offs, long long tstride,
+                                                      const float *p, const uint16_t *w, float *y,
+                                                      float *s_out)
+{
+    __shared__ __attribute__((aligned(16))) uint16_t Xh[64 * XS];
+    __shared__ __attribute__((aligned(16))) uint16_t A1[64 * AS];
+    __shared__ __attribute__((aligned(16))) uint16_t A2[64 * AS];
+    const Net N{w, p};
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, c = lane & 15, g = lane >> 4;
+    const int r0 = wave * 16;
+    for (int base = blockIdx.x * 64; base < n; base += gridDim.x * 64) {
+        float v[NS];
+        load_sample(src, offs, tstride, n, base + r0 + c, g, v);
+        float s, inv;
+        store_features(v, Xh, r0, s, inv);
+        if (g == 0 && base + r0 + c < n && s_out) s_out[base + r0 + c] = s;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        float z1[16], z2[16], yy[4];
+        forward_tile<false>(N, Xh, A1, A2, r0, z1, z2, yy);
+        if (c < OUT) {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                int row = base + r0 + g * 4 + k;
+                if (row < n) y[(long)row * OUT + c] = yy[k];
+            }
+        }
+        __syncthreads();
+    }
+}
+
+struct TrainSmem {
+    uint16_t Xh[64 * XS];
+    uint16_t XhT[K1 * MS];
+    uint16_t A1[64 * AS], A1T[HID * MS], A2[64 * AS], A2T[HID * MS];
+    uint16_t DY[64 * DS], DYT[16 * MS];
+    uint16_t DZ2[64 * AS], DZ2T[HID * MS], DZ1T[HID * MS];
+    float th[64][4];
+    float red[4][HID];
+};
+
+// One block: rounds of 64 samples (4 waves x 16), forward + backward through
+// bf16 MFMA; weight gradients accumulate in MFMA registers over the rounds
+// (K = the 64 samples of a round); writes this block's partial gradient
+// (NPARAM floats) and loss.
+__global__ void __launch_bounds__(256) train_kernel(int n, const float *src,
+                                                    const long long *offs, long long tstride,
+                                                    const float *tgt, float norm, const float *p,
+                                                    const uint16_t *w, float *part,
+                                                    float *part_loss)
+{
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+    TrainSmem &S = *reinterpret_cast<TrainSmem *>(smem_raw);
+    const Net N{w, p};
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, c = lane & 15, g = lane >> 4;
+    const int r0 = wave * 16;
+    // zero the padding that no round writes: xh^T rows 108..127, dy columns 16..39
+    for (int q = tid; q < (K1 - IN) * MS; q += 256) S.XhT[IN * MS + q] = 0;
+    for (int q = tid; q < 64 * DS; q += 256) S.DY[q] = 0;
+    __syncthreads();
+
+    f32x4 gw1[8], gw2[4], gw3;
+    for (int k = 0; k < 8; ++k) gw1[k] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int k = 0; k < 4; ++k) gw2[k] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    gw3 = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    float gb1[4] = {0, 0, 0, 0}, gb2[4] = {0, 0, 0, 0}, gb3 = 0.0f, lsum = 0.0f;
+    const float sc = 2.0f / norm;
+
+    for (int base = blockIdx.x * 64; base < n; base += gridDim.x * 64) {
+        // gather + scale (row r0 + c, sample g)
+        const int row = base + r0 + c;
+        float v[NS];
+        load_sample(src, offs, tstride, n, row, g, v);
+        float s, inv;
+        store_features(v, S.Xh, r0, s, inv, S.XhT, MS);
+        if (g == 0) {
+            float wgt = (row < n && s > S_MIN) ? 1.0f : 0.0f;
+#pragma unroll
+            for (int d = 0; d < OUT; ++d) {
+                float T = 0.0f;
+                if (row < n) T = tgt ? tgt[(long)row * OUT + d] : src[offs[row] + tstride + sidx(0, d, 0)];
+                S.th[r0 + c][d] = (T - v[sidx(0, d, 0)]) * inv;
+            }
+            S.th[r0 + c][3] = wgt;
+        }
+        __syncthreads();
+
+        float z1[16], z2[16], y[4];
+        forward_tile<true>(N, S.Xh, S.A1, S.A2, r0, z1, z2, y, S.A1T, S.A2T, MS);
+
+        // loss gradient dy (rows r0+4g+k, output c)
+        float dy[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int rr = r0 + g * 4 + k;
+            float e = 0.0f;
+            if (c < OUT) e = (y[k] - S.th[rr][c]) * S.th[rr][3];
+            lsum += e * e;
+            dy[k] = e * sc;
+            gb3 += dy[k];
+            uint16_t h = f2bf(dy[k]);
+            S.DY[rr * DS + c] = h;
+            S.DYT[c * MS + rr] = h;
+        }
+        __syncthreads();
+        // da2 = dy W3 -> dz2
+        {
+            const bf16x8 a = ld8(S.DY + (r0 + c) * DS + g * 8);
+#pragma unroll
+            for (int nt = 0; nt < 4; ++nt) {
+                f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+                acc = mfma(a, ld8(w + W_W3T + (nt * 16 + c) * 32 + g * 8), acc);
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    float dz = acc[k] * gelu_d(z2[nt * 4 + k]);
+                    gb2[nt] += dz;
+                    uint16_t h = f2bf(dz);
+                    S.DZ2[(r0 + g * 4 + k) * AS + nt * 16 + c] = h;
+                    S.DZ2T[(nt * 16 + c) * MS + r0 + g * 4 + k] = h;
+                }
+            }
+        }
+        __syncthreads();
+        // da1 = dz2 W2 -> dz1
+        {
+            bf16x8 a[2];
+#pragma unroll
+            for (int ks = 0; ks < 2; ++ks) a[ks] = ld8(S.DZ2 + (r0 + c) * AS + ks * 32 + g * 8);
+#pragma unroll
+            for (int nt = 0; nt < 4; ++nt) {
+                f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+                for (int ks = 0; ks < 2; ++ks)
+                    acc = mfma(a[ks], ld8(w + W_W2T + (nt * 16 + c) * HID + ks * 32 + g * 8), acc);
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    float dz = acc[k] * gelu_d(z1[nt * 4 + k]);
+                    gb1[nt] += dz;
+                    S.DZ1T[(nt * 16 + c) * MS + r0 + g * 4 + k] = f2bf(dz);
+                }
+            }
+        }
+        __syncthreads();
+        // weight gradients over the round's 64 samples: wave owns hidden-unit
+        // tile `wave` of W1 and W2 and input tile `wave` of W3
+#pragma unroll
+        for (int ks = 0; ks < 2; ++ks) {
+            const bf16x8 a1 = ld8(S.DZ1T + (wave * 16 + c) * MS + ks * 32 + g * 8);
+#pragma unroll
+            for (int kt = 0; kt < 8; ++kt)
+                gw1[kt] = mfma(a1, ld8(S.XhT + (kt * 16 + c) * MS + ks * 32 + g * 8), gw1[kt]);
+            const bf16x8 a2 = ld8(S.DZ2T + (wave * 16 + c) * MS + ks * 32 + g * 8);
+#pragma unroll
+            for (int kt = 0; kt < 4; ++kt)
+                gw2[kt] = mfma(a2, ld8(S.A1T + (kt * 16 + c) * MS + ks * 32 + g * 8), gw2[kt]);
+            gw3 = mfma(ld8(S.DYT + c * MS + ks * 32 + g * 8),
+                       ld8(S.A2T + (wave * 16 + c) * MS + ks * 32 + g * 8), gw3);
+        }
+        __syncthreads();
+    }
+
+    float *P = part + (long)blockIdx.x * NPARAM;
+#pragma unroll
+    for (int kt = 0; kt < 8; ++kt)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int j = wave * 16 + g * 4 + k, kk = kt * 16 + c;
+            if (kk < IN) P[P_W1 + j * IN + kk] = gw1[kt][k];
+        }
+#pragma unroll
+    for (int kt = 0; kt < 4; ++kt)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) P[P_W2 + (wave * 16 + g * 4 + k) * HID + kt * 16 + c] = gw2[kt][k];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int j = g * 4 + k;
+        if (j < OUT) P[P_W3 + j * HID + wave * 16 + c] = gw3[k];
+    }
+    // bias gradients and loss: lanes -> waves -> block, fixed order
+    auto lane_sum = [](float x) {
+        x += __shfl_xor(x, 16);
+        x += __shfl_xor(x, 32);
+        return x;
+    };
+    for (int nt = 0; nt < 4; ++nt) {
+        float x = lane_sum(gb1[nt]);
+        if (g == 0) S.red[wave][nt * 16 + c] = x;
+    }
+    __syncthreads();
+    if (tid < HID) P[P_B1 + tid] = ((S.red[0][tid] + S.red[1][tid]) + S.red[2][tid]) + S.red[3][tid];
+    __syncthreads();
+    for (int nt = 0; nt < 4; ++nt) {
+        float x = lane_sum(gb2[nt]);
+        if (g == 0) S.red[wave][nt * 16 + c] = x;
+    }
+    __syncthreads();
+    if (tid < HID) P[P_B2 + tid] = ((S.red[0][tid] + S.red[1][tid]) + S.red[2][tid]) + S.red[3][tid];
+    __syncthreads();
+    {
+        float x = lane_sum(gb3);
+        float l = lsum;
+        for (int o = 1; o < 64; o <<= 1) l += __shfl_xor(l, o);
+        if (g == 0) S.red[wave][c] = x;
+        if (lane == 0) S.red[wave][16] = l;
+    }
+    __syncthreads();
+    if (tid < OUT)
+        P[P_B3 + tid] = ((S.red[0][tid] + S.red[1][tid]) + S.red[2][tid]) + S.red[3][tid];
+    if (tid == 0 && part_loss)
+        part_loss[blockIdx.x] = ((S.red[0][16] + S.red[1][16]) + S.red[2][16]) + S.red[3][16];
+}
+
+// Sum of the block partials (fixed order), Adam, bf16 operand copies.
+__global__ void __launch_bounds__(256) adam_kernel(int nblocks, const float *part, float *p,
+                                                   float *m, float *v, float lr, float b1,
+                                                   float b2, float eps, float c1, float c2,
+                                                   uint16_t *w, float *grad_out)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= NPARAM) return;
+    float gsum = 0.0f;
+    for (int b = 0; b < nblocks; ++b) gsum += part[(long)b * NPARAM + q];
+    if (grad_out) grad_out[q] = gsum;
+    if (lr == 0.0f) return;
+    const float mm = b1 * m[q] + (1.0f - b1) * gsum;
+    const float vv = b2 * v[q] + (1.0f - b2) * gsum * gsum;
+    m[q] = mm;
+    v[q] = vv;
+    const float val = p[q] - lr * (mm / c1) / (sqrtf(vv / c2) + eps);
+    p[q] = val;
+    const uint16_t h = f2bf(val);
+    if (q < P_B1) {
+        const int j = q / IN, k = q % IN;
+        w[W_W1 + j * K1 + k] = h;
+    } else if (q >= P_W2 && q < P_B2) {
+        const int j = (q - P_W2) / HID, k = (q - P_W2) % HID;
+        w[W_W2 + j * HID + k] = h;
+        w[W_W2T + k * HID + j] = h;
+    } else if (q >= P_W3 && q < P_B3) {
+        const int j = (q - P_W3) / HID, k = (q - P_W3) % HID;
+        w[W_W3 + j * HID + k] = h;
+        w[W_W3T + k * 32 + j] = h;
+    }
+}
+
+}  // namespace
+}  // namespace lbc
+}  // namespace fdtd
+
+using namespace fdtd;
+using namespace fdtd::lbc;
+
+static hipStream_t S(void *s) { return reinterpret_cast<hipStream_t>(s); }
+
+extern "C" {
+
+int lbc_param_count(void) { return NPARAM; }
+int lbc_wbuf_elems(void) { return WBUF; }
+size_t lbc_train_smem(void) { return sizeof(TrainSmem); }
+
+int lbc_sample(int nx, int ny, int ld, const float *ez, const float *hx, const float *hy,
+               float *out, void *stream)
+{
+    if (nx < 8 || ny < 8 || ld < ny) return fail("lbc_sample: window %dx%d ld=%d", nx, ny, ld);
+    if (!ez || !hx || !hy || !out) return fail("lbc_sample: null pointer");
+    const int np = npos(nx, ny);
+    hipLaunchKernelGGL(sample_kernel, dim3((np + 255) / 256), dim3(256), 0, S(stream), nx, ny, ld,
+                       ez, hx, hy, out);
+    return hip_check(hipGetLastError(), "lbc_sample");
+}
+
+int lbc_pack(const float *params, uint16_t *wbuf, void *stream)
+{
+    if (!params || !wbuf) return fail("lbc_pack: null pointer");
+    hipLaunchKernelGGL(pack_kernel, dim3((WBUF + 255) / 256), dim3(256), 0, S(stream), params,
+                       wbuf);
+    return hip_check(hipGetLastError(), "lbc_pack");
+}
+
+int lbc_forward(int n, const float *src, const long long *offs, long long tstride,
+                const float *params, const uint16_t *wbuf, float *y, float *s, void *stream)
+{
+    if (n < 0) return fail("lbc_forward: n=%d", n);
+    if (n == 0) return 0;
+    if (!src || !offs || !params || !wbuf || !y) return fail("lbc_forward: null pointer");
+    int blocks = (n + 63) / 64;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(forward_kernel, dim3(blocks), dim3(256), 0, S(stream), n, src, offs,
+                       tstride, params, wbuf, y, s);
+    return hip_check(hipGetLastError(), "lbc_forward");
+}
+
+int lbc_train_grad(int n, const float *src, const long long *offs, long long tstride,
+                   const float *tgt, float norm, const float *params, const uint16_t *wbuf,
+                   int nblocks, float *part, float *part_loss, void *stream)
+{
+    if (n <= 0 || nblocks <= 0) return fail("lbc_train_grad: n=%d nblocks=%d", n, nblocks);
+    if (!(norm > 0.0f)) return fail("lbc_train_grad: norm must be positive");
+    if (!src || !offs || !params || !wbuf || !part) return fail("lbc_train_grad: null pointer");
+    static thread_local bool attr = false;
+    if (!attr) {
+        int rc = hip_check(hipFuncSetAttribute((const void *)train_kernel,
+                                               hipFuncAttributeMaxDynamicSharedMemorySize,
+                                               (int)sizeof(TrainSmem)),
+                           "lbc_train_grad: shared memory");
+        if (rc) return rc;
+        attr = true;
+    }
+    hipLaunchKernelGGL(train_kernel, dim3(nblocks), dim3(256), sizeof(TrainSmem), S(stream), n,
+                       src, offs, tstride, tgt, norm, params, wbuf, part, part_loss);
+    return hip_check(hipGetLastError(), "lbc_train_grad");
+}
+
+int lbc_adam(int nblocks, const float *part, float *params, float *m, float *v, int step,
+             float lr, float beta1, float beta2, float eps, uint16_t *wbuf, float *grad_out,
+             void *stream)
+{
+    if (nblocks <= 0 || !part) return fail("lbc_adam: nblocks=%d", nblocks);
+    if (lr != 0.0f && (!params || !m || !v || !wbuf || step < 1))
+        return fail("lbc_adam: params, m, v, wbuf and step >= 1 are required for an update");
+    double c1 = 1.0 - __builtin_pow((double)beta1, step > 0 ? step : 1);
+    double c2 = 1.0 - __builtin_pow((double)beta2, step > 0 ? step : 1);
+    hipLaunchKernelGGL(adam_kernel, dim3((NPARAM + 255) / 256), dim3(256), 0, S(stream), nblocks,
+                       part, params, m, v, lr, beta1, beta2, eps, (float)c1, (float)c2, wbuf,
+                       grad_out);
+    return hip_check(hipGetLastError(), "lbc_adam");
+}
+
+}  // extern "C"

@@ -1,0 +1,229 @@
+// Learned absorbing boundary: network constants, bf16 helpers, the canonical
+// boundary sample and the MFMA forward tile shared by the training and the
+// time-stepping kernels.
+//
+// The network maps the last four canonical samples of an edge position
+// (4 x 27 = 108 features: ez, normal-coupled H, tangential-coupled H at 3
+// depths x 3 tangential offsets) to the next-step ez of the outer 3 cells:
+//   s = max|x|, xh = bf16(x * (1/s))  (rows with s <= 1e-30 predict ez_now),
+//   a1 = bf16(gelu(W1 xh + b1)), a2 = bf16(gelu(W2 a1 + b2)), y = W3 a2 + b3,
+//   ez_next[d] = ez_now[d] + s * y[d].
+// Matrix products run on v_mfma_f32_16x16x32_bf16 (bf16 operands, FP32
+// accumulation).  No reference counterpart (the reference's boundaries are
+// the two-step 1-D ABC and the Sullivan PML).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace fdtd {
+namespace lbc {
+
+constexpr int IN = 108, HID = 64, OUT = 3, NS = 27, HIST = 4;
+// rows whose scale s = max|x| is at most S_MIN count as all-zero (no
+// prediction, no training weight): 1/s must stay finite
+constexpr float S_MIN = 1e-30f;
+constexpr int NPARAM = HID * IN + HID + HID * HID + HID + OUT * HID + OUT;  // 11331
+// flat FP32 parameter vector: W1 (64x108) b1 W2 (64x64) b2 W3 (3x64) b3
+constexpr int P_W1 = 0, P_B1 = P_W1 + HID * IN, P_W2 = P_B1 + HID, P_B2 = P_W2 + HID * HID,
+              P_W3 = P_B2 + HID, P_B3 = P_W3 + OUT * HID;
+// bf16 operand buffer (uint16 elements): W1 [64][128] (k >= 108 zero),
+// W2 [64][64], W3 [16][64] (rows >= 3 zero), W2^T [64][64], W3^T [64][32]
+// (k >= 3 zero)
+constexpr int K1 = 128;
+constexpr int W_W1 = 0, W_W2 = W_W1 + HID * K1, W_W3 = W_W2 + HID * HID,
+              W_W2T = W_W3 + 16 * HID, W_W3T = W_W2T + HID * HID, WBUF = W_W3T + HID * 32;
+
+// LDS row strides (bf16 elements; multiples of 8 keep 16-byte alignment and
+// spread 16 rows over distinct bank groups)
+constexpr int XS = 136;   // xh rows (128 + 8)
+constexpr int AS = 72;    // activation rows (64 + 8)
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ uint16_t f2bf(float x)
+{
+    uint32_t u = __float_as_uint(x);
+    u += 0x7FFFu + ((u >> 16) & 1u);          // nearest, ties to even
+    return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+__device__ __forceinline__ bf16x8 ld8(const uint16_t *p)
+{
+    uint4 v = *reinterpret_cast<const uint4 *>(p);
+    return __builtin_bit_cast(bf16x8, v);
+}
+
+__device__ __forceinline__ f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ float gelu(float z)
+{
+    return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f));
+}
+__device__ __forceinline__ float gelu_d(float z)
+{
+    return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) +
+           z * expf(-0.5f * z * z) * 0.39894228040143268f;
+}
+
+// Sample index of field f (0 ez, 1 normal-coupled H, 2 tangential-coupled
+// H), depth d, tangential offset o in {-1, 0, 1}.
+__host__ __device__ constexpr int sidx(int f, int d, int o) { return (f * 3 + d) * 3 + (o + 1); }
+
+// Boundary positions: top (edge 0) and bottom (1) tau in [0, ny) own the
+// corners; left (2) and right (3) tau in [3, nx-3).
+__host__ __device__ inline int npos(int nx, int ny) { return 2 * ny + 2 * (nx - 6); }
+__host__ __device__ inline void pos_of(int p, int nx, int ny, int &e, int &t)
+{
+    if (p < ny) { e = 0; t = p; }
+    else if (p < 2 * ny) { e = 1; t = p - ny; }
+    else if (p < 2 * ny + nx - 6) { e = 2; t = p - 2 * ny + 3; }
+    else { e = 3; t = p - 2 * ny - (nx - 6) + 3; }
+}
+// grid cell of canonical depth d at tangential index t on edge e
+__host__ __device__ inline void cell_of(int e, int d, int t, int nx, int ny, int &i, int &j)
+{
+    switch (e) {
+    case 0: i = d; j = t; break;
+    case 1: i = nx - 1 - d; j = t; break;
+    case 2: i = t; j = d; break;
+    default: i = t; j = ny - 1 - d; break;
+    }
+}
+
+// Canonical value (field f, depth d, tangential offset o, tangential index
+// t = clamped tau + o) of edge e, read through acc(field, i, j) with field 0 ez, 1 hx,
+// 2 hy.  The four edges are mapped onto the top edge by the symmetries of
+// the Yee update (x-mirror: hy -> -hy shifted one row; transpose: hx -> -hy^T,
+// hy -> -hx^T):
+//   top:    ez[d,t],      hn =  hy[d,t],       ht =  hx[d,t]
+//   bottom: ez[nx-1-d,t], hn = -hy[nx-2-d,t],  ht =  hx[nx-1-d,t]
+//   left:   ez[t,d],      hn = -hx[t,d],       ht = -hy[t,d]
+//   right:  ez[t,ny-1-d], hn =  hx[t,ny-2-d],  ht = -hy[t,ny-1-d]
+// ht is 0 at depth 0 on every edge and at tangential offset +1 (ht is
+// staggered half a cell along the edge: offsets -1 and 0 are the two H
+// points beside the cell), so a prediction reaches one cell to either side.
+template <class Acc>
+__device__ __forceinline__ float canon(const Acc &acc, int e, int f, int d, int o, int t, int nx,
+                                       int ny)
+{
+    if (f == 2 && (d == 0 || o == 1)) return 0.0f;
+    switch (e) {
+    case 0: return f == 0 ? acc(0, d, t) : (f == 1 ? acc(2, d, t) : acc(1, d, t));
+    case 1: return f == 0 ? acc(0, nx - 1 - d, t)
+                          : (f == 1 ? -acc(2, nx - 2 - d, t) : acc(1, nx - 1 - d, t));
+    case 2: return f == 0 ? acc(0, t, d) : (f == 1 ? -acc(1, t, d) : -acc(2, t, d));
+    default: return f == 0 ? acc(0, t, ny - 1 - d)
+                           : (f == 1 ? acc(1, t, ny - 2 - d) : -acc(2, t, ny - 1 - d));
+    }
+}
+
+struct Net {
+    const uint16_t *w;      // bf16 operand buffer (WBUF elements)
+    const float *p;         // FP32 parameters (biases read from here)
+};
+
+// Scales and stores one 16-row tile's features.  Lane (r = lane&15,
+// g = lane>>4) holds sample g (27 raw values) of row r in v[]; writes xh to
+// Xh[(r0 + r) * XS + 27 g + i] (and columns 108..127 as zero).  Returns
+// the row scale s and 1/s (0 when s <= S_MIN) to every lane of the row.
+__device__ __forceinline__ void store_features(const float (&v)[NS], uint16_t *Xh, int r0,
+                                               float &s, float &inv, uint16_t *XhT = nullptr,
+                                               int MS = 0)
+{
+    const int lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
+    float mx = 0.0f;
+#pragma unroll
+    for (int i = 0; i < NS; ++i) mx = fmaxf(mx, fabsf(v[i]));
+    mx = fmaxf(mx, __shfl_xor(mx, 16));
+    mx = fmaxf(mx, __shfl_xor(mx, 32));
+    s = mx;
+    inv = mx > S_MIN ? 1.0f / mx : 0.0f;
+    uint16_t *row = Xh + (r0 + r) * XS;
+#pragma unroll
+    for (int i = 0; i < NS; ++i) {
+        uint16_t b = f2bf(v[i] * inv);
+        row[g * NS + i] = b;
+        if (XhT) XhT[(g * NS + i) * MS + r0 + r] = b;
+    }
+#pragma unroll
+    for (int q = 0; q < 5; ++q) row[IN + g * 5 + q] = 0;
+}
+
+// Forward pass of the 16 rows r0..r0+15 held in Xh (one wave).  Outputs in
+// the MFMA accumulator layout: lane (c = lane&15, g = lane>>4) holds
+// z1/z2[nt*4+v] of row r0 + 4g + v, hidden unit 16 nt + c, and y[v] of row
+// r0 + 4g + v, output c (valid for c < 3).  a1/a2 (bf16) are left in A1/A2
+// rows; with A1T/A2T also stored transposed (column = row index).
+template <bool KEEP>
+__device__ __forceinline__ void forward_tile(const Net &N, const uint16_t *Xh, uint16_t *A1,
+                                             uint16_t *A2, int r0, float (&z1)[16],
+                                             float (&z2)[16], float (&y)[4],
+                                             uint16_t *A1T = nullptr, uint16_t *A2T = nullptr,
+                                             int MS = 0)
+{
+    const int lane = threadIdx.x & 63, c = lane & 15, g = lane >> 4;
+    const uint16_t *W1 = N.w + W_W1, *W2 = N.w + W_W2, *W3 = N.w + W_W3;
+    // layer 1: [16 x 128] x [128 x 64]
+    bf16x8 xa[4];
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks) xa[ks] = ld8(Xh + (r0 + c) * XS + ks * 32 + g * 8);
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt) {
+        f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int ks = 0; ks < 4; ++ks)
+            acc = mfma(xa[ks], ld8(W1 + (nt * 16 + c) * K1 + ks * 32 + g * 8), acc);
+        const float b = N.p[P_B1 + nt * 16 + c];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            float z = acc[v] + b;
+            z1[nt * 4 + v] = z;
+            uint16_t h = f2bf(gelu(z));
+            A1[(r0 + g * 4 + v) * AS + nt * 16 + c] = h;
+            if (KEEP) A1T[(nt * 16 + c) * MS + r0 + g * 4 + v] = h;
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    // layer 2: [16 x 64] x [64 x 64]
+    bf16x8 aa[2];
+#pragma unroll
+    for (int ks = 0; ks < 2; ++ks) aa[ks] = ld8(A1 + (r0 + c) * AS + ks * 32 + g * 8);
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt) {
+        f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int ks = 0; ks < 2; ++ks)
+            acc = mfma(aa[ks], ld8(W2 + (nt * 16 + c) * HID + ks * 32 + g * 8), acc);
+        const float b = N.p[P_B2 + nt * 16 + c];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            float z = acc[v] + b;
+            z2[nt * 4 + v] = z;
+            uint16_t h = f2bf(gelu(z));
+            A2[(r0 + g * 4 + v) * AS + nt * 16 + c] = h;
+            if (KEEP) A2T[(nt * 16 + c) * MS + r0 + g * 4 + v] = h;
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    // layer 3: [16 x 64] x [64 x 16] (outputs 3..15 are zero rows of W3)
+    f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int ks = 0; ks < 2; ++ks)
+        acc = mfma(ld8(A2 + (r0 + c) * AS + ks * 32 + g * 8), ld8(W3 + c * HID + ks * 32 + g * 8),
+                   acc);
+    const float b3 = c < OUT ? N.p[P_B3 + c] : 0.0f;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) y[v] = acc[v] + b3;
+}
+
+}  // namespace lbc
+}  // namespace fdtd

@@ -1,0 +1,127 @@
+"""Pins of the learned-boundary restatement (oracle/lbc_oracle.py), CPU only.
+
+bf16 rounding against torch's bfloat16 conversion; the backward pass against
+central differences of the FP64 forward; the canonical boundary sample
+against the fourfold symmetry of a centred pulse; the learned step against
+the plain update with the outer three cells held at zero; Adam's first step
+against its closed form."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import lbc_oracle as L
+from oracle import numpy_oracle as NO
+
+
+def test_bf16_matches_torch_rounding():
+    rng = np.random.default_rng(1)
+    x = rng.standard_normal(20000).astype(np.float32) * np.float32(10.0) ** rng.integers(-8, 8, 20000)
+    # exact ties: low 16 bits = 0x8000 with even and odd upper halves
+    ties = (np.arange(64, dtype=np.uint32) << 16 | 0x3F800000 | 0x8000).view(np.float32)
+    x = np.concatenate([x, ties, -ties, np.float32([0.0, -0.0, 1.0, 3.0e38])])
+    want = torch.from_numpy(x).to(torch.bfloat16).to(torch.float32).numpy()
+    assert np.array_equal(L.bf16(x), want)
+
+
+def _problem(rng, B=24):
+    X = rng.standard_normal((B, L.IN)).astype(np.float32) * np.float32(0.3)
+    X[3] = 0.0                                        # an all-zero row (weight 0)
+    T = rng.standard_normal((B, L.OUT)).astype(np.float32) * np.float32(0.1)
+    p = L.init_params(3)
+    p[L.NPARAM - 200:] *= 5.0                          # non-trivial output layer
+    return p.astype(np.float32), X, T
+
+
+def test_backward_matches_central_differences():
+    rng = np.random.default_rng(7)
+    p, X, T = _problem(rng)
+    loss, g = L.loss_and_grad(p, X, T, rounding=False)
+
+    def f(q):
+        y, _ = L.forward(q, X, rounding=False)
+        th, w = L.targets(X, T)
+        e = (y - th) * w[:, None]
+        return float(np.sum(e * e) / len(X))
+    assert abs(f(p) - loss) < 1e-12
+    idx = np.concatenate([rng.choice(L.NPARAM, 40, replace=False),
+                          [0, L.NPARAM - 1, L.NPARAM - 4, 64 * 108 + 5]])
+    p64 = p.astype(np.float64)
+    for k in idx:
+        h = 1e-3 * max(1.0, abs(p64[k]))
+        a, b = p64.copy(), p64.copy()
+        a[k] += h
+        b[k] -= h
+        fd = (f(a) - f(b)) / (2 * h)
+        assert abs(fd - g[k]) <= 1e-6 + 1e-4 * abs(fd), (k, fd, g[k])
+
+
+def test_rounded_gradient_close_to_exact():
+    rng = np.random.default_rng(8)
+    p, X, T = _problem(rng, 256)
+    _, g0 = L.loss_and_grad(p, X, T, rounding=False)
+    _, g1 = L.loss_and_grad(p, X, T, rounding=True)
+    cos = g0 @ g1 / np.linalg.norm(g0) / np.linalg.norm(g1)
+    assert cos > 0.999
+    assert np.linalg.norm(g1 - g0) / np.linalg.norm(g0) < 0.05
+
+
+def test_canonical_sample_fourfold_symmetry():
+    """A pulse from the centre of a square free-space grid is symmetric under
+    the x-mirror, the y-mirror and the transpose; the canonical samples of
+    the four edges of a centred window must then agree."""
+    n, c = 61, 30
+    st = {k: np.zeros((n, n), np.float32) for k in ("dz", "iz", "ez", "hx", "hy")}
+    naz = np.ones((n, n), np.float32)
+    t = np.arange(1, 26, dtype=np.float64)
+    w = np.exp(-0.5 * ((t - 8.0) / 3.0) ** 2).astype(np.float32)
+    for s in range(25):
+        NO.dfield(n, n, None, st["dz"], st["hx"], st["hy"])
+        st["dz"][c, c] = w[s]
+        NO.efield(naz, None, st["dz"], st["iz"], st["ez"])
+        NO.hfield(n, n, None, st["ez"], None, None, st["hx"], st["hy"])
+    a, b = 20, 41                                   # 21 x 21 window centred on (30, 30)
+    win = [st[k][a:b, a:b] for k in ("ez", "hx", "hy")]
+    S = L.sample(*win, 21, 21)
+    pos = L.positions(21, 21)
+    blocks = {e: np.array([S[p] for p, (ee, tt) in enumerate(pos) if ee == e and 3 <= tt < 18])
+              for e in range(4)}
+    scale = np.abs(blocks[0]).max()
+    assert scale > 1e-3
+    for e in (1, 2, 3):
+        assert np.abs(blocks[e] - blocks[0]).max() <= 1e-5 * scale, e
+    # the depth-0 tangential-coupled entries are zero by definition
+    assert not np.any(S[:, [L.sidx(2, 0, o) for o in (-1, 0, 1)]])
+
+
+def test_learned_step_with_zero_network_freezes_the_frame():
+    """With every parameter zero the prediction is ez_now, so the outer three
+    cells keep their initial zero: the same as the plain free-space update
+    with naz = 0 on those cells (ez = naz * dz = 0 there)."""
+    nx, ny, ns = 23, 29, 30
+    rng = np.random.default_rng(0)
+    naz = rng.uniform(0.3, 1.0, (nx, ny)).astype(np.float32)
+    nbz = rng.uniform(0.0, 0.05, (nx, ny)).astype(np.float32)
+    w = np.exp(-0.5 * ((np.arange(1, ns + 1) - 10.0) / 4.0) ** 2).astype(np.float32)
+    a = {k: np.zeros((nx, ny), np.float32) for k in ("dz", "iz", "ez", "hx", "hy")}
+    L.lbc_run(a, naz, nbz, np.zeros(L.NPARAM, np.float32), ns, sources=[(11, 14)],
+              src_wave=[w])
+    naz0 = naz.copy()
+    naz0[:3, :] = 0
+    naz0[-3:, :] = 0
+    naz0[:, :3] = 0
+    naz0[:, -3:] = 0
+    b = {k: np.zeros((nx, ny), np.float32) for k in ("dz", "iz", "ez", "hx", "hy", "ihx", "ihy")}
+    NO.tm_run(b, ns, None, naz0, nbz, [(11, 14)], [w])
+    for k in ("ez", "hx", "hy"):
+        assert np.array_equal(a[k], b[k]), k
+    assert np.abs(a["ez"]).max() > 1e-3
+
+
+def test_adam_first_step_closed_form():
+    g = np.array([0.5, -2.0, 1e-3, 0.0], np.float32)
+    p = np.zeros(4, np.float32)
+    m = np.zeros(4, np.float32)
+    v = np.zeros(4, np.float32)
+    L.adam(p, m, v, g, 1, 0.01)
+    want = -0.01 * np.sign(g) * np.abs(g) / (np.abs(g) + 1e-8)
+    assert np.allclose(p, want, rtol=1e-5, atol=1e-9)
