@@ -183,6 +183,17 @@ int fd2d_energy(int nx, int ny, int i0, int i1, int j0, int j1,
                 const float *ez, const float *hx, const float *hy,
                 double *work, double *result, void *stream);
 
+/* FP64 energy of the difference of two field sets over equal wn x wm
+ * windows -- window (ia0, ja0) of grid a (nxa x nya) against (ib0, jb0) of
+ * grid b: sum (ez_a - ez_b)^2 + (hx_a - hx_b)^2 + (hy_a - hy_b)^2.  Used to
+ * measure reflected energy against a reference run on a larger grid.  No
+ * reference counterpart.  Same work buffer and result handling as
+ * fd2d_energy.                                                              */
+int fd2d_energy_diff(int wn, int wm, int nxa, int nya, int ia0, int ja0, const float *eza,
+                     const float *hxa, const float *hya, int nxb, int nyb, int ib0, int jb0,
+                     const float *ezb, const float *hxb, const float *hyb, double *work,
+                     double *result, void *stream);
+
 /* Fused, temporally blocked TM time stepping.  One call advances the state
  * by nsteps leapfrog steps t = t0+1 .. t0+nsteps with the serial semantics
  * dfield -> hard sources -> efield -> hfield of fd2d/pycuda/fd2d_3_4.py:
@@ -302,32 +313,40 @@ int fdtd_graph_destroy(void *graph_exec);
  * network applied at the outer three cells of every edge.
  *
  * Network: 108 features (the last four canonical boundary samples of a
- * position, newest first, 27 values each) -> 64 -> 64 -> 3, GELU, bf16
+ * position, newest first, 27 features each) -> 64 -> 64 -> 3, GELU, bf16
  * weights and activations, FP32 accumulation (MFMA).  With s = max|x|:
- *   ez_next[d] = ez_now[d] + s * y[d],  y = net(bf16(x / s)),  d = 0..2.
+ *   ez_next[d] = base[d] + s * y[d],  y = net(bf16(x / s)),  d = 0..2,
+ * base = the two-step absorbing boundary of fd1d/pycuda/fd1d_1_2.py:53-61 on
+ * the three outer layers (ez_d(t+1) = ez_{d+1}(t-1)): a network with zero
+ * output is that classical boundary.
  * Parameters: one FP32 vector of lbc_param_count() values, in the order
  * W1 (64x108, row-major), b1 (64), W2 (64x64), b2, W3 (3x64), b3; the
  * kernels read bf16 copies from a buffer of lbc_wbuf_elems() uint16 values
  * written by lbc_pack (and kept current by lbc_adam).
  *
- * Canonical sample (27 values, index (f*3 + d)*3 + o + 1 for field f, depth
- * d = 0..2 from the edge, tangential offset o = -1..1, tangential index
- * clamped to the edge); f = 0 ez, 1 normal-coupled H, 2 tangential-coupled H:
- *   top    (i = d):      ez[d,t],      hy[d,t],       hx[d,t]
- *   bottom (i = nx-1-d): ez[nx-1-d,t], -hy[nx-2-d,t], hx[nx-1-d,t]
- *   left   (j = d):      ez[t,d],      -hx[t,d],      -hy[t,d]
- *   right  (j = ny-1-d): ez[t,ny-1-d], hx[t,ny-2-d],  -hy[t,ny-1-d]
- * with the tangential-coupled value 0 at depth 0 and at offset +1 (it is
- * staggered half a cell along the edge; offsets -1 and 0 are the two points
- * beside the cell).  Positions: top then
- * bottom (t in [0, ny), owning the corners), then left and right (t in
- * [3, nx-3)): lbc_npos = 2 ny + 2 (nx - 6).                                 */
+ * Canonical sample (29 floats): 27 features at index (f*3 + d)*3 + o + 1 for
+ * field f, feature depth d = 0..2 (grid depth D = d + 2: the innermost
+ * predicted cell and two cells inside it -- only ez at depth 2 reaches the
+ * interior, so the network never reads cells that only it writes),
+ * tangential offset o = -1..1 (tangential index clamped to the edge); f = 0
+ * ez, 1 normal-coupled H, 2 tangential-coupled H:
+ *   top    (i = D):      ez[D,t],      hy[D,t],       hx[D,t]
+ *   bottom (i = nx-1-D): ez[nx-1-D,t], -hy[nx-2-D,t], hx[nx-1-D,t]
+ *   left   (j = D):      ez[t,D],      -hx[t,D],      -hy[t,D]
+ *   right  (j = ny-1-D): ez[t,ny-1-D], hx[t,ny-2-D],  -hy[t,ny-1-D]
+ * with the tangential-coupled value 0 at offset +1 (it is staggered half a
+ * cell along the edge; offsets -1 and 0 are the two points beside the cell)
+ * and H in the last row or column of the window read as 0 (the H update
+ * never runs there); then ez of grid depth 1 and 0 at offset 0 (targets and
+ * baseline only).  Positions: top then bottom (t in [0, ny), owning the
+ * corners), then left and right (t in [3, nx-3)):
+ * lbc_npos = 2 ny + 2 (nx - 6).                                             */
 int lbc_param_count(void);
 int lbc_wbuf_elems(void);
 size_t lbc_train_smem(void);
 
 /* Canonical samples of an nx x ny window (row stride ld) of ez/hx/hy:
- * out[lbc_npos][27].                                                        */
+ * out[lbc_npos][29].                                                        */
 int lbc_sample(int nx, int ny, int ld, const float *ez, const float *hx, const float *hy,
                float *out, void *stream);
 
@@ -335,20 +354,26 @@ int lbc_sample(int nx, int ny, int ld, const float *ez, const float *hx, const f
 int lbc_pack(const float *params, uint16_t *wbuf, void *stream);
 
 /* Rows of features given as samples in device memory: row r's sample k
- * (k = 0 newest .. 3) starts at src + offs[r] - k*tstride.  Writes the
+ * (k = 0 newest .. 3; its first 27 floats are read) starts at
+ * src + offs[r] - k*tstride.  Writes the
  * normalised outputs y[r][3] and (if s is not NULL) the scales s[r].        */
 int lbc_forward(int n, const float *src, const long long *offs, long long tstride,
                 const float *params, const uint16_t *wbuf, float *y, float *s, void *stream);
 
-/* Training gradient of  sum_r w_r |y_r - (T_r - ez_now_r)/s_r|^2 / norm
+/* Training gradient of  sum_r w_r |y_r - (T_r - base_r)/s_r|^2 / norm
  * (w_r = 0 for all-zero rows), targets T_r = tgt[r][3] or, with tgt NULL,
- * the next sample's ez (src + offs[r] + tstride, depths 0..2, offset 0).
+ * the next sample's ez of grid depths 0..2 (src + offs[r] + tstride); the
+ * baseline reads the sample of age 1, which must be a full 29-float sample.
  * nblocks workgroups each write one partial gradient (lbc_param_count
  * floats) to part[block] and its loss sum to part_loss[block]; lbc_adam
- * reduces them in a fixed order (deterministic).                            */
+ * reduces them in a fixed order (deterministic).  noise > 0 adds uniform
+ * noise of that amplitude to the normalised inputs (a hash of row, feature
+ * and seed: reproducible), for a network that tolerates its own errors when
+ * its outputs feed its later inputs.                                        */
 int lbc_train_grad(int n, const float *src, const long long *offs, long long tstride,
                    const float *tgt, float norm, const float *params, const uint16_t *wbuf,
-                   int nblocks, float *part, float *part_loss, void *stream);
+                   int nblocks, float *part, float *part_loss, float noise, unsigned seed,
+                   void *stream);
 
 /* Time stepping with the learned boundary.  The TM descriptor gives the grid
  * without a PML (pml fields NULL; md.nbz optional: lossy or lossless), the
@@ -357,7 +382,7 @@ int lbc_train_grad(int n, const float *src, const long long *offs, long long tst
  * at most 8).  Each launch of k <= depth steps is the blocked launch with PEC
  * edges followed by a frame kernel that recomputes the outer k+3 cells with
  * the network applied every step (the blocked launch is exact deeper in).
- * hist, hist2: [lbc_hist_elems(nx, ny)] floats each, the last four boundary
+ * hist, hist2: [lbc_hist_elems(nx, ny)] floats each (npos x 4 x 29), the last four boundary
  * samples of every position (zero for fields at rest); the run leaves them
  * current in hist.  nsteps >= 2 (an even number of launches).               */
 #define FDTD_LBC_MARGIN 32

@@ -4,11 +4,16 @@ The reference has no learned boundary; this file states the exact semantics
 that the HIP path (simulation_amd/csrc/lbc*.hip) implements, so the GPU
 kernels have a checker:
 
-* the canonical boundary sample of an edge position (27 values: ez, the
-  normal-coupled H and the tangential-coupled H at 3 depths x 3 tangential
-  offsets), the four edges mapped onto one frame by the exact symmetries of
-  the Yee update (x-mirror, y-mirror, transpose);
-* the network 108 -> 64 -> 64 -> 3 (GELU), with the GPU's rounding points
+* the canonical boundary sample of an edge position (27 features: ez, the
+  normal-coupled H and the tangential-coupled H at grid depths 3..5 x 3
+  tangential offsets; plus the ez of depths 2, 1 and 0, targets only), the four
+  edges mapped onto one frame by the exact symmetries of the Yee update
+  (x-mirror, y-mirror, transpose);
+* the baseline (the reference's two-step absorbing boundary,
+  fd1d/pycuda/fd1d_1_2.py:53-61, on the three outer layers) that the network
+  corrects;
+* the odd network y = (f(x) - f(-x))/2, f: 108 -> 64 -> 64 -> 3 (GELU),
+  with the GPU's rounding points
   (inputs and hidden activations rounded to bf16, weights bf16, products
   accumulated in FP32 on the GPU and FP64 here), and its backward pass;
 * one learned-boundary time step: the free-space/lossy update of
@@ -28,7 +33,10 @@ from oracle import numpy_oracle as NO
 
 IN, HID, OUT = 108, 64, 3
 S_MIN = np.float32(1e-30)      # rows with max|x| <= S_MIN count as all-zero
-NSAMP = 27
+NFEAT = 27                     # features per sample
+NSAMP = 30                     # stored sample: features, then ez of grid depths 2, 1, 0
+S_EZ2, S_EZ1, S_EZ0 = 27, 28, 29
+FD0 = 3                        # grid depth of feature depth 0
 HIST = 4
 NPARAM = HID * IN + HID + HID * HID + HID + OUT * HID + OUT
 
@@ -109,64 +117,90 @@ def scale(X: np.ndarray):
     return s, inv, xh
 
 
-def forward(params, X, rounding: bool = True):
-    """Network on raw features X (B,108) -> (y (B,3) FP64, cache).  With
-    rounding=True the bf16 rounding points of the kernels are applied (FP64
-    sums in between); with False everything is FP64 (the exact function of
-    the FP32 master weights)."""
-    P = unpack(params)
-    s, inv, xh = scale(X)
-    r = bf16 if rounding else (lambda a: np.asarray(a, np.float64))
-    if not rounding:
-        xh = (np.asarray(X, np.float64) * inv[:, None].astype(np.float64))
+def _branch(P, xh, r, rounding):
     W1, W2, W3 = (r(P[k]).astype(np.float64) for k in ("W1", "W2", "W3"))
-    xh = np.asarray(xh, np.float64)
     z1 = xh @ W1.T + P["b1"].astype(np.float64)
     a1 = r(gelu(z1).astype(np.float32) if rounding else gelu(z1)).astype(np.float64)
     z2 = a1 @ W2.T + P["b2"].astype(np.float64)
     a2 = r(gelu(z2).astype(np.float32) if rounding else gelu(z2)).astype(np.float64)
     y = a2 @ W3.T + P["b3"].astype(np.float64)
-    return y, dict(s=s, inv=inv, xh=xh, z1=z1, a1=a1, z2=z2, a2=a2, W1=W1, W2=W2, W3=W3)
+    return y, dict(xh=xh, z1=z1, a1=a1, z2=z2, a2=a2, W1=W1, W2=W2, W3=W3)
 
 
-def targets(X, T):
-    """Normalised targets (T - ez_now) / s for raw features X and next-step
-    ez T (B,3), FP32 as the kernels compute them, and the sample weights
-    (0 for rows with s <= S_MIN)."""
+def forward(params, X, rounding: bool = True):
+    """Network on raw features X (B,108) -> (y (B,3) FP64, cache).
+
+    The network is made odd, y(-x) = -y(x), as the exact boundary map is
+    linear: y = (f(xh) - f(-xh)) / 2 with f the 108-64-64-3 GELU network (a
+    constant or even part would pump a one-signed field into the grid through
+    the scale s).  With rounding=True the bf16 rounding points of the kernels
+    are applied (FP64 sums in between; bf16(-x) = -bf16(x)); with False
+    everything is FP64 (the exact function of the FP32 master weights)."""
+    P = unpack(params)
+    s, inv, xh = scale(X)
+    r = bf16 if rounding else (lambda a: np.asarray(a, np.float64))
+    if not rounding:
+        xh = (np.asarray(X, np.float64) * inv[:, None].astype(np.float64))
+    xh = np.asarray(xh, np.float64)
+    yp, cp = _branch(P, xh, r, rounding)
+    ym, cm = _branch(P, -xh, r, rounding)
+    return 0.5 * (yp - ym), dict(s=s, inv=inv, plus=cp, minus=cm)
+
+
+def base(S1) -> np.ndarray:
+    """Baseline next-step ez (B,3) of grid depths 0..2, FP32, from the full
+    samples S1 (B,30) of age 1 (two steps before the predicted one): the
+    two-step absorbing boundary of fd1d/pycuda/fd1d_1_2.py:53-61 on the three
+    outer layers, ez_d(t+1) = ez_{d+1}(t-1)."""
+    S1 = np.asarray(S1, np.float32)
+    return np.stack([S1[:, S_EZ1], S1[:, S_EZ2], S1[:, sidx(0, 0, 0)]], axis=1)
+
+
+def targets(X, T, S1):
+    """Normalised targets (T - base) / s for features X (B,108), next-step
+    ez T (B,3) of grid depths 0..2 and age-1 samples S1 (B,30), FP32 as the
+    kernels compute them, and the sample weights (0 for rows with
+    s <= S_MIN)."""
     X = np.asarray(X, np.float32)
     s, inv, _ = scale(X)
-    ez0 = X[:, [sidx(0, d, 0) for d in range(OUT)]]
-    th = ((np.asarray(T, np.float32) - ez0) * inv[:, None]).astype(np.float32)
+    th = ((np.asarray(T, np.float32) - base(S1)) * inv[:, None]).astype(np.float32)
     w = (s > S_MIN).astype(np.float32)
     return th, w
 
 
-def loss_and_grad(params, X, T, nrm: int | None = None, rounding: bool = True):
+def loss_and_grad(params, X, T, S1=None, nrm: int | None = None, rounding: bool = True):
     """Mean-square loss sum_b w_b |y_b - th_b|^2 / nrm and its gradient with
     respect to the flat FP32 parameters (straight through the bf16 weight
-    rounding).  rounding=True rounds the backward MFMA operands (dy, dz2, dz1)
-    to bf16 as the kernels do."""
+    rounding).  S1: age-1 samples for the baseline (default: the age-1
+    features padded with zero outer ez).  The odd network's gradient is the
+    f(xh) branch's with dy/2 plus the f(-xh) branch's with -dy/2.
+    rounding=True rounds the backward MFMA operands (dy, dz2, dz1) to bf16 as
+    the kernels do."""
     y, c = forward(params, X, rounding)
-    th, w = targets(X, T)
+    if S1 is None:
+        S1 = pad_samples(np.asarray(X, np.float32)[:, NFEAT:2 * NFEAT])
+    th, w = targets(X, T, S1)
     B = len(X) if nrm is None else nrm
     e = (y - th) * w[:, None]
     loss = float(np.sum(e * e) / B)
     r = (lambda a: bf16(np.asarray(a, np.float32)).astype(np.float64)) if rounding else (lambda a: a)
-    dy = 2.0 * e / B
-    dyr = r(dy)
-    gW3 = dyr.T @ c["a2"]
-    gb3 = dy.sum(0)
-    da2 = dyr @ c["W3"]
-    dz2 = da2 * gelu_grad(c["z2"])
-    dz2r = r(dz2)
-    gW2 = dz2r.T @ c["a1"]
-    gb2 = dz2.sum(0)
-    da1 = dz2r @ c["W2"]
-    dz1 = da1 * gelu_grad(c["z1"])
-    dz1r = r(dz1)
-    gW1 = dz1r.T @ c["xh"]
-    gb1 = dz1.sum(0)
-    g = np.concatenate([gW1.reshape(-1), gb1, gW2.reshape(-1), gb2, gW3.reshape(-1), gb3])
+    g = np.zeros(NPARAM)
+    for sgn, cc in ((0.5, c["plus"]), (-0.5, c["minus"])):
+        dy = sgn * 2.0 * e / B
+        dyr = r(dy)
+        gW3 = dyr.T @ cc["a2"]
+        gb3 = dy.sum(0)
+        da2 = dyr @ cc["W3"]
+        dz2 = da2 * gelu_grad(cc["z2"])
+        dz2r = r(dz2)
+        gW2 = dz2r.T @ cc["a1"]
+        gb2 = dz2.sum(0)
+        da1 = dz2r @ cc["W2"]
+        dz1 = da1 * gelu_grad(cc["z1"])
+        dz1r = r(dz1)
+        gW1 = dz1r.T @ cc["xh"]
+        gb1 = dz1.sum(0)
+        g += np.concatenate([gW1.reshape(-1), gb1, gW2.reshape(-1), gb2, gW3.reshape(-1), gb3])
     return loss, g
 
 
@@ -183,6 +217,12 @@ def adam(params, m, v, g, step: int, lr: float, b1=0.9, b2=0.999, eps=1e-8):
 
 # ------------------------------------------------------------------ boundary samples
 
+def pad_samples(F) -> np.ndarray:
+    """(B,27) features -> (B,30) samples with zero outer ez."""
+    F = np.asarray(F, np.float32)
+    return np.concatenate([F, np.zeros((len(F), NSAMP - NFEAT), np.float32)], axis=1)
+
+
 def positions(nx: int, ny: int):
     """(edge, tau) of every boundary position in kernel order: top and bottom
     tau in [0, ny) (they own the corners), left and right tau in [3, nx-3)."""
@@ -197,52 +237,67 @@ def cell(e: int, d: int, t: int, nx: int, ny: int):
 
 
 def sample(ez, hx, hy, nx: int, ny: int) -> np.ndarray:
-    """Canonical 27-value samples (npos, 27) of the current fields.
-    top:    ez[d,t],      hn =  hy[d,t],        ht =  hx[d,t]
-    bottom: ez[nx-1-d,t], hn = -hy[nx-2-d,t],   ht =  hx[nx-1-d,t]
-    left:   ez[t,d],      hn = -hx[t,d],        ht = -hy[t,d]
-    right:  ez[t,ny-1-d], hn =  hx[t,ny-2-d],   ht = -hy[t,ny-1-d]
-    ht at depth 0 is 0 on every edge (never updated on the bottom and right
-    edges, never read by an update on the other two), and ht is taken only at
-    offsets -1 and 0 -- the two H points beside the cell (ht is staggered by
-    half a cell along the edge), so a prediction reaches one cell to either
-    side; tangential indices are clamped to the edge."""
+    """Canonical samples (npos, 30) of the current fields: 27 features of
+    grid depths D = 3, 4, 5 (feature depth d = D - 3; the cells just inside
+    the three predicted ones, so the network never reads its own output):
+    top:    ez[D,t],      hn =  hy[D,t],        ht =  hx[D,t]
+    bottom: ez[nx-1-D,t], hn = -hy[nx-2-D,t],   ht =  hx[nx-1-D,t]
+    left:   ez[t,D],      hn = -hx[t,D],        ht = -hy[t,D]
+    right:  ez[t,ny-1-D], hn =  hx[t,ny-2-D],   ht = -hy[t,ny-1-D]
+    ht is taken only at offsets -1 and 0 -- the two H points beside the cell
+    (ht is staggered by half a cell along the edge), so a prediction reaches
+    one cell to either side; H in the last row or column reads as 0 (the H
+    update never runs there, so a window of a larger grid must not see it
+    either); tangential indices are clamped to the edge.  Then ez of grid
+    depths 2, 1, 0 at offset 0 (targets and baseline, not network inputs)."""
     pos = positions(nx, ny)
     out = np.zeros((len(pos), NSAMP), np.float32)
+
+    def H(a, i, j):
+        return np.float32(0.0) if (i == nx - 1 or j == ny - 1) else a[i, j]
+
+    def ezc(e, D, t):
+        i, j = cell(e, D, t, nx, ny)
+        return ez[i, j]
     for p, (e, t) in enumerate(pos):
         L = ny if e < 2 else nx
         for o in (-1, 0, 1):
             tt = min(max(t + o, 0), L - 1)
             for d in range(3):
+                D = d + FD0
                 if e == 0:
-                    v = (ez[d, tt], hy[d, tt], hx[d, tt])
+                    v = (H(hy, D, tt), H(hx, D, tt))
                 elif e == 1:
-                    v = (ez[nx - 1 - d, tt], -hy[nx - 2 - d, tt], hx[nx - 1 - d, tt])
+                    v = (-H(hy, nx - 2 - D, tt), H(hx, nx - 1 - D, tt))
                 elif e == 2:
-                    v = (ez[tt, d], -hx[tt, d], -hy[tt, d])
+                    v = (-H(hx, tt, D), -H(hy, tt, D))
                 else:
-                    v = (ez[tt, ny - 1 - d], hx[tt, ny - 2 - d], -hy[tt, ny - 1 - d])
-                out[p, sidx(0, d, o)] = v[0]
-                out[p, sidx(1, d, o)] = v[1]
-                out[p, sidx(2, d, o)] = v[2] if (d > 0 and o < 1) else 0.0
+                    v = (H(hx, tt, ny - 2 - D), -H(hy, tt, ny - 1 - D))
+                out[p, sidx(0, d, o)] = ezc(e, D, tt)
+                out[p, sidx(1, d, o)] = v[0]
+                out[p, sidx(2, d, o)] = v[1] if o < 1 else 0.0
+        out[p, S_EZ2] = ezc(e, 2, t)
+        out[p, S_EZ1] = ezc(e, 1, t)
+        out[p, S_EZ0] = ezc(e, 0, t)
     return out
 
 
 def features(hist) -> np.ndarray:
     """108 features from the last four samples, newest first."""
-    return np.concatenate(list(hist), axis=1).astype(np.float32)
+    return np.concatenate([h[:, :NFEAT] for h in hist], axis=1).astype(np.float32)
 
 
 def predict(params, hist):
-    """Predicted next-step ez (npos, 3) in FP32: ez_now + s * y."""
+    """Predicted next-step ez (npos, 3) of grid depths 0..2 in FP32:
+    base + s * y."""
     X = features(hist)
     y, c = forward(params, X, True)
     s = c["s"]
-    ez0 = X[:, [sidx(0, d, 0) for d in range(OUT)]]
+    b = base(hist[1])
     yf = y.astype(np.float32)
-    pred = (ez0 + (s[:, None] * yf).astype(np.float32)).astype(np.float32)
+    pred = (b + (s[:, None] * yf).astype(np.float32)).astype(np.float32)
     small = s <= S_MIN
-    pred[small] = ez0[small]
+    pred[small] = b[small]
     return pred
 
 

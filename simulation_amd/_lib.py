@@ -125,6 +125,8 @@ SIGNATURES = {
     "fd2d_pmlparam": (_I, [_I, _I, _I, C.POINTER(FdtdPml), _P]),
     "fd2d_dielectric": (_I, [_I, _I, _I, _D, _I, _DP, _I, _P, _P, _P]),
     "fd2d_energy": (_I, [_I, _I, _I, _I, _I, _I, _P, _P, _P, _P, _DP, _P]),
+    "fd2d_energy_diff": (_I, [_I, _I, _I, _I, _I, _I, _P, _P, _P, _I, _I, _I, _I, _P, _P, _P, _P,
+                              _DP, _P]),
     "fd2d_tm_run": (_I, [C.POINTER(Fd2dTmDesc), _I, _I, _P]),
     "fd2d_tm_plan": (_I, [C.POINTER(Fd2dTmDesc), _I, _IP, _IP, _IP, _IP]),
     "fd2d_batch_units": (_I, [C.POINTER(Fd2dGrid), _I, _I, _I, _I, _IP]),
@@ -143,7 +145,8 @@ SIGNATURES = {
     "lbc_sample": (_I, [_I, _I, _I, _P, _P, _P, _P, _P]),
     "lbc_pack": (_I, [_P, _P, _P]),
     "lbc_forward": (_I, [_I, _P, _P, C.c_longlong, _P, _P, _P, _P, _P]),
-    "lbc_train_grad": (_I, [_I, _P, _P, C.c_longlong, _P, _F, _P, _P, _I, _P, _P, _P]),
+    "lbc_train_grad": (_I, [_I, _P, _P, C.c_longlong, _P, _F, _P, _P, _I, _P, _P, _F, C.c_uint,
+                            _P]),
     "lbc_adam": (_I, [_I, _P, _P, _P, _P, _I, _F, _F, _F, _F, _P, _P, _P]),
 }
 

@@ -241,6 +241,29 @@ fd2d_energy_partial(int ny, int i0, int i1, int j0, int j1, const float *__restr
     if (threadIdx.x == 0) work[1 + blockIdx.x] = r;
 }
 
+// Energy of the difference of two field sets over equal windows: window
+// (ia0, ja0) of grid a (row length nya) against (ib0, jb0) of grid b.
+__global__ void __launch_bounds__(ET)
+fd2d_energy_diff_partial(int wn, int wm, int nya, int ia0, int ja0, const float *__restrict__ eza,
+                         const float *__restrict__ hxa, const float *__restrict__ hya, int nyb,
+                         int ib0, int jb0, const float *__restrict__ ezb,
+                         const float *__restrict__ hxb, const float *__restrict__ hyb,
+                         double *__restrict__ work)
+{
+    const size_t ncell = (size_t)wn * wm;
+    double acc = 0.0;
+    for (size_t k = (size_t)blockIdx.x * ET + threadIdx.x; k < ncell; k += (size_t)EB * ET) {
+        const size_t r = k / wm, c = k % wm;
+        const size_t na = (ia0 + r) * nya + (ja0 + c), nb = (ib0 + r) * nyb + (jb0 + c);
+        const double e = (double)eza[na] - (double)ezb[nb];
+        const double a = (double)hxa[na] - (double)hxb[nb];
+        const double b = (double)hya[na] - (double)hyb[nb];
+        acc += e * e + a * a + b * b;
+    }
+    const double r = block_sum(acc);
+    if (threadIdx.x == 0) work[1 + blockIdx.x] = r;
+}
+
 __global__ void __launch_bounds__(ET) fd2d_energy_final(double *__restrict__ work)
 {
     double acc = 0.0;
@@ -329,6 +352,19 @@ int launch_fd2d_energy(int nx, int ny, int i0, int i1, int j0, int j1, const flo
     if (rc) return rc;
     hipLaunchKernelGGL(fd2d_energy_final, dim3(1), dim3(ET), 0, s, work);
     return hip_check(hipGetLastError(), "fd2d_energy launch");
+}
+
+int launch_fd2d_energy_diff(int wn, int wm, int nya, int ia0, int ja0, const float *eza,
+                            const float *hxa, const float *hya, int nyb, int ib0, int jb0,
+                            const float *ezb, const float *hxb, const float *hyb, double *work,
+                            hipStream_t s)
+{
+    hipLaunchKernelGGL(fd2d_energy_diff_partial, dim3(EB), dim3(ET), 0, s, wn, wm, nya, ia0, ja0,
+                       eza, hxa, hya, nyb, ib0, jb0, ezb, hxb, hyb, work);
+    int rc = hip_check(hipGetLastError(), "fd2d_energy_diff launch");
+    if (rc) return rc;
+    hipLaunchKernelGGL(fd2d_energy_final, dim3(1), dim3(ET), 0, s, work);
+    return hip_check(hipGetLastError(), "fd2d_energy_diff launch");
 }
 
 }  // namespace fdtd

@@ -431,6 +431,27 @@ int fd2d_energy(int nx, int ny, int i0, int i1, int j0, int j1, const float *ez,
     return hip_check(hipStreamSynchronize(S(stream)), "fd2d_energy sync");
 }
 
+int fd2d_energy_diff(int wn, int wm, int nxa, int nya, int ia0, int ja0, const float *eza,
+                     const float *hxa, const float *hya, int nxb, int nyb, int ib0, int jb0,
+                     const float *ezb, const float *hxb, const float *hyb, double *work,
+                     double *result, void *stream)
+{
+    clear_error();
+    if (!eza || !hxa || !hya || !ezb || !hxb || !hyb || !work)
+        return fail("fd2d_energy_diff: null pointer");
+    if (wn < 0 || wm < 0 || ia0 < 0 || ja0 < 0 || ib0 < 0 || jb0 < 0 || ia0 + wn > nxa ||
+        ja0 + wm > nya || ib0 + wn > nxb || jb0 + wm > nyb)
+        return fail("fd2d_energy_diff: window %dx%d at (%d,%d)/(%d,%d) outside %dx%d / %dx%d", wn,
+                    wm, ia0, ja0, ib0, jb0, nxa, nya, nxb, nyb);
+    int rc = launch_fd2d_energy_diff(wn, wm, nya, ia0, ja0, eza, hxa, hya, nyb, ib0, jb0, ezb, hxb,
+                                     hyb, work, S(stream));
+    if (rc || !result) return rc;
+    rc = hip_check(hipMemcpyAsync(result, work, sizeof(double), hipMemcpyDeviceToHost, S(stream)),
+                   "fd2d_energy_diff copy");
+    if (rc) return rc;
+    return hip_check(hipStreamSynchronize(S(stream)), "fd2d_energy_diff sync");
+}
+
 static int tm_validate(const fd2d_tm_desc *d, int t0, int nsteps)
 {
     if (!d) return fail("fd2d_tm: desc is NULL");
@@ -582,7 +603,7 @@ int fd2d_tm_run(const fd2d_tm_desc *d, int t0, int nsteps, void *stream)
 size_t lbc_hist_elems(int nx, int ny)
 {
     if (nx < 8 || ny < 8) return 0;
-    return (size_t)(2 * ny + 2 * (nx - 6)) * 4 * 27;
+    return (size_t)(2 * ny + 2 * (nx - 6)) * 4 * 29;
 }
 
 int fd2d_lbc_margin(void) { return FDTD_LBC_MARGIN; }

@@ -80,6 +80,10 @@ struct LbcFrameParams {
     const uint16_t *w;                          // network: bf16 operands
     const float *params;                        //          FP32 parameters (biases)
 };
+int launch_fd2d_energy_diff(int wn, int wm, int nya, int ia0, int ja0, const float *eza,
+                            const float *hxa, const float *hya, int nyb, int ib0, int jb0,
+                            const float *ezb, const float *hxb, const float *hyb, double *work,
+                            hipStream_t s);
 size_t lbc_frame_smem();
 int lbc_frame_blocks(int nx, int ny, int K);
 int launch_lbc_frame(const LbcFrameParams &p, hipStream_t s);

@@ -15,7 +15,7 @@ namespace {
 constexpr int MS = 72;     // transposed tiles: 64 rows of a round + 8
 constexpr int DS = 40;     // dy rows (32 + 8)
 
-// Samples of an nx x ny field window (row stride ld) -> out[npos][27].
+// Samples of an nx x ny field window (row stride ld) -> out[npos][NS].
 __global__ void __launch_bounds__(256) sample_kernel(int nx, int ny, int ld, const float *ez,
                                                      const float *hx, const float *hy, float *out)
 {
@@ -33,6 +33,9 @@ __global__ void __launch_bounds__(256) sample_kernel(int nx, int ny, int ld, con
         for (int f = 0; f < 3; ++f)
             for (int d = 0; d < 3; ++d) o[sidx(f, d, oo)] = canon(acc, e, f, d, oo, tt, nx, ny);
     }
+    o[S_EZ2] = canon(acc, e, 0, -1, 0, t, nx, ny);
+    o[S_EZ1] = canon(acc, e, 0, -2, 0, t, nx, ny);
+    o[S_EZ0] = canon(acc, e, 0, -3, 0, t, nx, ny);
 }
 
 // bf16 operand buffer from FP32 parameters (padding written as zero).
@@ -63,15 +66,15 @@ __global__ void __launch_bounds__(256) pack_kernel(const float *p, uint16_t *w)
 // offs[row] - s * tstride.
 __device__ __forceinline__ void load_sample(const float *src, const long long *offs,
                                             long long tstride, int n, int row, int g,
-                                            float (&v)[NS])
+                                            float (&v)[NF])
 {
     if (row < n) {
         const float *q = src + offs[row] - (long long)g * tstride;
 #pragma unroll
-        for (int i = 0; i < NS; ++i) v[i] = q[i];
+        for (int i = 0; i < NF; ++i) v[i] = q[i];
     } else {
 #pragma unroll
-        for (int i = 0; i < NS; ++i) v[i] = 0.0f;
+        for (int i = 0; i < NF; ++i) v[i] = 0.0f;
     }
 }
 
@@ -87,23 +90,25 @@ __global__ void __launch_bounds__(256) forward_kernel(int n, const float *src,
     const Net N{w, p};
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, c = lane & 15, g = lane >> 4;
     const int r0 = wave * 16;
-    for (int base = blockIdx.x * 64; base < n; base += gridDim.x * 64) {
-        float v[NS];
-        load_sample(src, offs, tstride, n, base + r0 + c, g, v);
+    // 32 samples per round: tile rows c and c + 8 hold sample 8 wave + (c & 7)
+    // and its negation
+    for (int base = blockIdx.x * 32; base < n; base += gridDim.x * 32) {
+        const int row = base + wave * 8 + (c & 7);
+        float v[NF];
+        load_sample(src, offs, tstride, n, row, g, v);
         float s, inv;
         store_features(v, Xh, r0, s, inv);
-        if (g == 0 && base + r0 + c < n && s_out) s_out[base + r0 + c] = s;
+        if (g == 0 && c < 8 && row < n && s_out) s_out[row] = s;
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         float z1[16], z2[16], yy[4];
         forward_tile<false>(N, Xh, A1, A2, r0, z1, z2, yy);
-        if (c < OUT) {
 #pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                int row = base + r0 + g * 4 + k;
-                if (row < n) y[(long)row * OUT + c] = yy[k];
-            }
+        for (int k = 0; k < 4; ++k) {
+            const float yo = odd_out(yy[k]);
+            const int rr = base + wave * 8 + g * 4 + k;
+            if (g < 2 && c < OUT && rr < n) y[(long)rr * OUT + c] = yo;
         }
         __syncthreads();
     }
@@ -115,19 +120,20 @@ struct TrainSmem {
     uint16_t A1[64 * AS], A1T[HID * MS], A2[64 * AS], A2T[HID * MS];
     uint16_t DY[64 * DS], DYT[16 * MS];
     uint16_t DZ2[64 * AS], DZ2T[HID * MS], DZ1T[HID * MS];
-    float th[64][4];
+    float th[32][4];
     float red[4][HID];
 };
 
-// One block: rounds of 64 samples (4 waves x 16), forward + backward through
-// bf16 MFMA; weight gradients accumulate in MFMA registers over the rounds
-// (K = the 64 samples of a round); writes this block's partial gradient
-// (NPARAM floats) and loss.
+// One block: rounds of 32 samples as 64 MFMA rows (4 waves x 16; tile rows
+// c and c + 8 hold a sample and its negation, for the odd network), forward
+// + backward through bf16 MFMA; weight gradients accumulate in MFMA
+// registers over the rounds (K = the 64 rows of a round); writes this
+// block's partial gradient (NPARAM floats) and loss.
 __global__ void __launch_bounds__(256) train_kernel(int n, const float *src,
                                                     const long long *offs, long long tstride,
                                                     const float *tgt, float norm, const float *p,
                                                     const uint16_t *w, float *part,
-                                                    float *part_loss)
+                                                    float *part_loss, float noise, uint32_t seed)
 {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
     TrainSmem &S = *reinterpret_cast<TrainSmem *>(smem_raw);
@@ -146,37 +152,46 @@ __global__ void __launch_bounds__(256) train_kernel(int n, const float *src,
     float gb1[4] = {0, 0, 0, 0}, gb2[4] = {0, 0, 0, 0}, gb3 = 0.0f, lsum = 0.0f;
     const float sc = 2.0f / norm;
 
-    for (int base = blockIdx.x * 64; base < n; base += gridDim.x * 64) {
-        // gather + scale (row r0 + c, sample g)
-        const int row = base + r0 + c;
-        float v[NS];
+    for (int base = blockIdx.x * 32; base < n; base += gridDim.x * 32) {
+        // gather + scale (tile row c: sample wave*8 + (c & 7), age g)
+        const int si = wave * 8 + (c & 7);
+        const int row = base + si;
+        float v[NF];
         load_sample(src, offs, tstride, n, row, g, v);
         float s, inv;
-        store_features(v, S.Xh, r0, s, inv, S.XhT, MS);
-        if (g == 0) {
+        store_features(v, S.Xh, r0, s, inv, S.XhT, MS, noise, (uint32_t)row, seed);
+        if (g == 0 && c < 8) {
             float wgt = (row < n && s > S_MIN) ? 1.0f : 0.0f;
 #pragma unroll
             for (int d = 0; d < OUT; ++d) {
-                float T = 0.0f;
-                if (row < n) T = tgt ? tgt[(long)row * OUT + d] : src[offs[row] + tstride + sidx(0, d, 0)];
-                S.th[r0 + c][d] = (T - v[sidx(0, d, 0)]) * inv;
+                float T = 0.0f, b = 0.0f;
+                if (row < n) {
+                    const float *nxt = src + offs[row] + tstride;
+                    T = tgt ? tgt[(long)row * OUT + d]
+                            : nxt[d == 2 ? S_EZ2 : (d == 1 ? S_EZ1 : S_EZ0)];
+                    b = base_of(d, src + offs[row] - tstride);
+                }
+                S.th[si][d] = (T - b) * inv;
             }
-            S.th[r0 + c][3] = wgt;
+            S.th[si][3] = wgt;
         }
         __syncthreads();
 
         float z1[16], z2[16], y[4];
         forward_tile<true>(N, S.Xh, S.A1, S.A2, r0, z1, z2, y, S.A1T, S.A2T, MS);
 
-        // loss gradient dy (rows r0+4g+k, output c)
+        // loss gradient: e of sample wave*8 + (4g + k) & 7 through the odd
+        // output; row dy = +-e * (2 / norm) / 2 (the f(x) and f(-x) branches)
         float dy[4];
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
+            const float yo = odd_out(y[k]);
             const int rr = r0 + g * 4 + k;
+            const int sj = wave * 8 + ((g * 4 + k) & 7);
             float e = 0.0f;
-            if (c < OUT) e = (y[k] - S.th[rr][c]) * S.th[rr][3];
-            lsum += e * e;
-            dy[k] = e * sc;
+            if (c < OUT) e = (yo - S.th[sj][c]) * S.th[sj][3];
+            if (g < 2) lsum += e * e;
+            dy[k] = (g < 2 ? 0.5f : -0.5f) * (e * sc);
             gb3 += dy[k];
             uint16_t h = f2bf(dy[k]);
             S.DY[rr * DS + c] = h;
@@ -363,7 +378,7 @@ int lbc_forward(int n, const float *src, const long long *offs, long long tstrid
     if (n < 0) return fail("lbc_forward: n=%d", n);
     if (n == 0) return 0;
     if (!src || !offs || !params || !wbuf || !y) return fail("lbc_forward: null pointer");
-    int blocks = (n + 63) / 64;
+    int blocks = (n + 31) / 32;
     if (blocks > 4096) blocks = 4096;
     hipLaunchKernelGGL(forward_kernel, dim3(blocks), dim3(256), 0, S(stream), n, src, offs,
                        tstride, params, wbuf, y, s);
@@ -372,8 +387,10 @@ int lbc_forward(int n, const float *src, const long long *offs, long long tstrid
 
 int lbc_train_grad(int n, const float *src, const long long *offs, long long tstride,
                    const float *tgt, float norm, const float *params, const uint16_t *wbuf,
-                   int nblocks, float *part, float *part_loss, void *stream)
+                   int nblocks, float *part, float *part_loss, float noise, unsigned seed,
+                   void *stream)
 {
+    if (!(noise >= 0.0f)) return fail("lbc_train_grad: noise=%g", noise);
     if (n <= 0 || nblocks <= 0) return fail("lbc_train_grad: n=%d nblocks=%d", n, nblocks);
     if (!(norm > 0.0f)) return fail("lbc_train_grad: norm must be positive");
     if (!src || !offs || !params || !wbuf || !part) return fail("lbc_train_grad: null pointer");
@@ -387,7 +404,8 @@ int lbc_train_grad(int n, const float *src, const long long *offs, long long tst
         attr = true;
     }
     hipLaunchKernelGGL(train_kernel, dim3(nblocks), dim3(256), sizeof(TrainSmem), S(stream), n,
-                       src, offs, tstride, tgt, norm, params, wbuf, part, part_loss);
+                       src, offs, tstride, tgt, norm, params, wbuf, part, part_loss, noise,
+                       (uint32_t)seed);
     return hip_check(hipGetLastError(), "lbc_train_grad");
 }
 

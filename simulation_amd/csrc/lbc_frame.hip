@@ -9,13 +9,14 @@
 // and right edges H at depth d takes ez at depth d-1, so H reaches K+2):
 // after K steps the blocked launch is exact at depth >= K+3.
 //
-// Each workgroup owns one 48-cell chunk of one edge's frame (depth K+3) and
+// Each workgroup owns one 40-cell chunk of one edge's frame (depth K+3) and
 // evolves, in LDS, the chunk's dependency region.  Per step the valid part
 // of a region shrinks by one cell on a cut side, except along an edge in the
 // direction of increasing index, where it shrinks by two: H at tau takes ez
 // at tau+1, whose prediction reads the samples at tau+2 (Yee staggering).
-// The samples handed to the next launch reach one cell beyond the chunk.
-// Regions: depth 2K+4 (3K+4 for the two corner chunks of the top edge, whose
+// The samples handed to the next launch reach one cell beyond the chunk and
+// down to grid depth 6 (the bottom edge's H between depths 5 and 6).
+// Regions: depth 2K+6 (3K+6 for the two corner chunks of the top edge, whose
 // side-edge positions run down into the cut), widened along the edge by K+1
 // cells before and 2K+1 after the chunk.  Per step: network for every boundary position
 // whose cells lie in the region (features = the position's last four
@@ -35,9 +36,13 @@ namespace fdtd {
 namespace lbc {
 namespace {
 
-constexpr int CH = 48;                        // chunk length along the edge
+constexpr int CH = 40;                        // chunk length along the edge
 constexpr int KMAX = 8;
-constexpr int RMAX = (3 * KMAX + 4) * (CH + 3 * KMAX + 2);   // region cells (28 x 74)
+// region cells: corner chunks (3K+6) x (CH + 2K + 1), others at most
+// (2K+6) x (CH + 3K + 2) (30 x 57 and 22 x 66 at K = 8)
+constexpr int RMAX_C = (3 * KMAX + 6) * (CH + 2 * KMAX + 1);
+constexpr int RMAX_E = (2 * KMAX + 6) * (CH + 3 * KMAX + 2);
+constexpr int RMAX = RMAX_C > RMAX_E ? RMAX_C : RMAX_E;
 constexpr int PMAX = 128;                     // positions per workgroup
 
 struct FrameSmem {
@@ -46,10 +51,12 @@ struct FrameSmem {
     uint16_t Xh[64 * XS];
     uint16_t A1[64 * AS], A2[64 * AS];
     float pred[PMAX][OUT];
-    float srow[64];
+    float srow[32];
     int pos[PMAX];          // edge << 24 | tau
     int own[PMAX];          // global history index, or -1 when not owned
 };
+
+static_assert(sizeof(FrameSmem) + 64 <= 160 * 1024, "frame kernel LDS above 160 KiB");
 
 struct Geo {
     int ri0, rj0, RH, RW;           // region origin and size (global)
@@ -85,10 +92,10 @@ __global__ void __launch_bounds__(256) frame_kernel(LbcFrameParams P)
         else if (b < 2 * nct + ncl) { kind = 2; q = b - 2 * nct; }
         else { kind = 3; q = b - 2 * nct - ncl; }
     }
-    int R0 = 2 * K + 4;
+    int R0 = 2 * K + 6;
     if (kind < 2) {
         const int c0 = q * CH, c1 = min(ny, c0 + CH);
-        if (kind == 0 && (q == 0 || q == nct - 1)) R0 = 3 * K + 4;
+        if (kind == 0 && (q == 0 || q == nct - 1)) R0 = 3 * K + 6;
         G.rj0 = max(0, c0 - K - 1);
         G.RW = min(ny, c1 + 2 * K + 1) - G.rj0;
         G.RH = R0;
@@ -109,7 +116,7 @@ __global__ void __launch_bounds__(256) frame_kernel(LbcFrameParams P)
     const int RH = G.RH, RW = G.RW, NC = RH * RW;
 
     // ---------------------------------------------------------- positions
-    // (thread 0 builds the list; at most CH + 3K + 2 + 2 (3K) entries)
+    // (thread 0 builds the list; at most CH + 3K + 2 + 2 (3K + 3) = 120)
     __shared__ int np_s;
     if (tid == 0) {
         int n = 0;
@@ -173,33 +180,35 @@ __global__ void __launch_bounds__(256) frame_kernel(LbcFrameParams P)
 
     for (int lev = 1; lev <= K; ++lev) {
         // ------------------------------------------------------ network
-        for (int base = 0; base < np; base += 64) {
+        // 32 positions per round: tile rows c and c + 8 hold position
+        // base + 8 wave + (c & 7) and its negation (odd network)
+        for (int base = 0; base < np; base += 32) {
             const int r0 = wave * 16;
-            const int p = base + r0 + c;
-            float v[NS];
+            const int p = base + wave * 8 + (c & 7);
+            float v[NF];
             if (p < np) {
                 const float *src = S.ring[p][(lev - 1 - g) & 3];
 #pragma unroll
-                for (int i = 0; i < NS; ++i) v[i] = src[i];
+                for (int i = 0; i < NF; ++i) v[i] = src[i];
             } else {
 #pragma unroll
-                for (int i = 0; i < NS; ++i) v[i] = 0.0f;
+                for (int i = 0; i < NF; ++i) v[i] = 0.0f;
             }
             float s, inv;
             store_features(v, S.Xh, r0, s, inv);
-            if (g == 0) S.srow[r0 + c] = s;
+            if (g == 0 && c < 8) S.srow[wave * 8 + c] = s;
             __syncthreads();
             float z1[16], z2[16], y[4];
             forward_tile<false>(N, S.Xh, S.A1, S.A2, r0, z1, z2, y);
-            if (c < OUT) {
 #pragma unroll
-                for (int k = 0; k < 4; ++k) {
-                    const int pp = base + r0 + g * 4 + k;
-                    if (pp < np) {
-                        const float sr = S.srow[r0 + g * 4 + k];
-                        const float ez0 = S.ring[pp][(lev - 1) & 3][sidx(0, c, 0)];
-                        S.pred[pp][c] = sr > S_MIN ? ez0 + sr * y[k] : ez0;
-                    }
+            for (int k = 0; k < 4; ++k) {
+                const float yo = odd_out(y[k]);
+                const int q8 = wave * 8 + g * 4 + k;          // valid for g < 2
+                const int pp = base + q8;
+                if (g < 2 && c < OUT && pp < np) {
+                    const float sr = S.srow[q8];
+                    const float b = base_of(c, S.ring[pp][(lev - 2) & 3]);
+                    S.pred[pp][c] = sr > S_MIN ? b + sr * yo : b;
                 }
             }
             __syncthreads();
@@ -249,10 +258,16 @@ __global__ void __launch_bounds__(256) frame_kernel(LbcFrameParams P)
         for (int k = tid; k < np * NS; k += 256) {
             const int p = k / NS, i = k % NS;
             const int e = S.pos[p] >> 24, t = S.pos[p] & 0xFFFFFF;
-            const int f = i / 9, d = (i / 3) % 3, o = i % 3 - 1;
-            const int L = e < 2 ? ny : nx;
-            const int tt = min(max(t + o, 0), L - 1);
-            S.ring[p][lev & 3][i] = canon(acc, e, f, d, o, tt, nx, ny);
+            float v;
+            if (i < NF) {
+                const int f = i / 9, d = (i / 3) % 3, o = i % 3 - 1;
+                const int L = e < 2 ? ny : nx;
+                const int tt = min(max(t + o, 0), L - 1);
+                v = canon(acc, e, f, d, o, tt, nx, ny);
+            } else {
+                v = canon(acc, e, 0, NF - 1 - i, 0, t, nx, ny);   // S_EZ2, S_EZ1, S_EZ0
+            }
+            S.ring[p][lev & 3][i] = v;
         }
         __syncthreads();
     }

@@ -3,11 +3,19 @@
 // time-stepping kernels.
 //
 // The network maps the last four canonical samples of an edge position
-// (4 x 27 = 108 features: ez, normal-coupled H, tangential-coupled H at 3
-// depths x 3 tangential offsets) to the next-step ez of the outer 3 cells:
-//   s = max|x|, xh = bf16(x * (1/s))  (rows with s <= 1e-30 predict ez_now),
-//   a1 = bf16(gelu(W1 xh + b1)), a2 = bf16(gelu(W2 a1 + b2)), y = W3 a2 + b3,
-//   ez_next[d] = ez_now[d] + s * y[d].
+// (4 x 27 = 108 features: ez, normal-coupled H, tangential-coupled H at grid
+// depths 2..4 x 3 tangential offsets) to the next-step ez of the outer 3
+// cells:
+//   s = max|x|, xh = bf16(x * (1/s))  (rows with s <= 1e-30 predict base),
+//   f(xh) = W3 bf16(gelu(W2 bf16(gelu(W1 xh + b1)) + b2)) + b3,
+//   y = (f(xh) - f(-xh)) / 2          (odd: the exact boundary map is linear),
+//   ez_next[d] = base[d] + s * y[d],
+// base = the two-step absorbing boundary of fd1d/pycuda/fd1d_1_2.py:53-61
+// applied to the three outer layers (depth d takes depth d+1's ez of two
+// steps earlier): the network learns a correction to it, and a network with
+// zero output is that classical boundary.  Inputs are the samples of grid
+// depths 3..5 (see canon).  In a 16-row MFMA tile, rows r and r + 8 hold a
+// sample and its negation.
 // Matrix products run on v_mfma_f32_16x16x32_bf16 (bf16 operands, FP32
 // accumulation).  No reference counterpart (the reference's boundaries are
 // the two-step 1-D ABC and the Sullivan PML).
@@ -18,7 +26,10 @@
 namespace fdtd {
 namespace lbc {
 
-constexpr int IN = 108, HID = 64, OUT = 3, NS = 27, HIST = 4;
+constexpr int IN = 108, HID = 64, OUT = 3, NF = 27, HIST = 4;
+// stored sample: the NF features, then ez of grid depths 2, 1, 0 at offset 0
+// (training targets and baseline; never network inputs)
+constexpr int NS = NF + 3, S_EZ2 = NF, S_EZ1 = NF + 1, S_EZ0 = NF + 2;
 // rows whose scale s = max|x| is at most S_MIN count as all-zero (no
 // prediction, no training weight): 1/s must stay finite
 constexpr float S_MIN = 1e-30f;
@@ -95,31 +106,68 @@ __host__ __device__ inline void cell_of(int e, int d, int t, int nx, int ny, int
     }
 }
 
-// Canonical value (field f, depth d, tangential offset o, tangential index
-// t = clamped tau + o) of edge e, read through acc(field, i, j) with field 0 ez, 1 hx,
-// 2 hy.  The four edges are mapped onto the top edge by the symmetries of
-// the Yee update (x-mirror: hy -> -hy shifted one row; transpose: hx -> -hy^T,
-// hy -> -hx^T):
-//   top:    ez[d,t],      hn =  hy[d,t],       ht =  hx[d,t]
-//   bottom: ez[nx-1-d,t], hn = -hy[nx-2-d,t],  ht =  hx[nx-1-d,t]
-//   left:   ez[t,d],      hn = -hx[t,d],       ht = -hy[t,d]
-//   right:  ez[t,ny-1-d], hn =  hx[t,ny-2-d],  ht = -hy[t,ny-1-d]
-// ht is 0 at depth 0 on every edge and at tangential offset +1 (ht is
-// staggered half a cell along the edge: offsets -1 and 0 are the two H
-// points beside the cell), so a prediction reaches one cell to either side.
+// Canonical value (field f, feature depth d, tangential offset o, tangential
+// index t = clamped tau + o) of edge e, read through acc(field, i, j) with
+// field 0 ez, 1 hx, 2 hy.  Feature depth d is grid depth d + 3: the three
+// cells just inside the predicted ones, so the network never reads its own
+// output back (a closed loop through its own predictions drifts).
+// The four edges are mapped onto the top edge by the symmetries of the Yee
+// update (x-mirror: hy -> -hy shifted one row; transpose: hx -> -hy^T,
+// hy -> -hx^T); with D = d + 3:
+//   top:    ez[D,t],      hn =  hy[D,t],       ht =  hx[D,t]
+//   bottom: ez[nx-1-D,t], hn = -hy[nx-2-D,t],  ht =  hx[nx-1-D,t]
+//   left:   ez[t,D],      hn = -hx[t,D],       ht = -hy[t,D]
+//   right:  ez[t,ny-1-D], hn =  hx[t,ny-2-D],  ht = -hy[t,ny-1-D]
+// ht is 0 at tangential offset +1 (ht is staggered half a cell along the
+// edge: offsets -1 and 0 are the two H points beside the cell), so a
+// prediction reaches one cell to either side.  H in the last row or column
+// reads as 0: the H update never runs there (it stops at nx-2, ny-2), so a
+// window cut from a larger grid (training data) must not see it either.
+constexpr int FD0 = 3;      // grid depth of feature depth 0
 template <class Acc>
 __device__ __forceinline__ float canon(const Acc &acc, int e, int f, int d, int o, int t, int nx,
                                        int ny)
 {
-    if (f == 2 && (d == 0 || o == 1)) return 0.0f;
+    if (f == 2 && o == 1) return 0.0f;
+    const int D = d + FD0;
+    int fld, i, j;
+    float sg = 1.0f;
     switch (e) {
-    case 0: return f == 0 ? acc(0, d, t) : (f == 1 ? acc(2, d, t) : acc(1, d, t));
-    case 1: return f == 0 ? acc(0, nx - 1 - d, t)
-                          : (f == 1 ? -acc(2, nx - 2 - d, t) : acc(1, nx - 1 - d, t));
-    case 2: return f == 0 ? acc(0, t, d) : (f == 1 ? -acc(1, t, d) : -acc(2, t, d));
-    default: return f == 0 ? acc(0, t, ny - 1 - d)
-                           : (f == 1 ? acc(1, t, ny - 2 - d) : -acc(2, t, ny - 1 - d));
+    case 0: i = D; j = t; fld = f == 0 ? 0 : (f == 1 ? 2 : 1); break;
+    case 1:
+        j = t;
+        if (f == 0) { fld = 0; i = nx - 1 - D; }
+        else if (f == 1) { fld = 2; i = nx - 2 - D; sg = -1.0f; }
+        else { fld = 1; i = nx - 1 - D; }
+        break;
+    case 2: i = t; j = D; fld = f == 0 ? 0 : (f == 1 ? 1 : 2); sg = f == 0 ? 1.0f : -1.0f; break;
+    default:
+        i = t;
+        if (f == 0) { fld = 0; j = ny - 1 - D; }
+        else if (f == 1) { fld = 1; j = ny - 2 - D; }
+        else { fld = 2; j = ny - 1 - D; sg = -1.0f; }
+        break;
     }
+    if (fld != 0 && (i == nx - 1 || j == ny - 1)) return 0.0f;
+    const float v = acc(fld, i, j);
+    return sg < 0.0f ? -v : v;
+}
+
+// Baseline of output d (grid depth d) from the sample of age 1 (the state
+// before the current one): the two-step chain ez_d(t+1) = ez_{d+1}(t-1).
+__device__ __forceinline__ float base_of(int d, const float *s1)
+{
+    return d == 2 ? s1[sidx(0, 0, 0)] : (d == 1 ? s1[S_EZ2] : s1[S_EZ1]);
+}
+
+// Odd output of sample 4g + k (g < 2) of a tile from the accumulator-layout
+// y[k] of rows 4g + k and 4g + k + 8 (lane ^ 32): (f(x) - f(-x)) / 2.  All
+// lanes must call it; lanes with g >= 2 get the same value.
+__device__ __forceinline__ float odd_out(float yk)
+{
+    const int g = (threadIdx.x & 63) >> 4;
+    const float other = __shfl_xor(yk, 32);
+    return g < 2 ? 0.5f * (yk - other) : 0.5f * (other - yk);
 }
 
 struct Net {
@@ -128,27 +176,43 @@ struct Net {
 };
 
 // Scales and stores one 16-row tile's features.  Lane (r = lane&15,
-// g = lane>>4) holds sample g (27 raw values) of row r in v[]; writes xh to
-// Xh[(r0 + r) * XS + 27 g + i] (and columns 108..127 as zero).  Returns
+// g = lane>>4) holds the 27 features of sample g of row r in v[]; writes xh
+// to Xh[(r0 + r) * XS + 27 g + i] (and columns 108..127 as zero).  Returns
 // the row scale s and 1/s (0 when s <= S_MIN) to every lane of the row.
-__device__ __forceinline__ void store_features(const float (&v)[NS], uint16_t *Xh, int r0,
+// Counter-based uniform noise in [-1, 1) (integer hash of the arguments).
+__device__ __forceinline__ float hash_unit(uint32_t a, uint32_t b, uint32_t c)
+{
+    uint32_t h = a * 0x9E3779B1u ^ (b + 0x7F4A7C15u) * 0x85EBCA77u ^ c * 0xC2B2AE3Du;
+    h ^= h >> 15; h *= 0x2C1B3C6Du; h ^= h >> 12; h *= 0x297A2D39u; h ^= h >> 15;
+    return (float)(h >> 8) * (2.0f / 16777216.0f) - 1.0f;
+}
+
+// Rows r and r + 8 of a tile hold a sample and its negation (the caller
+// loads sample r & 7 into both).  With noise > 0 (training only) each
+// normalised input gets + noise * u, u uniform in [-1, 1) from
+// hash_unit(key, feature index, seed), negated with the sample.
+__device__ __forceinline__ void store_features(const float (&v)[NF], uint16_t *Xh, int r0,
                                                float &s, float &inv, uint16_t *XhT = nullptr,
-                                               int MS = 0)
+                                               int MS = 0, float noise = 0.0f,
+                                               uint32_t key = 0, uint32_t seed = 0)
 {
     const int lane = threadIdx.x & 63, r = lane & 15, g = lane >> 4;
+    const bool neg = r >= 8;
     float mx = 0.0f;
 #pragma unroll
-    for (int i = 0; i < NS; ++i) mx = fmaxf(mx, fabsf(v[i]));
+    for (int i = 0; i < NF; ++i) mx = fmaxf(mx, fabsf(v[i]));
     mx = fmaxf(mx, __shfl_xor(mx, 16));
     mx = fmaxf(mx, __shfl_xor(mx, 32));
     s = mx;
     inv = mx > S_MIN ? 1.0f / mx : 0.0f;
     uint16_t *row = Xh + (r0 + r) * XS;
 #pragma unroll
-    for (int i = 0; i < NS; ++i) {
-        uint16_t b = f2bf(v[i] * inv);
-        row[g * NS + i] = b;
-        if (XhT) XhT[(g * NS + i) * MS + r0 + r] = b;
+    for (int i = 0; i < NF; ++i) {
+        float x = v[i] * inv;
+        if (noise > 0.0f && inv > 0.0f) x = x + noise * hash_unit(key, g * NF + i, seed);
+        uint16_t b = f2bf(neg ? -x : x);
+        row[g * NF + i] = b;
+        if (XhT) XhT[(g * NF + i) * MS + r0 + r] = b;
     }
 #pragma unroll
     for (int q = 0; q < 5; ++q) row[IN + g * 5 + q] = 0;

@@ -5,9 +5,12 @@ The reference absorbs outgoing waves with a Sullivan PML
 (fd2d/pycuda/fd2d_3_2.py:126-135, 219-260); here a small network (108 -> 64
 -> 64 -> 3, GELU, bf16 weights, FP32 accumulation on MFMA) predicts the next
 ez of the outer three cells of every edge from the last four canonical
-boundary samples (27 values each: ez, normal- and tangential-coupled H over
-3 depths x 3 tangential offsets; all four edges share one network through
-the exact mirror/transpose symmetries of the Yee update).
+boundary samples (27 features each: ez, normal- and tangential-coupled H over
+grid depths 3..5 -- just inside the predicted cells -- x 3 tangential
+offsets; all four edges share one network through the exact mirror/transpose
+symmetries of the Yee update), as an odd correction, y(-x) = -y(x), to the
+reference's two-step boundary (fd1d/pycuda/fd1d_1_2.py:53-61) applied to the
+three outer layers.
 
 Training data come from wide-PML runs of the plain solver (npml=64): the
 boundary samples of an inner window are recorded every step and the network
@@ -28,12 +31,14 @@ from . import _lib, fd2d
 from ._dev import ptr, require_gpu, stream
 from ._lib import call
 
-IN, HID, OUT, NSAMP = 108, 64, 3, 27
+IN, HID, OUT = 108, 64, 3
+NFEAT = 27          # features per sample
+NSAMP = 30          # stored sample: features, then ez of grid depths 2, 1, 0
 
 
 def sidx(f: int, d: int, o: int) -> int:
     """Index of field f (0 ez, 1 normal-coupled H, 2 tangential-coupled H),
-    depth d and tangential offset o in a 27-value sample."""
+    feature depth d (grid depth d + 3) and tangential offset o in a sample."""
     return (f * 3 + d) * 3 + (o + 1)
 
 
@@ -96,14 +101,15 @@ class Net:
         nrm = float(norm if norm is not None else n)
         call("lbc_train_grad", n, ptr(src), offs.data_ptr(), int(tstride),
              None if tgt is None else ptr(tgt), nrm, ptr(self.params), self.wbuf.data_ptr(), nb,
-             ptr(part), ptr(ploss), stream())
+             ptr(part), ptr(ploss), 0.0, 0, stream())
         call("lbc_adam", nb, ptr(part), None, None, None, 0, 0.0, 0.9, 0.999, 1e-8, None, ptr(g),
              stream())
         return float(ploss.double().sum()) / nrm, g
 
     def train_step(self, src, offs, tstride, lr: float, tgt=None, nblocks: int = 0,
-                   part=None, ploss=None) -> torch.Tensor:
-        """One Adam step on the rows; returns the device loss partials."""
+                   part=None, ploss=None, noise: float = 0.0) -> torch.Tensor:
+        """One Adam step on the rows (inputs perturbed by `noise`, a fresh
+        pattern each step); returns the device loss partials."""
         n = offs.numel()
         nb = nblocks or max(1, min(512, (n + 255) // 256))
         if part is None or part.shape[0] < nb:
@@ -113,7 +119,7 @@ class Net:
         self.step += 1
         call("lbc_train_grad", n, ptr(src), offs.data_ptr(), int(tstride),
              None if tgt is None else ptr(tgt), float(n), ptr(self.params), self.wbuf.data_ptr(),
-             nb, ptr(part), ptr(ploss), stream())
+             nb, ptr(part), ptr(ploss), float(noise), self.step, stream())
         call("lbc_adam", nb, ptr(part), ptr(self.params), ptr(self.m), ptr(self.v), self.step,
              float(lr), 0.9, 0.999, 1e-8, self.wbuf.data_ptr(), None, stream())
         return ploss[:nb]
@@ -121,7 +127,7 @@ class Net:
 
 def sample(ez: torch.Tensor, hx: torch.Tensor, hy: torch.Tensor, out: torch.Tensor | None = None,
            window=None) -> torch.Tensor:
-    """Canonical boundary samples (npos, 27) of ez/hx/hy (or of the window
+    """Canonical boundary samples (npos, 30) of ez/hx/hy (or of the window
     (i0, i1, j0, j1) of them)."""
     nxg, nyg = ez.shape
     i0, i1, j0, j1 = window if window is not None else (0, nxg, 0, nyg)
@@ -178,6 +184,321 @@ class LearnedTMSolver:
         return self.tm.energy(window)
 
     def history(self) -> np.ndarray:
-        """(npos, 4, 27) boundary samples, newest first."""
+        """(npos, 4, 30) boundary samples, newest first."""
         torch.cuda.synchronize()
         return self.hist.view(-1, 4, NSAMP).cpu().numpy()
+
+
+# ----------------------------------------------------------------- training
+
+@dataclass
+class TrainConfig:
+    """Wide-PML training runs and the optimisation.
+
+    Each run is an N x N grid, N = window + 2 (npml + gap), with an npml=64
+    Sullivan PML; the learned boundary is trained for the inner window (its
+    edges gap cells clear of the PML).  Every run starts from a point pulse --
+    dz = exp(-r^2 / (2 w^2)), w U[1.5, 4] cells, all other fields zero -- at
+    least `margin` cells inside the window, with 0-3 lossy cylinders (eps_r
+    U[1,8], sigma U[0,0.05]) at least 10 cells clear of the window edge.  A
+    pulse rather than the reference's hard source (fd2d/pycuda/fd2d_3_1.py:69):
+    a hard source pins ez at its cell and leaves a static vortex of H around
+    it, a field that never radiates and whose far part depends on how the grid
+    is closed -- not what an absorbing boundary is for.  The window's boundary
+    samples are recorded every step."""
+    runs: int = 512
+    window: int = 160
+    npml: int = 64
+    gap: int = 8
+    steps: int = 520
+    margin: int = 32
+    seed: int = 0
+    train_steps: int = 8000
+    batch: int = 8192
+    lr: float = 2e-3
+    lr_min: float = 2e-5
+    noise: float = 0.0            # input noise amplitude during training
+    s_floor: float = 1e-6         # rows whose max|feature| is below this are dropped
+
+
+def training_specs(cfg: TrainConfig):
+    rng = np.random.default_rng(cfg.seed)
+    w, o = cfg.window, cfg.npml + cfg.gap
+    specs = []
+    for _ in range(cfg.runs):
+        width = float(rng.uniform(1.5, 4.0))
+        si, sj = (float(v) for v in rng.uniform(cfg.margin, w - cfg.margin, 2))
+        cyl = []
+        for _ in range(int(rng.integers(0, 4))):
+            r = float(rng.uniform(0.05, 0.15) * w)
+            cx, cy = (float(v) for v in rng.uniform(10 + r, w - 10 - r, 2))
+            cyl.append((cx + o, cy + o, r, float(rng.uniform(1.0, 8.0)), float(rng.uniform(0.0, 0.05))))
+        specs.append(dict(pulse=(si + o, sj + o, width), cylinders=cyl))
+    return specs
+
+
+def pulse(n: int, m: int, ci: float, cj: float, width: float, device) -> torch.Tensor:
+    """n x m FP32 field exp(-((i-ci)^2 + (j-cj)^2) / (2 width^2))."""
+    i = np.arange(n, dtype=np.float64)[:, None] - ci
+    j = np.arange(m, dtype=np.float64)[None, :] - cj
+    f = np.exp(-(i * i + j * j) / (2.0 * width * width)).astype(np.float32)
+    return torch.from_numpy(f).to(device)
+
+
+def generate(cfg: TrainConfig, device=None, log=None) -> torch.Tensor:
+    """Boundary samples of the training runs: G[run, 3 + t, position, 30] is
+    the window's sample after step t + 1 (the first three slices are the
+    fields at rest)."""
+    dev = device or require_gpu()
+    w, o = cfg.window, cfg.npml + cfg.gap
+    N = w + 2 * o
+    npw = npos(w, w)
+    G = torch.zeros((cfg.runs, cfg.steps + 3, npw, NSAMP), dtype=torch.float32, device=dev)
+    dt = fd2d.time_step()
+    for r, sp in enumerate(training_specs(cfg)):
+        pml = fd2d.new_pmlayer(N, N, dev)
+        fd2d.pmlparam(N, N, cfg.npml, pml)
+        md = fd2d.dielectric(N, N, cfg.npml, 0, dt, 1.0, 0.0, dev,
+                             cylinders=sp["cylinders"] or np.zeros((0, 5)))
+        s = fd2d.TMSolver(N, N, pml, md, [], [], nsteps_max=cfg.steps, device=dev)
+        s.dz.copy_(pulse(N, N, *sp["pulse"], dev))
+        for t in range(cfg.steps):
+            s.run(1)
+            sample(s.ez, s.hx, s.hy, out=G[r, 3 + t], window=(o, o + w, o, o + w))
+        if log and (r + 1) % 64 == 0:
+            torch.cuda.synchronize()
+            log(f"generated {r + 1}/{cfg.runs} runs")
+    torch.cuda.synchronize()
+    return G
+
+
+def training_rows(G: torch.Tensor, net: Net, s_floor: float) -> torch.Tensor:
+    """Offsets (into G) of the rows with a target (slices 3 .. T+1) whose
+    features exceed s_floor."""
+    R, T3, npw, _ = G.shape
+    ts = npw * NSAMP
+    keep = []
+    for r in range(R):
+        t = torch.arange(3, T3 - 1, device=G.device, dtype=torch.int64)
+        p = torch.arange(npw, device=G.device, dtype=torch.int64)
+        offs = ((r * T3 + t)[:, None] * npw + p[None, :]).reshape(-1) * NSAMP
+        _, s = net.forward(G, offs, ts)
+        keep.append(offs[s > s_floor])
+    return torch.cat(keep)
+
+
+def train(cfg: TrainConfig, G: torch.Tensor, net: Net | None = None, log=None) -> tuple:
+    """Adam on random batches of the recorded rows; cosine learning rate.
+    Returns (net, history [(step, train loss, validation loss)])."""
+    dev = G.device
+    net = net or Net(None, dev, seed=cfg.seed)
+    ts = G.shape[2] * NSAMP
+    rows = training_rows(G, net, cfg.s_floor)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(cfg.seed + 1)
+    perm = torch.randperm(rows.numel(), device=dev, generator=gen)
+    nval = min(65536, rows.numel() // 10)
+    val, trn = rows[perm[:nval]], rows[perm[nval:]]
+    nb = max(1, min(512, (cfg.batch + 255) // 256))
+    part = torch.empty((nb, net.nparam), dtype=torch.float32, device=dev)
+    ploss = torch.empty(nb, dtype=torch.float32, device=dev)
+    hist = []
+    if log:
+        v0, _ = net.grad(G, val, ts)
+        z = Net(np.zeros(net.nparam, np.float32), dev)
+        vb, _ = z.grad(G, val, ts)
+        log(f"{rows.numel()} rows ({trn.numel()} train, {nval} validation); "
+            f"validation loss: initial {v0:.3e}, baseline (zero network) {vb:.3e}")
+    for k in range(cfg.train_steps):
+        lr = cfg.lr_min + 0.5 * (cfg.lr - cfg.lr_min) * (1 + math.cos(math.pi * k / cfg.train_steps))
+        idx = torch.randint(0, trn.numel(), (cfg.batch,), device=dev, generator=gen)
+        pl = net.train_step(G, trn[idx], ts, lr, nblocks=nb, part=part, ploss=ploss,
+                            noise=cfg.noise)
+        if (k + 1) % 500 == 0 or k == cfg.train_steps - 1:
+            tl = float(pl.double().sum()) / cfg.batch
+            vl, _ = net.grad(G, val, ts)
+            hist.append((k + 1, tl, vl))
+            if log:
+                log(f"step {k + 1}: train {tl:.3e} validation {vl:.3e} lr {lr:.2e}")
+    return net, hist
+
+
+# --------------------------------------------------------------- evaluation
+
+def _solver(method: str, n: int, net, cylinders, start, steps, dev, **kw):
+    """(solver, window offset) of a boundary method around an n x n window.
+    start: ("pulse", ci, cj, width) -- initial dz pulse -- or ("source", i, j,
+    t0, spread) -- hard Gaussian source (fd2d/pycuda/fd2d_3_1.py:69);
+    coordinates relative to the window."""
+    from .fd1d import waveform
+    pad = {"learned": 0, "abc": 0, "pec": 0, "pml8": 8, "pml32": 32, "ref": 64 + 8}[method]
+    npml = {"pml8": 8, "pml32": 32, "ref": 64}.get(method, 0)
+    N = n + 2 * pad
+    dt = fd2d.time_step()
+    cyl = [(cx + pad, cy + pad, r, e, sg) for cx, cy, r, e, sg in cylinders]
+    md = fd2d.dielectric(N, N, npml, 0, dt, 1.0, 0.0, dev, cylinders=cyl or np.zeros((0, 5)))
+    src = []
+    if start[0] == "source":
+        _, si, sj, t0, spread = start
+        wave = waveform("gaussian", steps, 0, np.float32, center=t0, spread=spread)
+        src = [(si + pad, sj + pad, wave)]
+    if method in ("learned", "abc"):
+        use = net if method == "learned" else Net(np.zeros(_lib.lib().lbc_param_count(), np.float32), dev)
+        s = LearnedTMSolver(N, N, use, md, src, [], nsteps_max=steps, device=dev, **kw)
+        f = s.tm
+    else:
+        pml = None
+        if npml:
+            pml = fd2d.new_pmlayer(N, N, dev)
+            fd2d.pmlparam(N, N, npml, pml)
+        s = fd2d.TMSolver(N, N, pml, md, src, [], nsteps_max=steps, device=dev, **kw)
+        f = s
+    if start[0] == "pulse":
+        _, ci, cj, width = start
+        f.dz.copy_(pulse(N, N, ci + pad, cj + pad, width, dev))
+    return s, pad
+
+
+def _fields(s):
+    return (s.tm if isinstance(s, LearnedTMSolver) else s)
+
+
+def reflection(net: Net | None, n: int = 2048, steps: int | None = None, chunk: int = 100,
+               methods=("learned", "abc", "pml32", "pml8", "pec"), start=None, cylinders=None,
+               device=None, log=None) -> dict:
+    """Reflected energy of each boundary method on an n x n window.
+
+    Methods: learned (the network), abc (zero network: the two-step boundary
+    it corrects), pml32 / pml8 (Sullivan PML of that thickness outside the
+    window), pec (the bare grid).  Reference: the same window inside an
+    npml=64 grid with an 8-cell gap.  Every `chunk` steps the FP64 energy of
+    the difference between a method's window and the reference window is
+    taken (fd2d_energy_diff) over the window less its outer three cells (the
+    cells the learned boundary writes; the grid's last row and column also
+    hold no H there); the reflection ratio is its maximum over the run
+    divided by the peak energy of the reference on the same cells.  Defaults: a point
+    pulse (width 3 cells) at (n/2 + n/16, n/2 - n/32), one cylinder (radius
+    n/10, eps_r 4, sigma 0.02) centred at (n/2 - n/8, n/2 + n/8), and steps
+    long enough for the wave to leave through the corners."""
+    dev = device or require_gpu()
+    c = n // 2
+    source = start or ("pulse", c + n / 16, c - n / 32, 3.0)
+    cylinders = cylinders if cylinders is not None else [(c - n / 8, c + n / 8, n / 10, 4.0, 0.02)]
+    steps = steps or int(1.6 * n) + 400
+    steps -= steps % chunk
+    ref, rp = _solver("ref", n, net, cylinders, source, steps, dev)
+    sol = {m: _solver(m, n, net, cylinders, source, steps, dev) for m in methods}
+    work = torch.empty(_lib.ENERGY_WORK, dtype=torch.float64, device=dev)
+    err = {m: [] for m in methods}
+    eref = []
+    F = 3                  # measured window: the n x n window less its outer 3 cells
+    wn = n - 2 * F
+    for t in range(0, steps, chunk):
+        ref.run(chunk)
+        rf = _fields(ref)
+        eref.append(fd2d.energy(rf.ez, rf.hx, rf.hy, (rp + F, rp + F + wn, rp + F, rp + F + wn),
+                                work))
+        for m, (s, p) in sol.items():
+            s.run(chunk)
+            f = _fields(s)
+            out = C.c_double()
+            call("fd2d_energy_diff", wn, wn, f.nx, f.ny, p + F, p + F, ptr(f.ez), ptr(f.hx),
+                 ptr(f.hy), rf.nx, rf.ny, rp + F, rp + F, ptr(rf.ez), ptr(rf.hx), ptr(rf.hy),
+                 ptr(work, torch.float64), C.byref(out), stream())
+            err[m].append(out.value)
+    peak = max(eref)
+    res = {"n": n, "steps": steps, "peak_energy": peak, "final_ref_energy": eref[-1],
+           "ratio": {m: max(e) / peak for m, e in err.items()},
+           "final_ratio": {m: e[-1] / peak for m, e in err.items()},
+           "curve_t": list(range(chunk, steps + 1, chunk)),
+           "curve": {m: [x / peak for x in e] for m, e in err.items()},
+           "ref_curve": [x / peak for x in eref]}
+    if log:
+        log("reflection " + " ".join(f"{m}={r:.2e}" for m, r in res["ratio"].items()))
+    return res
+
+
+def speed(net: Net, n: int = 2048, steps: int = 2000, methods=("learned", "pml32"), device=None,
+          **kw) -> dict:
+    """Wall time of `steps` steps for each method around an n x n window:
+    raw cell-updates/s of its grid and window (physical) cell-updates/s."""
+    dev = device or require_gpu()
+    c = n // 2
+    src = ("pulse", c, c, 3.0)
+    cyl = [(c - n / 8, c + n / 8, n / 10, 4.0, 0.02)]
+    out = {}
+    for m in methods:
+        s, p = _solver(m, n, net, cyl, src, steps + 16, dev, **kw)
+        s.run(16)
+        torch.cuda.synchronize()
+        import time
+        t0 = time.perf_counter()
+        s.run(steps)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        N = n + 2 * p
+        out[m] = {"grid": N, "seconds": dt, "mcups_grid": N * N * steps / dt / 1e6,
+                  "mcups_window": n * n * steps / dt / 1e6}
+        del s
+        torch.cuda.empty_cache()
+    return out
+
+
+def save(net: Net, path: str, meta: dict | None = None) -> None:
+    """Parameters (FP32) and metadata as .npz (no pickled objects)."""
+    import json
+    np.savez(path, params=net.numpy(), meta=np.array(json.dumps(meta or {})))
+
+
+def load(path: str, device=None) -> Net:
+    d = np.load(path, allow_pickle=False)
+    return Net(d["params"], device)
+
+
+def main(argv=None) -> int:
+    """python -m simulation_amd.learned [--runs R] [--window W] [--steps T]
+    [--train-steps K] [--eval N] [--out weights.npz]: generate, train,
+    evaluate, save."""
+    import argparse
+    import json
+    import time
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--runs", type=int, default=512)
+    ap.add_argument("--window", type=int, default=160)
+    ap.add_argument("--steps", type=int, default=520)
+    ap.add_argument("--train-steps", type=int, default=8000)
+    ap.add_argument("--batch", type=int, default=8192)
+    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--noise", type=float, default=0.0)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--eval", type=int, nargs="*", default=[512])
+    ap.add_argument("--out", default="")
+    ap.add_argument("--report", default="")
+    a = ap.parse_args(argv)
+    t0 = time.perf_counter()
+
+    def log(m):
+        print(f"[{time.perf_counter() - t0:7.1f}s] {m}", flush=True)
+    cfg = TrainConfig(runs=a.runs, window=a.window, steps=a.steps, train_steps=a.train_steps,
+                      batch=a.batch, lr=a.lr, seed=a.seed, noise=a.noise)
+    dev = require_gpu()
+    G = generate(cfg, dev, log)
+    log(f"samples {tuple(G.shape)} ({G.numel() * 4 / 2**30:.1f} GiB)")
+    net, hist = train(cfg, G, None, log)
+    del G
+    torch.cuda.empty_cache()
+    rep = {"config": cfg.__dict__, "train": hist, "reflection": {}}
+    for n in a.eval:
+        r = reflection(net, n, device=dev, log=log)
+        rep["reflection"][str(n)] = {k: r[k] for k in ("n", "steps", "ratio", "final_ratio")}
+    if a.out:
+        save(net, a.out, {"config": cfg.__dict__, "train": hist})
+        log(f"saved {a.out}")
+    if a.report:
+        with open(a.report, "w") as f:
+            json.dump(rep, f, indent=1)
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())

@@ -1,12 +1,13 @@
 """Learned-boundary time stepping (fd2d_lbc_run) against the oracle's
 learned step (oracle/lbc_oracle.py: lbc_run).
 
-With a network whose output does not depend on its hidden layers (W3 = 0,
-bf16-exact b3) every prediction is ez_now + s * b3 in FP32 on both sides, so
-the whole path -- blocked launch, frame regions and ownership, history ring
-and its hand-over between launches -- must match bit for bit.  A generic
-network adds the MFMA's FP32 summation order (and the odd bf16 rounding
-flip of a hidden activation): rel L2 <= 1e-4 per field."""
+With a zero network every prediction is the baseline (the two-step chain
+ez_d(t+1) = ez_{d+1}(t-1)) in FP32 on both sides, so the whole path --
+blocked launch, frame regions and ownership, history ring and its hand-over
+between launches -- must match bit for bit.  (Constant outputs cannot be
+used for this: the network is odd, its biases cancel.)  A generic network
+adds the MFMA's FP32 summation order (and the odd bf16 rounding flip of a
+hidden activation): rel L2 <= 1e-4 per field."""
 import numpy as np
 import pytest
 import torch
@@ -51,11 +52,10 @@ def _gpu_run(nx, ny, md, params, runs, src, wave, probe, dev, **kw):
 
 
 @pytest.mark.parametrize("depth,rows,runs", [(8, 0, (60, 30)), (3, 16, (67, 8)), (8, 24, (2, 82))])
-def test_constant_network_bit_exact(gpu, depth, rows, runs):
+def test_zero_network_bit_exact(gpu, depth, rows, runs):
     nx, ny = 96, 112
-    print(f"[lbc solver] constant net depth={depth} rows={rows} runs={runs}", flush=True)
+    print(f"[lbc solver] zero net depth={depth} rows={rows} runs={runs}", flush=True)
     p = np.zeros(L.NPARAM, np.float32)
-    p[L.NPARAM - 3:] = [-0.125, 0.0625, -0.03125]
     md = _medium(nx, ny, gpu)
     src, probe = (34, 40), (40, 40)
     wave = _wave(sum(runs))
