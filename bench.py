@@ -60,6 +60,14 @@ def parse():
     ap.add_argument("--graph", action="store_true", help="batch: time one captured graph launch")
     ap.add_argument("--tune", action="store_true",
                     help="choose depth/rows/cols with the autotuner before timing")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run a plain 1 GiB device copy three times (rocprofv3 FETCH_SIZE/WRITE_SIZE "
+                         "scale) and exit")
+    ap.add_argument("--record-traffic", nargs=6, metavar=("KEY", "KERNEL", "FETCH_CSV", "WRITE_CSV",
+                                                          "COPY_FETCH_CSV", "COPY_WRITE_CSV"),
+                    help="host only: HBM bytes per launch of KERNEL from rocprofv3 --pmc "
+                         "counter_collection.csv files, scaled by the copy calibration, into "
+                         "profiles/traffic.json under KEY")
     a = ap.parse_args()
     if a.workload == "large":
         if a.size == 4096:
@@ -134,6 +142,63 @@ def traffic_for(workload: str):
     return None if rec is None else rec.get("bytes_per_launch")
 
 
+def copy_calibration() -> None:
+    """Three copies of 1 GiB with torch's device copy (for the counter scale)."""
+    import torch
+    n = 1 << 28
+    x = torch.ones(n, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": 4 * n, "copies": 3}), flush=True)
+
+
+def _counter_rows(path, name_part=None):
+    import csv
+    out = []
+    with open(path) as f:
+        for r in csv.DictReader(f):
+            if name_part is None or name_part in r["Kernel_Name"]:
+                out.append((int(r["Dispatch_Id"]), r["Kernel_Name"], float(r["Counter_Value"])))
+    return out
+
+
+def record_traffic(key, kernel, fetch_csv, write_csv, cfetch_csv, cwrite_csv) -> dict:
+    """Bytes per launch of `kernel` (every dispatch whose name contains it;
+    a launch of the split TM path is one interior + one boundary dispatch)
+    from FETCH_SIZE and WRITE_SIZE (KiB per dispatch), each scaled by what
+    the counter reads for the largest copy dispatch of a 1 GiB copy."""
+    cf = max(v for _, _, v in _counter_rows(cfetch_csv, "copy"))
+    cw = max(v for _, _, v in _counter_rows(cwrite_csv, "copy"))
+    sf, sw = 2.0 ** 30 / (cf * 1024), 2.0 ** 30 / (cw * 1024)
+    fr, wr = _counter_rows(fetch_csv, kernel), _counter_rows(write_csv, kernel)
+    # the launches of the timed depth: the kernel instances dispatched most
+    # often (the interior and boundary parts of a launch come in equal
+    # numbers; warm-up and tail launches of other depths are other instances)
+    count = {}
+    for _, n, _ in fr:
+        count[n] = count.get(n, 0) + 1
+    top = max(count.values())
+    parts = sorted(n for n, c in count.items() if c == top)
+
+    def per_launch(rows, scale):
+        return sum(sum(v for _, n, v in rows if n == p) / top for p in parts) * 1024 * scale
+    fetch, write = per_launch(fr, sf), per_launch(wr, sw)
+    rec = {"bytes_per_launch": int(fetch + write), "fetch_bytes": int(fetch), "write_bytes": int(write),
+           "fetch_scale": round(sf, 4), "write_scale": round(sw, 4), "launches": top,
+           "kernels": parts}
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    data = {}
+    if os.path.exists(path):
+        data = json.load(open(path))
+    data[key] = rec
+    with open(path, "w") as f:
+        json.dump(data, f, indent=1, sort_keys=True)
+    print(json.dumps({key: rec}))
+    return rec
+
+
 def bench_batch(a):
     """Batched workload: 256 square grids, sides uniform in [128, 1024] (seed
     0), 1-5 lossy cylinders each, npml=8, hard Gaussian point source, one probe
@@ -196,10 +261,15 @@ def bench_batch(a):
 
 def main():
     a = parse()
+    if a.record_traffic:
+        record_traffic(*a.record_traffic)
+        return
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus N>1 is not supported "
                                    "(the multi-GPU slab decomposition is a design, not code)"}))
         sys.exit(2)
+    if a.copy_calibration:
+        return copy_calibration()
     if a.workload == "batch":
         return bench_batch(a)
     import torch

@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "fdtd_internal.h"
+#include "lbc_impl.h"
 
 namespace fdtd {
 
@@ -603,7 +604,7 @@ int fd2d_tm_run(const fd2d_tm_desc *d, int t0, int nsteps, void *stream)
 size_t lbc_hist_elems(int nx, int ny)
 {
     if (nx < 8 || ny < 8) return 0;
-    return (size_t)(2 * ny + 2 * (nx - 6)) * 4 * 29;
+    return (size_t)lbc::npos(nx, ny) * lbc::HIST * lbc::NS;
 }
 
 int fd2d_lbc_margin(void) { return FDTD_LBC_MARGIN; }

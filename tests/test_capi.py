@@ -117,3 +117,15 @@ def test_product_path_has_no_oracle_or_cpu_fallback():
             if f.endswith((".py", ".hip", ".h", ".cpp")):
                 txt = open(os.path.join(dirpath, f)).read()
                 assert "oracle" not in txt.lower().replace("the oracle", ""), f
+
+
+def test_lbc_sizes_match_the_oracle_layout():
+    """Buffer sizes the learned-boundary entry points expect (the caller
+    allocates from these) agree with the sample layout of the oracle."""
+    from oracle import lbc_oracle as LO
+    from simulation_amd import _lib
+    L = _lib.lib()
+    assert L.lbc_param_count() == LO.NPARAM
+    for nx, ny in ((64, 64), (96, 112), (2048, 2048)):
+        assert L.lbc_hist_elems(nx, ny) == len(LO.positions(nx, ny)) * LO.HIST * LO.NSAMP
+    assert L.fd2d_lbc_margin() >= 3 * 8 + 6      # deepest frame region (3K+6 at K=8)
