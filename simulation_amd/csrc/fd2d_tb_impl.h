@@ -304,19 +304,22 @@ __device__ __forceinline__ void iterate(const TbParams &P, const Geom &g, const 
             Dpend = Dn;
         } else {
             // last level: D/E of row r and H of row q are final for this launch
-            if (r >= g.i0 && r < g.i1) {
+            // (cells within skip_w of an edge are left to another kernel)
+            const int sw = P.skip_w;
+            const int c0 = max(g.vj0, sw), c1 = min(g.vj1, ny - sw);
+            if (r >= g.i0 && r < g.i1 && r >= sw && r < P.nx - sw) {
                 const unsigned row = (unsigned)r * (unsigned)ny;
-                st<C, VEC>(P.dz_o, row, g.j0, g.vj0, g.vj1, Dn.dz);
-                if (LOSSY) st<C, VEC>(P.iz_o, row, g.j0, g.vj0, g.vj1, Dn.iz);
-                if (P.ez_o) st<C, VEC>(P.ez_o, row, g.j0, g.vj0, g.vj1, En);
+                st<C, VEC>(P.dz_o, row, g.j0, c0, c1, Dn.dz);
+                if (LOSSY) st<C, VEC>(P.iz_o, row, g.j0, c0, c1, Dn.iz);
+                if (P.ez_o) st<C, VEC>(P.ez_o, row, g.j0, c0, c1, En);
             }
-            if (q >= g.i0 && q < g.i1) {
+            if (q >= g.i0 && q < g.i1 && q >= sw && q < P.nx - sw) {
                 const unsigned row = (unsigned)q * (unsigned)ny;
-                st<C, VEC>(P.hx_o, row, g.j0, g.vj0, g.vj1, Hout.hx);
-                st<C, VEC>(P.hy_o, row, g.j0, g.vj0, g.vj1, Hout.hy);
+                st<C, VEC>(P.hx_o, row, g.j0, c0, c1, Hout.hx);
+                st<C, VEC>(P.hy_o, row, g.j0, c0, c1, Hout.hy);
                 if (PML) {
-                    st<C, VEC>(P.ihx_o, row, g.j0, g.vj0, g.vj1, Hout.ix);
-                    st<C, VEC>(P.ihy_o, row, g.j0, g.vj0, g.vj1, Hout.iy);
+                    st<C, VEC>(P.ihx_o, row, g.j0, c0, c1, Hout.ix);
+                    st<C, VEC>(P.ihy_o, row, g.j0, c0, c1, Hout.iy);
                 }
             }
         }
@@ -458,6 +461,7 @@ tm_tb_batch_kernel(const fd2d_grid *__restrict__ grids, const int *__restrict__ 
         P.probe_j[0] = G.probe_j;
         P.probe_out = G.probe_out;
         P.probe_len = G.probe_len;
+        P.skip_w = 0;
         const int js = sx * (S - 2 * halo) - halo;
         Geom g;
         g.j0 = js + (int)threadIdx.x * C;

@@ -137,8 +137,8 @@ int plan_tm(const fd2d_tm_desc *d, int nsteps, Plan *pl)
 // created on first use and kept for the life of the process (no device
 // memory is involved).
 struct Aux {
-    hipStream_t s = nullptr;
-    hipEvent_t fork = nullptr, join = nullptr;
+    hipStream_t s = nullptr, s2 = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr, join2 = nullptr;
 };
 
 int aux_for_current_device(Aux **out)
@@ -151,8 +151,10 @@ int aux_for_current_device(Aux **out)
     Aux &a = tab[dev];
     if (!a.s) {
         rc = hip_check(hipStreamCreateWithFlags(&a.s, hipStreamNonBlocking), "aux stream");
+        if (!rc) rc = hip_check(hipStreamCreateWithFlags(&a.s2, hipStreamNonBlocking), "aux stream");
         if (!rc) rc = hip_check(hipEventCreateWithFlags(&a.fork, hipEventDisableTiming), "aux event");
         if (!rc) rc = hip_check(hipEventCreateWithFlags(&a.join, hipEventDisableTiming), "aux event");
+        if (!rc) rc = hip_check(hipEventCreateWithFlags(&a.join2, hipEventDisableTiming), "aux event");
         if (rc) return rc;
     }
     *out = &a;
@@ -665,15 +667,11 @@ int fd2d_lbc_run(const fd2d_tm_desc *d, const fd2d_lbc_desc *b, int t0, int nste
         p.ihx_o = p.ihy_o = nullptr;
         p.ez_o = (l == L - 1) ? d->ez : nullptr;
         p.t_first = t0 + done + 1;
-        rc = hip_check(hipEventRecord(ax->fork, S(stream)), "fork event");
-        if (!rc) rc = hip_check(hipStreamWaitEvent(ax->s, ax->fork, 0), "fork wait");
-        if (!rc) rc = launch_fd2d_tb(p, k, pl.cols, lossy, false, 1, ax->s);
-        if (!rc) rc = launch_fd2d_tb(p, k, pl.cols, lossy, false, 0, S(stream));
-        if (!rc) rc = hip_check(hipEventRecord(ax->join, ax->s), "join event");
-        if (!rc) rc = hip_check(hipStreamWaitEvent(S(stream), ax->join, 0), "join wait");
-        if (rc) return rc;
-        // the frame of depth k+3, recomputed with the network, over the
-        // blocked launch's output
+        p.skip_w = k + 3;          // the frame kernel writes the outer k+3 cells
+        // the frame of depth k+3, recomputed with the network: it reads the
+        // launch's input set only and writes cells the blocked launch leaves
+        // alone, so the three kernels run concurrently (frame and boundary
+        // part on two library streams, interior part on the caller's)
         LbcFrameParams f{};
         f.nx = d->nx; f.ny = d->ny; f.K = k;
         f.dz_i = in[0]; f.iz_i = lossy ? in[1] : nullptr; f.hx_i = in[2]; f.hy_i = in[3];
@@ -682,7 +680,16 @@ int fd2d_lbc_run(const fd2d_tm_desc *d, const fd2d_lbc_desc *b, int t0, int nste
         f.naz = d->md.naz; f.nbz = d->md.nbz;
         f.hist_in = H[l % 2]; f.hist_out = H[(l + 1) % 2];
         f.w = b->wbuf; f.params = b->params;
-        rc = launch_lbc_frame(f, S(stream));
+        rc = hip_check(hipEventRecord(ax->fork, S(stream)), "fork event");
+        if (!rc) rc = hip_check(hipStreamWaitEvent(ax->s2, ax->fork, 0), "fork wait");
+        if (!rc) rc = hip_check(hipStreamWaitEvent(ax->s, ax->fork, 0), "fork wait");
+        if (!rc) rc = launch_lbc_frame(f, ax->s2);
+        if (!rc) rc = launch_fd2d_tb(p, k, pl.cols, lossy, false, 1, ax->s);
+        if (!rc) rc = launch_fd2d_tb(p, k, pl.cols, lossy, false, 0, S(stream));
+        if (!rc) rc = hip_check(hipEventRecord(ax->join, ax->s), "join event");
+        if (!rc) rc = hip_check(hipEventRecord(ax->join2, ax->s2), "join event");
+        if (!rc) rc = hip_check(hipStreamWaitEvent(S(stream), ax->join, 0), "join wait");
+        if (!rc) rc = hip_check(hipStreamWaitEvent(S(stream), ax->join2, 0), "join wait");
         if (rc) return rc;
         done += k;
     }

@@ -67,6 +67,7 @@ struct TbParams {
     int probe_i[FDTD_MAX_PROBES], probe_j[FDTD_MAX_PROBES];
     float *probe_out;
     int probe_len;
+    int skip_w;                    // leave cells within skip_w of an edge unwritten (0: none)
 };
 
 // Arguments of one learned-boundary frame launch (lbc_frame.hip).
