@@ -33,6 +33,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -53,10 +55,14 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=("tm", "large", "batch"), default="tm",
+    ap.add_argument("--workload", choices=("tm", "large", "batch", "learned"), default="tm",
                     help="tm: the N x N single-cylinder configuration (default, N=4096); "
                          "large: 16384^2, 16 cylinders, 4 sources, 5000 steps; "
-                         "batch: 256 uneven grids")
+                         "batch: 256 uneven grids; learned: 2048^2 learned boundary against "
+                         "npml=32 PML")
+    ap.add_argument("--weights", default="",
+                    help="learned: network parameters (.npz from simulation_amd.learned); "
+                         "default: the committed weights, else the zero network")
     ap.add_argument("--graph", action="store_true", help="batch: time one captured graph launch")
     ap.add_argument("--tune", action="store_true",
                     help="choose depth/rows/cols with the autotuner before timing")
@@ -259,6 +265,70 @@ def bench_batch(a):
     print(json.dumps(rec), flush=True)
 
 
+def bench_learned(a):
+    """2-D TM 2048^2 window, point pulse and one lossy cylinder: the learned
+    boundary (no PML; frame kernel + blocked launch) against the npml=32 PML
+    form (2112^2 grid, the PML outside the window), both timed over the same
+    steps.  value = window cell-updates/s of the learned form; the PML form's
+    rate and the ratio are reported beside it."""
+    import torch
+    from simulation_amd import learned as LB
+    dev = torch.device("cuda", 0)
+    n = a.size if a.size != 4096 else 2048
+    K = a.steps if a.steps != 2000 else 1600
+    W = a.warmup
+    wpath = a.weights or os.path.join(ROOT, "simulation_amd", "data", "lbc_weights.npz")
+    if os.path.exists(wpath):
+        net, wname = LB.load(wpath, dev), os.path.relpath(wpath, ROOT)
+    else:
+        net, wname = LB.Net(np.zeros(LB._lib.lib().lbc_param_count(), np.float32), dev), "zero network"
+    c = n // 2
+    start = ("pulse", c, c, 3.0)
+    cyl = [(c - n / 8, c + n / 8, n / 10, 4.0, 0.02)]
+    res = {}
+    for m in ("learned", "pml32"):
+        s, p = LB._solver(m, n, net, cyl, start, W + K, dev, depth=a.depth, rows=a.rows,
+                          cols=a.cols)
+        s.run(W)
+        torch.cuda.synchronize()
+        st = torch.cuda.current_stream()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        t0 = time.perf_counter()
+        e0.record(st)
+        s.run(K)
+        e1.record(st)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        N = n + 2 * p
+        res[m] = {"grid": N, "ms_per_step": wall * 1e3 / K, "events_ms": e0.elapsed_time(e1),
+                  "mcups_grid": N * N * K / wall / 1e6, "mcups_window": n * n * K / wall / 1e6}
+        del s
+        torch.cuda.empty_cache()
+    L = (K + 7) // 8
+    L += L % 2
+    launch_ms = res["learned"]["events_ms"] / L
+    bpl = 44 * n * n * K / L
+    achieved = bpl / (launch_ms * 1e-3) / 1e9
+    ratio = res["learned"]["mcups_window"] / res["pml32"]["mcups_window"]
+    rec = {
+        "metric": METRIC, "value": round(res["learned"]["mcups_window"], 1),
+        "unit": "Mcell-updates/s", "n_gpus": 1, "steps": K, "warmup": W,
+        "ms_per_step": round(res["learned"]["ms_per_step"], 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (point pulse, one lossy cylinder; fields start at zero)",
+        "config": {"workload": f"2-D TM {n}x{n} window, learned boundary ({wname}) vs npml=32 PML",
+                   "grid": [n, n], "learned": res["learned"], "pml32": res["pml32"],
+                   "speedup_vs_pml32_window": round(ratio, 3)},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4), "traffic": None,
+                     "kernel": "tm_tb_kernel + lbc frame_kernel (learned form, 44 B/cell)",
+                     "avg_launch_ms": round(launch_ms, 4), "alg_bytes_per_launch": int(bpl)},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(rec), flush=True)
+
+
 def main():
     a = parse()
     if a.record_traffic:
@@ -272,6 +342,8 @@ def main():
         return copy_calibration()
     if a.workload == "batch":
         return bench_batch(a)
+    if a.workload == "learned":
+        return bench_learned(a)
     import torch
     from simulation_amd import fd2d
 

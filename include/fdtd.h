@@ -319,8 +319,10 @@ int fdtd_graph_destroy(void *graph_exec);
  * base = the two-step absorbing boundary of fd1d/pycuda/fd1d_1_2.py:53-61 on
  * the three outer layers (ez_d(t+1) = ez_{d+1}(t-1)): a network with zero
  * output is that classical boundary.
+ * f(x) = W3 gelu(W2 gelu(W1 x + b1) + b2) + b3 + L x (a linear path beside
+ * the hidden layers), gelu(z) = z sigma(1.702 z), y = (f(x) - f(-x)) / 2.
  * Parameters: one FP32 vector of lbc_param_count() values, in the order
- * W1 (64x108, row-major), b1 (64), W2 (64x64), b2, W3 (3x64), b3; the
+ * W1 (64x108, row-major), b1 (64), W2 (64x64), b2, W3 (3x64), b3, L (3x108); the
  * kernels read bf16 copies from a buffer of lbc_wbuf_elems() uint16 values
  * written by lbc_pack (and kept current by lbc_adam).
  *
@@ -397,11 +399,12 @@ int fd2d_lbc_run(const fd2d_tm_desc *d, const fd2d_lbc_desc *b, int t0, int nste
                  void *stream);
 
 /* Reduces the partials; writes the gradient to grad_out (if not NULL); with
- * lr != 0 applies one Adam step (bias corrections for `step` >= 1) to params,
- * m, v and refreshes the bf16 copies in wbuf.                               */
+ * lr != 0 applies one Adam step (bias corrections for `step` >= 1) to the
+ * parameters [first, first + count) (count 0: all), their m, v, and
+ * refreshes their bf16 copies in wbuf.                                      */
 int lbc_adam(int nblocks, const float *part, float *params, float *m, float *v, int step,
              float lr, float beta1, float beta2, float eps, uint16_t *wbuf, float *grad_out,
-             void *stream);
+             int first, int count, void *stream);
 
 #ifdef __cplusplus
 }

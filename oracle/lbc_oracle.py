@@ -12,7 +12,8 @@ kernels have a checker:
 * the baseline (the reference's two-step absorbing boundary,
   fd1d/pycuda/fd1d_1_2.py:53-61, on the three outer layers) that the network
   corrects;
-* the odd network y = (f(x) - f(-x))/2, f: 108 -> 64 -> 64 -> 3 (GELU),
+* the odd network y = (f(x) - f(-x))/2, f: 108 -> 64 -> 64 -> 3 (GELU in
+  the sigmoid form z sigma(1.702 z)) plus a linear path L x from the inputs,
   with the GPU's rounding points
   (inputs and hidden activations rounded to bf16, weights bf16, products
   accumulated in FP32 on the GPU and FP64 here), and its backward pass;
@@ -38,7 +39,7 @@ NSAMP = 30                     # stored sample: features, then ez of grid depths
 S_EZ2, S_EZ1, S_EZ0 = 27, 28, 29
 FD0 = 3                        # grid depth of feature depth 0
 HIST = 4
-NPARAM = HID * IN + HID + HID * HID + HID + OUT * HID + OUT
+NPARAM = HID * IN + HID + HID * HID + HID + OUT * HID + OUT + OUT * IN
 
 
 def sidx(f: int, d: int, o: int) -> int:
@@ -60,12 +61,13 @@ def bf16(x) -> np.ndarray:
 # ------------------------------------------------------------------ params
 
 def unpack(p: np.ndarray) -> dict:
-    """Flat FP32 parameter vector -> W1 (64,108), b1, W2 (64,64), b2, W3 (3,64), b3."""
+    """Flat FP32 parameter vector -> W1 (64,108), b1, W2 (64,64), b2, W3 (3,64),
+    b3, L (3,108) (the linear path from the inputs to the outputs)."""
     p = np.asarray(p, dtype=np.float32)
     o = 0
     out = {}
     for name, shape in (("W1", (HID, IN)), ("b1", (HID,)), ("W2", (HID, HID)), ("b2", (HID,)),
-                        ("W3", (OUT, HID)), ("b3", (OUT,))):
+                        ("W3", (OUT, HID)), ("b3", (OUT,)), ("L", (OUT, IN))):
         n = int(np.prod(shape))
         out[name] = p[o:o + n].reshape(shape)
         o += n
@@ -75,7 +77,7 @@ def unpack(p: np.ndarray) -> dict:
 
 def pack(d: dict) -> np.ndarray:
     return np.concatenate([np.asarray(d[k], np.float32).reshape(-1)
-                           for k in ("W1", "b1", "W2", "b2", "W3", "b3")])
+                           for k in ("W1", "b1", "W2", "b2", "W3", "b3", "L")])
 
 
 def init_params(seed: int = 0) -> np.ndarray:
@@ -84,22 +86,29 @@ def init_params(seed: int = 0) -> np.ndarray:
     w1 = rng.uniform(-1, 1, (HID, IN)) * math.sqrt(3.0 / IN)
     w2 = rng.uniform(-1, 1, (HID, HID)) * math.sqrt(3.0 / HID)
     w3 = rng.uniform(-1, 1, (OUT, HID)) * math.sqrt(3.0 / HID) * 0.1
-    return pack(dict(W1=w1, b1=np.zeros(HID), W2=w2, b2=np.zeros(HID), W3=w3, b3=np.zeros(OUT)))
+    return pack(dict(W1=w1, b1=np.zeros(HID), W2=w2, b2=np.zeros(HID), W3=w3, b3=np.zeros(OUT),
+                     L=np.zeros((OUT, IN))))
 
 
 # ------------------------------------------------------------------ GELU
 
-_erf = np.vectorize(math.erf, otypes=[np.float64])
+GELU_A = 1.702
+
+
+def _sig(u):
+    return 1.0 / (1.0 + np.exp(-u))
 
 
 def gelu(z):
+    """GELU in the sigmoid form of Hendrycks & Gimpel (2016): z sigma(1.702 z)."""
     z = np.asarray(z, np.float64)
-    return 0.5 * z * (1.0 + _erf(z / math.sqrt(2.0)))
+    return z * _sig(GELU_A * z)
 
 
 def gelu_grad(z):
     z = np.asarray(z, np.float64)
-    return 0.5 * (1.0 + _erf(z / math.sqrt(2.0))) + z * np.exp(-0.5 * z * z) / math.sqrt(2 * math.pi)
+    s = _sig(GELU_A * z)
+    return s * (1.0 + GELU_A * z * (1.0 - s))
 
 
 # ------------------------------------------------------------------ network
@@ -118,12 +127,12 @@ def scale(X: np.ndarray):
 
 
 def _branch(P, xh, r, rounding):
-    W1, W2, W3 = (r(P[k]).astype(np.float64) for k in ("W1", "W2", "W3"))
+    W1, W2, W3, Lm = (r(P[k]).astype(np.float64) for k in ("W1", "W2", "W3", "L"))
     z1 = xh @ W1.T + P["b1"].astype(np.float64)
     a1 = r(gelu(z1).astype(np.float32) if rounding else gelu(z1)).astype(np.float64)
     z2 = a1 @ W2.T + P["b2"].astype(np.float64)
     a2 = r(gelu(z2).astype(np.float32) if rounding else gelu(z2)).astype(np.float64)
-    y = a2 @ W3.T + P["b3"].astype(np.float64)
+    y = a2 @ W3.T + P["b3"].astype(np.float64) + xh @ Lm.T
     return y, dict(xh=xh, z1=z1, a1=a1, z2=z2, a2=a2, W1=W1, W2=W2, W3=W3)
 
 
@@ -200,7 +209,9 @@ def loss_and_grad(params, X, T, S1=None, nrm: int | None = None, rounding: bool 
         dz1r = r(dz1)
         gW1 = dz1r.T @ cc["xh"]
         gb1 = dz1.sum(0)
-        g += np.concatenate([gW1.reshape(-1), gb1, gW2.reshape(-1), gb2, gW3.reshape(-1), gb3])
+        gL = dyr.T @ cc["xh"]
+        g += np.concatenate([gW1.reshape(-1), gb1, gW2.reshape(-1), gb2, gW3.reshape(-1), gb3,
+                             gL.reshape(-1)])
     return loss, g
 
 

@@ -147,7 +147,7 @@ SIGNATURES = {
     "lbc_forward": (_I, [_I, _P, _P, C.c_longlong, _P, _P, _P, _P, _P]),
     "lbc_train_grad": (_I, [_I, _P, _P, C.c_longlong, _P, _F, _P, _P, _I, _P, _P, _F, C.c_uint,
                             _P]),
-    "lbc_adam": (_I, [_I, _P, _P, _P, _P, _I, _F, _F, _F, _F, _P, _P, _P]),
+    "lbc_adam": (_I, [_I, _P, _P, _P, _P, _I, _F, _F, _F, _F, _P, _P, _I, _I, _P]),
 }
 
 _lib = None

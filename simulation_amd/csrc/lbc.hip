@@ -55,9 +55,12 @@ __global__ void __launch_bounds__(256) pack_kernel(const float *p, uint16_t *w)
     } else if (q < W_W3T) {
         int k = (q - W_W2T) / HID, j = (q - W_W2T) % HID;      // W2T[k][j] = W2[j][k]
         v = p[P_W2 + j * HID + k];
-    } else {
+    } else if (q < W_L) {
         int k = (q - W_W3T) / 32, j = (q - W_W3T) % 32;        // W3T[k][j] = W3[j][k]
         if (j < OUT) v = p[P_W3 + j * HID + k];
+    } else {
+        int j = (q - W_L) / K1, k = (q - W_L) % K1;
+        if (j < OUT && k < IN) v = p[P_L + j * IN + k];
     }
     w[q] = f2bf(v);
 }
@@ -145,10 +148,11 @@ __global__ void __launch_bounds__(256) train_kernel(int n, const float *src,
     for (int q = tid; q < 64 * DS; q += 256) S.DY[q] = 0;
     __syncthreads();
 
-    f32x4 gw1[8], gw2[4], gw3;
+    f32x4 gw1[8], gw2[4], gw3, gl[2];
     for (int k = 0; k < 8; ++k) gw1[k] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
     for (int k = 0; k < 4; ++k) gw2[k] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
     gw3 = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    gl[0] = gl[1] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
     float gb1[4] = {0, 0, 0, 0}, gb2[4] = {0, 0, 0, 0}, gb3 = 0.0f, lsum = 0.0f;
     const float sc = 2.0f / norm;
 
@@ -248,8 +252,12 @@ __global__ void __launch_bounds__(256) train_kernel(int n, const float *src,
 #pragma unroll
             for (int kt = 0; kt < 4; ++kt)
                 gw2[kt] = mfma(a2, ld8(S.A1T + (kt * 16 + c) * MS + ks * 32 + g * 8), gw2[kt]);
-            gw3 = mfma(ld8(S.DYT + c * MS + ks * 32 + g * 8),
-                       ld8(S.A2T + (wave * 16 + c) * MS + ks * 32 + g * 8), gw3);
+            const bf16x8 ay = ld8(S.DYT + c * MS + ks * 32 + g * 8);
+            gw3 = mfma(ay, ld8(S.A2T + (wave * 16 + c) * MS + ks * 32 + g * 8), gw3);
+            // linear path: input tiles 2 wave and 2 wave + 1 of L
+#pragma unroll
+            for (int u = 0; u < 2; ++u)
+                gl[u] = mfma(ay, ld8(S.XhT + ((2 * wave + u) * 16 + c) * MS + ks * 32 + g * 8), gl[u]);
         }
         __syncthreads();
     }
@@ -270,6 +278,11 @@ __global__ void __launch_bounds__(256) train_kernel(int n, const float *src,
     for (int k = 0; k < 4; ++k) {
         const int j = g * 4 + k;
         if (j < OUT) P[P_W3 + j * HID + wave * 16 + c] = gw3[k];
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            const int kk = (2 * wave + u) * 16 + c;
+            if (j < OUT && kk < IN) P[P_L + j * IN + kk] = gl[u][k];
+        }
     }
     // bias gradients and loss: lanes -> waves -> block, fixed order
     auto lane_sum = [](float x) {
@@ -309,14 +322,14 @@ __global__ void __launch_bounds__(256) train_kernel(int n, const float *src,
 __global__ void __launch_bounds__(256) adam_kernel(int nblocks, const float *part, float *p,
                                                    float *m, float *v, float lr, float b1,
                                                    float b2, float eps, float c1, float c2,
-                                                   uint16_t *w, float *grad_out)
+                                                   uint16_t *w, float *grad_out, int q0, int q1)
 {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= NPARAM) return;
     float gsum = 0.0f;
     for (int b = 0; b < nblocks; ++b) gsum += part[(long)b * NPARAM + q];
     if (grad_out) grad_out[q] = gsum;
-    if (lr == 0.0f) return;
+    if (lr == 0.0f || q < q0 || q >= q1) return;
     const float mm = b1 * m[q] + (1.0f - b1) * gsum;
     const float vv = b2 * v[q] + (1.0f - b2) * gsum * gsum;
     m[q] = mm;
@@ -335,6 +348,9 @@ __global__ void __launch_bounds__(256) adam_kernel(int nblocks, const float *par
         const int j = (q - P_W3) / HID, k = (q - P_W3) % HID;
         w[W_W3 + j * HID + k] = h;
         w[W_W3T + k * 32 + j] = h;
+    } else if (q >= P_L) {
+        const int j = (q - P_L) / IN, k = (q - P_L) % IN;
+        w[W_L + j * K1 + k] = h;
     }
 }
 
@@ -411,8 +427,10 @@ int lbc_train_grad(int n, const float *src, const long long *offs, long long tst
 
 int lbc_adam(int nblocks, const float *part, float *params, float *m, float *v, int step,
              float lr, float beta1, float beta2, float eps, uint16_t *wbuf, float *grad_out,
-             void *stream)
+             int first, int count, void *stream)
 {
+    if (first < 0 || count < 0) return fail("lbc_adam: first=%d count=%d", first, count);
+    const int q0 = first, q1 = count ? first + count : NPARAM;
     if (nblocks <= 0 || !part) return fail("lbc_adam: nblocks=%d", nblocks);
     if (lr != 0.0f && (!params || !m || !v || !wbuf || step < 1))
         return fail("lbc_adam: params, m, v, wbuf and step >= 1 are required for an update");
@@ -420,7 +438,7 @@ int lbc_adam(int nblocks, const float *part, float *params, float *m, float *v, 
     double c2 = 1.0 - __builtin_pow((double)beta2, step > 0 ? step : 1);
     hipLaunchKernelGGL(adam_kernel, dim3((NPARAM + 255) / 256), dim3(256), 0, S(stream), nblocks,
                        part, params, m, v, lr, beta1, beta2, eps, (float)c1, (float)c2, wbuf,
-                       grad_out);
+                       grad_out, q0, q1);
     return hip_check(hipGetLastError(), "lbc_adam");
 }
 

@@ -54,6 +54,9 @@ struct FrameSmem {
     float srow[32];
     int pos[PMAX];          // edge << 24 | tau
     int own[PMAX];          // global history index, or -1 when not owned
+    // where each sample value comes from: bits 0-11 region cell, 12-13 field
+    // (0 ez, 1 hx, 2 hy, 3 constant 0), bit 15 negate
+    uint16_t src[PMAX][NS];
 };
 
 static_assert(sizeof(FrameSmem) + 64 <= 160 * 1024, "frame kernel LDS above 160 KiB");
@@ -147,7 +150,7 @@ __global__ void __launch_bounds__(256) frame_kernel(LbcFrameParams P)
     }
 
     // ---------------------------------------------------------- load
-    for (int k = tid; k < NC; k += 256) {
+    for (int k = tid; k < RH * RW; k += 256) {
         const int r = k / RW, cc = k % RW;
         const long o = (long)(G.ri0 + r) * ny + (G.rj0 + cc);
         S.dz[k] = P.dz_i[o];
@@ -169,14 +172,38 @@ __global__ void __launch_bounds__(256) frame_kernel(LbcFrameParams P)
     __syncthreads();
 
     const Net N{P.w, P.params};
-    // region accessor for the canonical samples (clamped into the region:
-    // positions near a cut side read stale neighbours, within the part of
-    // the region that is no longer valid anyway)
-    auto acc = [&](int f, int i, int j) {
-        int r = min(max(i - G.ri0, 0), RH - 1), cc = min(max(j - G.rj0, 0), RW - 1);
-        const float *F = f == 0 ? S.ez : (f == 1 ? S.hx : S.hy);
-        return F[r * RW + cc];
-    };
+    // sample sources, resolved once: canon() run on an accessor that returns
+    // the address instead of the value (clamped into the region: positions
+    // near a cut side read stale neighbours, within the part of the region
+    // that is no longer valid anyway)
+    for (int k = tid; k < np * NS; k += 256) {
+        const int p = k / NS, i = k % NS;
+        const int e = S.pos[p] >> 24, t = S.pos[p] & 0xFFFFFF;
+        int code = -1;                       // set by the accessor
+        auto where = [&](int f, int ii, int jj) {
+            const int r = min(max(ii - G.ri0, 0), RH - 1), cc = min(max(jj - G.rj0, 0), RW - 1);
+            code = (f << 12) | (r * RW + cc);
+            return 1.0f;
+        };
+        float sg;
+        if (i < NF) {
+            const int f = i / 9, d = (i / 3) % 3, o = i % 3 - 1;
+            const int L = e < 2 ? ny : nx;
+            const int tt = min(max(t + o, 0), L - 1);
+            sg = canon(where, e, f, d, o, tt, nx, ny);
+        } else {
+            sg = canon(where, e, 0, NF - 1 - i, 0, t, nx, ny);   // S_EZ2, S_EZ1, S_EZ0
+        }
+        uint16_t v;
+        if (code < 0 || sg == 0.0f) v = (uint16_t)(3 << 12);
+        else v = (uint16_t)(code | (sg < 0.0f ? 0x8000 : 0));
+        S.src[p][i] = v;
+    }
+    // cell loops without divisions: thread tid starts at cell tid and steps
+    // by 256 = DR rows + DC columns
+    const int DR = 256 / RW, DC = 256 % RW;
+    const int r_start = tid / RW, c_start = tid % RW;
+    __syncthreads();
 
     for (int lev = 1; lev <= K; ++lev) {
         // ------------------------------------------------------ network
@@ -214,8 +241,7 @@ __global__ void __launch_bounds__(256) frame_kernel(LbcFrameParams P)
             __syncthreads();
         }
         // ------------------------------------------------------ D, E update
-        for (int k = tid; k < NC; k += 256) {
-            const int r = k / RW, cc = k % RW;
+        for (int k = tid, r = r_start, cc = c_start; k < NC; k += 256) {
             const int i = G.ri0 + r, j = G.rj0 + cc;
             float dz = S.dz[k];
             if (i >= 1 && j >= 1 && r >= 1 && cc >= 1) {
@@ -231,6 +257,8 @@ __global__ void __launch_bounds__(256) frame_kernel(LbcFrameParams P)
             } else {
                 S.ez[k] = S.na[k] * dz;
             }
+            r += DR; cc += DC;
+            if (cc >= RW) { cc -= RW; ++r; }
         }
         __syncthreads();
         // ------------------------------------------------------ learned cells
@@ -244,29 +272,24 @@ __global__ void __launch_bounds__(256) frame_kernel(LbcFrameParams P)
         }
         __syncthreads();
         // ------------------------------------------------------ H update
-        for (int k = tid; k < NC; k += 256) {
-            const int r = k / RW, cc = k % RW;
+        for (int k = tid, r = r_start, cc = c_start; k < NC; k += 256) {
             const int i = G.ri0 + r, j = G.rj0 + cc;
             if (i < nx - 1 && j < ny - 1 && r + 1 < RH && cc + 1 < RW) {
                 const float ez = S.ez[k];
                 S.hx[k] = S.hx[k] + 0.5f * (ez - S.ez[k + 1]);
                 S.hy[k] = S.hy[k] - 0.5f * (ez - S.ez[k + RW]);
             }
+            r += DR; cc += DC;
+            if (cc >= RW) { cc -= RW; ++r; }
         }
         __syncthreads();
         // ------------------------------------------------------ new samples
         for (int k = tid; k < np * NS; k += 256) {
             const int p = k / NS, i = k % NS;
-            const int e = S.pos[p] >> 24, t = S.pos[p] & 0xFFFFFF;
-            float v;
-            if (i < NF) {
-                const int f = i / 9, d = (i / 3) % 3, o = i % 3 - 1;
-                const int L = e < 2 ? ny : nx;
-                const int tt = min(max(t + o, 0), L - 1);
-                v = canon(acc, e, f, d, o, tt, nx, ny);
-            } else {
-                v = canon(acc, e, 0, NF - 1 - i, 0, t, nx, ny);   // S_EZ2, S_EZ1, S_EZ0
-            }
+            const unsigned sv = S.src[p][i];
+            const unsigned f = (sv >> 12) & 3u, cell = sv & 0xFFFu;
+            float v = f == 0 ? S.ez[cell] : (f == 1 ? S.hx[cell] : (f == 2 ? S.hy[cell] : 0.0f));
+            if (sv & 0x8000u) v = -v;
             S.ring[p][lev & 3][i] = v;
         }
         __syncthreads();

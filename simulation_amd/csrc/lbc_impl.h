@@ -7,7 +7,8 @@
 // depths 2..4 x 3 tangential offsets) to the next-step ez of the outer 3
 // cells:
 //   s = max|x|, xh = bf16(x * (1/s))  (rows with s <= 1e-30 predict base),
-//   f(xh) = W3 bf16(gelu(W2 bf16(gelu(W1 xh + b1)) + b2)) + b3,
+//   f(xh) = W3 bf16(gelu(W2 bf16(gelu(W1 xh + b1)) + b2)) + b3 + L xh,
+//   gelu(z) = z sigma(1.702 z),
 //   y = (f(xh) - f(-xh)) / 2          (odd: the exact boundary map is linear),
 //   ez_next[d] = base[d] + s * y[d],
 // base = the two-step absorbing boundary of fd1d/pycuda/fd1d_1_2.py:53-61
@@ -33,16 +34,18 @@ constexpr int NS = NF + 3, S_EZ2 = NF, S_EZ1 = NF + 1, S_EZ0 = NF + 2;
 // rows whose scale s = max|x| is at most S_MIN count as all-zero (no
 // prediction, no training weight): 1/s must stay finite
 constexpr float S_MIN = 1e-30f;
-constexpr int NPARAM = HID * IN + HID + HID * HID + HID + OUT * HID + OUT;  // 11331
-// flat FP32 parameter vector: W1 (64x108) b1 W2 (64x64) b2 W3 (3x64) b3
+constexpr int NPARAM = HID * IN + HID + HID * HID + HID + OUT * HID + OUT + OUT * IN;  // 11655
+// flat FP32 parameter vector: W1 (64x108) b1 W2 (64x64) b2 W3 (3x64) b3, then
+// the linear path L (3x108) from the inputs straight to the outputs
 constexpr int P_W1 = 0, P_B1 = P_W1 + HID * IN, P_W2 = P_B1 + HID, P_B2 = P_W2 + HID * HID,
-              P_W3 = P_B2 + HID, P_B3 = P_W3 + OUT * HID;
+              P_W3 = P_B2 + HID, P_B3 = P_W3 + OUT * HID, P_L = P_B3 + OUT;
 // bf16 operand buffer (uint16 elements): W1 [64][128] (k >= 108 zero),
 // W2 [64][64], W3 [16][64] (rows >= 3 zero), W2^T [64][64], W3^T [64][32]
-// (k >= 3 zero)
+// (k >= 3 zero), L [16][128] (rows >= 3, k >= 108 zero)
 constexpr int K1 = 128;
 constexpr int W_W1 = 0, W_W2 = W_W1 + HID * K1, W_W3 = W_W2 + HID * HID,
-              W_W2T = W_W3 + 16 * HID, W_W3T = W_W2T + HID * HID, WBUF = W_W3T + HID * 32;
+              W_W2T = W_W3 + 16 * HID, W_W3T = W_W2T + HID * HID, W_L = W_W3T + HID * 32,
+              WBUF = W_L + 16 * K1;
 
 // LDS row strides (bf16 elements; multiples of 8 keep 16-byte alignment and
 // spread 16 rows over distinct bank groups)
@@ -71,14 +74,19 @@ __device__ __forceinline__ f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c)
     return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
 
-__device__ __forceinline__ float gelu(float z)
+// GELU in the sigmoid form of Hendrycks & Gimpel (2016), z * sigma(1.702 z),
+// with the hardware exp2 and reciprocal: a handful of instructions, where
+// the erf form costs tens, and the result is rounded to bf16 anyway.
+constexpr float GELU_A = 1.702f;
+__device__ __forceinline__ float sigm(float u)
 {
-    return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f));
+    return __builtin_amdgcn_rcpf(1.0f + __expf(-u));
 }
+__device__ __forceinline__ float gelu(float z) { return z * sigm(GELU_A * z); }
 __device__ __forceinline__ float gelu_d(float z)
 {
-    return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) +
-           z * expf(-0.5f * z * z) * 0.39894228040143268f;
+    const float sg = sigm(GELU_A * z);
+    return sg * (1.0f + GELU_A * z * (1.0f - sg));
 }
 
 // Sample index of field f (0 ez, 1 normal-coupled H, 2 tangential-coupled
@@ -278,12 +286,15 @@ __device__ __forceinline__ void forward_tile(const Net &N, const uint16_t *Xh, u
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    // layer 3: [16 x 64] x [64 x 16] (outputs 3..15 are zero rows of W3)
+    // layer 3: [16 x 64] x [64 x 16] (outputs 3..15 are zero rows of W3),
+    // plus the linear path [16 x 128] x [128 x 16]
     f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
     for (int ks = 0; ks < 2; ++ks)
         acc = mfma(ld8(A2 + (r0 + c) * AS + ks * 32 + g * 8), ld8(W3 + c * HID + ks * 32 + g * 8),
                    acc);
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks) acc = mfma(xa[ks], ld8(N.w + W_L + c * K1 + ks * 32 + g * 8), acc);
     const float b3 = c < OUT ? N.p[P_B3 + c] : 0.0f;
 #pragma unroll
     for (int v = 0; v < 4; ++v) y[v] = acc[v] + b3;

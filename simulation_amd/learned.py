@@ -46,6 +46,12 @@ def npos(nx: int, ny: int) -> int:
     return 2 * ny + 2 * (nx - 6)
 
 
+# flat parameter layout: W1 (64x108) b1 W2 (64x64) b2 W3 (3x64) b3 L (3x108)
+P_W3 = HID * IN + HID + HID * HID + HID
+P_L = P_W3 + OUT * HID + OUT
+NPARAM = P_L + OUT * IN
+
+
 def init_params(seed: int = 0) -> np.ndarray:
     """Fan-in scaled uniform weights, zero biases, a small last layer."""
     rng = np.random.default_rng(seed)
@@ -53,7 +59,7 @@ def init_params(seed: int = 0) -> np.ndarray:
     w2 = rng.uniform(-1, 1, (HID, HID)) * math.sqrt(3.0 / HID)
     w3 = rng.uniform(-1, 1, (OUT, HID)) * math.sqrt(3.0 / HID) * 0.1
     return np.concatenate([w1.ravel(), np.zeros(HID), w2.ravel(), np.zeros(HID), w3.ravel(),
-                           np.zeros(OUT)]).astype(np.float32)
+                           np.zeros(OUT), np.zeros(OUT * IN)]).astype(np.float32)
 
 
 class Net:
@@ -103,13 +109,14 @@ class Net:
              None if tgt is None else ptr(tgt), nrm, ptr(self.params), self.wbuf.data_ptr(), nb,
              ptr(part), ptr(ploss), 0.0, 0, stream())
         call("lbc_adam", nb, ptr(part), None, None, None, 0, 0.0, 0.9, 0.999, 1e-8, None, ptr(g),
-             stream())
+             0, 0, stream())
         return float(ploss.double().sum()) / nrm, g
 
     def train_step(self, src, offs, tstride, lr: float, tgt=None, nblocks: int = 0,
-                   part=None, ploss=None, noise: float = 0.0) -> torch.Tensor:
+                   part=None, ploss=None, noise: float = 0.0, only=None) -> torch.Tensor:
         """One Adam step on the rows (inputs perturbed by `noise`, a fresh
-        pattern each step); returns the device loss partials."""
+        pattern each step; only = (first, count) restricts the update to those
+        parameters); returns the device loss partials."""
         n = offs.numel()
         nb = nblocks or max(1, min(512, (n + 255) // 256))
         if part is None or part.shape[0] < nb:
@@ -120,8 +127,9 @@ class Net:
         call("lbc_train_grad", n, ptr(src), offs.data_ptr(), int(tstride),
              None if tgt is None else ptr(tgt), float(n), ptr(self.params), self.wbuf.data_ptr(),
              nb, ptr(part), ptr(ploss), float(noise), self.step, stream())
+        q0, qn = only or (0, 0)
         call("lbc_adam", nb, ptr(part), ptr(self.params), ptr(self.m), ptr(self.v), self.step,
-             float(lr), 0.9, 0.999, 1e-8, self.wbuf.data_ptr(), None, stream())
+             float(lr), 0.9, 0.999, 1e-8, self.wbuf.data_ptr(), None, int(q0), int(qn), stream())
         return ploss[:nb]
 
 
@@ -213,7 +221,8 @@ class TrainConfig:
     steps: int = 520
     margin: int = 32
     seed: int = 0
-    train_steps: int = 8000
+    linear_steps: int = 4000      # phase 1: the linear path alone (network output held at 0)
+    train_steps: int = 8000       # phase 2: the hidden layers, linear path frozen
     batch: int = 8192
     lr: float = 2e-3
     lr_min: float = 2e-5
@@ -288,10 +297,15 @@ def training_rows(G: torch.Tensor, net: Net, s_floor: float) -> torch.Tensor:
 
 
 def train(cfg: TrainConfig, G: torch.Tensor, net: Net | None = None, log=None) -> tuple:
-    """Adam on random batches of the recorded rows; cosine learning rate.
-    Returns (net, history [(step, train loss, validation loss)])."""
+    """Adam on random batches of the recorded rows, cosine learning rate, in
+    two phases: the linear path alone (the hidden layers' output held at 0
+    by W3 = 0), then the hidden layers with the linear path frozen.
+    Returns (net, history [(phase, step, train loss, validation loss)])."""
     dev = G.device
-    net = net or Net(None, dev, seed=cfg.seed)
+    if net is None:
+        p = init_params(cfg.seed)
+        p[P_W3:P_L] = 0.0
+        net = Net(p, dev)
     ts = G.shape[2] * NSAMP
     rows = training_rows(G, net, cfg.s_floor)
     gen = torch.Generator(device=dev)
@@ -305,22 +319,30 @@ def train(cfg: TrainConfig, G: torch.Tensor, net: Net | None = None, log=None) -
     hist = []
     if log:
         v0, _ = net.grad(G, val, ts)
-        z = Net(np.zeros(net.nparam, np.float32), dev)
-        vb, _ = z.grad(G, val, ts)
         log(f"{rows.numel()} rows ({trn.numel()} train, {nval} validation); "
-            f"validation loss: initial {v0:.3e}, baseline (zero network) {vb:.3e}")
-    for k in range(cfg.train_steps):
-        lr = cfg.lr_min + 0.5 * (cfg.lr - cfg.lr_min) * (1 + math.cos(math.pi * k / cfg.train_steps))
-        idx = torch.randint(0, trn.numel(), (cfg.batch,), device=dev, generator=gen)
-        pl = net.train_step(G, trn[idx], ts, lr, nblocks=nb, part=part, ploss=ploss,
-                            noise=cfg.noise)
-        if (k + 1) % 500 == 0 or k == cfg.train_steps - 1:
-            tl = float(pl.double().sum()) / cfg.batch
-            vl, _ = net.grad(G, val, ts)
-            hist.append((k + 1, tl, vl))
-            if log:
-                log(f"step {k + 1}: train {tl:.3e} validation {vl:.3e} lr {lr:.2e}")
+            f"validation loss at start (the two-step boundary): {v0:.3e}")
+    phases = (("linear", cfg.linear_steps, (P_L, OUT * IN)), ("hidden", cfg.train_steps, (0, P_L)))
+    for name, nsteps, only in phases:
+        for k in range(nsteps):
+            lr = cfg.lr_min + 0.5 * (cfg.lr - cfg.lr_min) * (1 + math.cos(math.pi * k / nsteps))
+            idx = torch.randint(0, trn.numel(), (cfg.batch,), device=dev, generator=gen)
+            pl = net.train_step(G, trn[idx], ts, lr, nblocks=nb, part=part, ploss=ploss,
+                                noise=cfg.noise, only=only)
+            if (k + 1) % 1000 == 0 or k == nsteps - 1:
+                tl = float(pl.double().sum()) / cfg.batch
+                vl, _ = net.grad(G, val, ts)
+                hist.append((name, k + 1, tl, vl))
+                if log:
+                    log(f"{name} step {k + 1}: train {tl:.3e} validation {vl:.3e} lr {lr:.2e}")
     return net, hist
+
+
+def scaled(net: Net, lam: float) -> Net:
+    """A copy whose hidden-layer output (W3, b3) is scaled by lam; lam = 0
+    leaves the linear path alone."""
+    p = net.numpy().copy()
+    p[P_W3:P_L] *= lam
+    return Net(p, net.device)
 
 
 # --------------------------------------------------------------- evaluation
@@ -466,6 +488,7 @@ def main(argv=None) -> int:
     ap.add_argument("--runs", type=int, default=512)
     ap.add_argument("--window", type=int, default=160)
     ap.add_argument("--steps", type=int, default=520)
+    ap.add_argument("--linear-steps", type=int, default=4000)
     ap.add_argument("--train-steps", type=int, default=8000)
     ap.add_argument("--batch", type=int, default=8192)
     ap.add_argument("--lr", type=float, default=2e-3)
@@ -480,7 +503,8 @@ def main(argv=None) -> int:
     def log(m):
         print(f"[{time.perf_counter() - t0:7.1f}s] {m}", flush=True)
     cfg = TrainConfig(runs=a.runs, window=a.window, steps=a.steps, train_steps=a.train_steps,
-                      batch=a.batch, lr=a.lr, seed=a.seed, noise=a.noise)
+                      linear_steps=a.linear_steps, batch=a.batch, lr=a.lr, seed=a.seed,
+                      noise=a.noise)
     dev = require_gpu()
     G = generate(cfg, dev, log)
     log(f"samples {tuple(G.shape)} ({G.numel() * 4 / 2**30:.1f} GiB)")
