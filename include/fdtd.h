@@ -313,23 +313,26 @@ int fdtd_graph_destroy(void *graph_exec);
  * network applied at the outer three cells of every edge.
  *
  * Network: 108 features (the last four canonical boundary samples of a
- * position, newest first, 27 features each) -> 64 -> 64 -> 3, GELU, bf16
- * weights and activations, FP32 accumulation (MFMA).  With s = max|x|:
- *   ez_next[d] = base[d] + s * y[d],  y = net(bf16(x / s)),  d = 0..2,
+ * position, newest first, 27 features each) -> 3 outputs.  With s = max|x|,
+ * xf = x * (1/s) in FP32 and xh = bf16(xf):
+ *   ez_next[d] = base[d] + s * y[d],  d = 0..2,
+ *   y = L xf + (f(xh) - f(-xh)) / 2,
+ *   f(x) = W3 bf16(gelu(W2 bf16(gelu(W1 x + b1)) + b2)) + b3,
+ *   gelu(z) = z sigma(1.702 z),
  * base = the two-step absorbing boundary of fd1d/pycuda/fd1d_1_2.py:53-61 on
- * the three outer layers (ez_d(t+1) = ez_{d+1}(t-1)): a network with zero
- * output is that classical boundary.
- * f(x) = W3 gelu(W2 gelu(W1 x + b1) + b2) + b3 + L x (a linear path beside
- * the hidden layers), gelu(z) = z sigma(1.702 z), y = (f(x) - f(-x)) / 2.
+ * the three outer layers (ez_d(t+1) = ez_{d+1}(t-1)): a zero network is that
+ * classical boundary.  The linear path L (FP32, fitted by ridge least
+ * squares: lbc_gram, lbc_fit_linear) carries most of the correction; the
+ * hidden layers (bf16 weights and activations, FP32 accumulation on MFMA,
+ * trained by lbc_train_grad / lbc_adam) fit what it leaves.
  * Parameters: one FP32 vector of lbc_param_count() values, in the order
- * W1 (64x108, row-major), b1 (64), W2 (64x64), b2, W3 (3x64), b3, L (3x108); the
- * kernels read bf16 copies from a buffer of lbc_wbuf_elems() uint16 values
- * written by lbc_pack (and kept current by lbc_adam).
+ * W1 (64x108, row-major), b1 (64), W2 (64x64), b2, W3 (3x64), b3, L (3x108);
+ * the hidden layers read bf16 copies from a buffer of lbc_wbuf_elems()
+ * uint16 values written by lbc_pack (and kept current by lbc_adam).
  *
- * Canonical sample (29 floats): 27 features at index (f*3 + d)*3 + o + 1 for
- * field f, feature depth d = 0..2 (grid depth D = d + 2: the innermost
- * predicted cell and two cells inside it -- only ez at depth 2 reaches the
- * interior, so the network never reads cells that only it writes),
+ * Canonical sample (30 floats): 27 features at index (f*3 + d)*3 + o + 1 for
+ * field f, feature depth d = 0..2 (grid depth D = d + 3: the three cells just
+ * inside the predicted ones, so the network never reads its own output),
  * tangential offset o = -1..1 (tangential index clamped to the edge); f = 0
  * ez, 1 normal-coupled H, 2 tangential-coupled H:
  *   top    (i = D):      ez[D,t],      hy[D,t],       hx[D,t]
@@ -339,8 +342,8 @@ int fdtd_graph_destroy(void *graph_exec);
  * with the tangential-coupled value 0 at offset +1 (it is staggered half a
  * cell along the edge; offsets -1 and 0 are the two points beside the cell)
  * and H in the last row or column of the window read as 0 (the H update
- * never runs there); then ez of grid depth 1 and 0 at offset 0 (targets and
- * baseline only).  Positions: top then bottom (t in [0, ny), owning the
+ * never runs there); then ez of grid depths 2, 1 and 0 at offset 0 (targets
+ * and baseline only).  Positions: top then bottom (t in [0, ny), owning the
  * corners), then left and right (t in [3, nx-3)):
  * lbc_npos = 2 ny + 2 (nx - 6).                                             */
 int lbc_param_count(void);
@@ -348,7 +351,7 @@ int lbc_wbuf_elems(void);
 size_t lbc_train_smem(void);
 
 /* Canonical samples of an nx x ny window (row stride ld) of ez/hx/hy:
- * out[lbc_npos][29].                                                        */
+ * out[lbc_npos][30].                                                        */
 int lbc_sample(int nx, int ny, int ld, const float *ez, const float *hx, const float *hy,
                float *out, void *stream);
 
@@ -365,7 +368,9 @@ int lbc_forward(int n, const float *src, const long long *offs, long long tstrid
 /* Training gradient of  sum_r w_r |y_r - (T_r - base_r)/s_r|^2 / norm
  * (w_r = 0 for all-zero rows), targets T_r = tgt[r][3] or, with tgt NULL,
  * the next sample's ez of grid depths 0..2 (src + offs[r] + tstride); the
- * baseline reads the sample of age 1, which must be a full 29-float sample.
+ * baseline reads the sample of age 1, which must be a full 30-float sample.
+ * The linear path is fixed here: the hidden layers fit what it leaves, and
+ * the gradient entries of L are zero.
  * nblocks workgroups each write one partial gradient (lbc_param_count
  * floats) to part[block] and its loss sum to part_loss[block]; lbc_adam
  * reduces them in a fixed order (deterministic).  noise > 0 adds uniform
@@ -384,7 +389,7 @@ int lbc_train_grad(int n, const float *src, const long long *offs, long long tst
  * at most 8).  Each launch of k <= depth steps is the blocked launch with PEC
  * edges followed by a frame kernel that recomputes the outer k+3 cells with
  * the network applied every step (the blocked launch is exact deeper in).
- * hist, hist2: [lbc_hist_elems(nx, ny)] floats each (npos x 4 x 29), the last four boundary
+ * hist, hist2: [lbc_hist_elems(nx, ny)] floats each (npos x 4 x 30), the last four boundary
  * samples of every position (zero for fields at rest); the run leaves them
  * current in hist.  nsteps >= 2 (an even number of launches).               */
 #define FDTD_LBC_MARGIN 32
@@ -405,6 +410,23 @@ int fd2d_lbc_run(const fd2d_tm_desc *d, const fd2d_lbc_desc *b, int t0, int nste
 int lbc_adam(int nblocks, const float *part, float *params, float *m, float *v, int step,
              float lr, float beta1, float beta2, float eps, uint16_t *wbuf, float *grad_out,
              int first, int count, void *stream);
+
+/* Ridge least squares for the linear path.  lbc_gram accumulates, over the
+ * rows (as lbc_train_grad, targets from the next sample) with s > 0, the FP64
+ * sums G = sum xn xn^T (108 x 108) and R = sum xn t^T (108 x 3) of the
+ * normalised inputs xn (bf16-rounded when round_bf16) and residual targets
+ * t = (T - base) / s, plus the row count: nblocks workgroups write
+ * part[block][lbc_gram_elems()] (row-major [G | R], then the count).
+ * lbc_fit_linear reduces the partials in a fixed order into gram, then (if
+ * params is not NULL) solves (G + rho * count * I) L^T = R by Cholesky in
+ * FP64 and writes L into params; status[0] = 0, or 1 + k if pivot k is not
+ * positive (L then unchanged).  Call lbc_pack afterwards only if the bf16
+ * buffer must follow (L is read in FP32).                                   */
+int lbc_gram_elems(void);
+int lbc_gram(int n, const float *src, const long long *offs, long long tstride, int round_bf16,
+             int nblocks, double *part, void *stream);
+int lbc_fit_linear(int nblocks, const double *part, double *gram, double rho, float *params,
+                   int *status, void *stream);
 
 #ifdef __cplusplus
 }

@@ -12,11 +12,14 @@ kernels have a checker:
 * the baseline (the reference's two-step absorbing boundary,
   fd1d/pycuda/fd1d_1_2.py:53-61, on the three outer layers) that the network
   corrects;
-* the odd network y = (f(x) - f(-x))/2, f: 108 -> 64 -> 64 -> 3 (GELU in
-  the sigmoid form z sigma(1.702 z)) plus a linear path L x from the inputs,
-  with the GPU's rounding points
-  (inputs and hidden activations rounded to bf16, weights bf16, products
-  accumulated in FP32 on the GPU and FP64 here), and its backward pass;
+* the network y = L xf + (f(xh) - f(-xh))/2: a linear path on the FP32
+  normalised inputs xf plus the odd part of f: 108 -> 64 -> 64 -> 3 (GELU in
+  the sigmoid form z sigma(1.702 z)) on their bf16 rounding xh, with the
+  GPU's rounding points (inputs and hidden activations rounded to bf16,
+  weights bf16, products accumulated in FP32 on the GPU and FP64 here), and
+  its backward pass (hidden layers only);
+* the least-squares fit of the linear path (normal equations with a ridge
+  term, FP64);
 * one learned-boundary time step: the free-space/lossy update of
   fd2d/program/fd2d_3_3.py:68-98 (as oracle/numpy_oracle.py) without a PML,
   with ez of the outer three cells of every edge replaced by the network's
@@ -40,6 +43,8 @@ S_EZ2, S_EZ1, S_EZ0 = 27, 28, 29
 FD0 = 3                        # grid depth of feature depth 0
 HIST = 4
 NPARAM = HID * IN + HID + HID * HID + HID + OUT * HID + OUT + OUT * IN
+P_W3 = HID * IN + HID + HID * HID + HID      # offsets of W3 and of the linear path L
+P_L = P_W3 + OUT * HID + OUT
 
 
 def sidx(f: int, d: int, o: int) -> int:
@@ -126,25 +131,35 @@ def scale(X: np.ndarray):
     return s, inv, xh
 
 
+def normalised(X, rounding: bool = True) -> np.ndarray:
+    """FP32 normalised inputs x * (1/s) (the linear path's operand), FP64
+    products with rounding=False."""
+    s, inv, _ = scale(X)
+    if rounding:
+        return (np.asarray(X, np.float32) * inv[:, None]).astype(np.float32)
+    return np.asarray(X, np.float64) * inv[:, None].astype(np.float64)
+
+
 def _branch(P, xh, r, rounding):
-    W1, W2, W3, Lm = (r(P[k]).astype(np.float64) for k in ("W1", "W2", "W3", "L"))
+    W1, W2, W3 = (r(P[k]).astype(np.float64) for k in ("W1", "W2", "W3"))
     z1 = xh @ W1.T + P["b1"].astype(np.float64)
     a1 = r(gelu(z1).astype(np.float32) if rounding else gelu(z1)).astype(np.float64)
     z2 = a1 @ W2.T + P["b2"].astype(np.float64)
     a2 = r(gelu(z2).astype(np.float32) if rounding else gelu(z2)).astype(np.float64)
-    y = a2 @ W3.T + P["b3"].astype(np.float64) + xh @ Lm.T
+    y = a2 @ W3.T + P["b3"].astype(np.float64)
     return y, dict(xh=xh, z1=z1, a1=a1, z2=z2, a2=a2, W1=W1, W2=W2, W3=W3)
 
 
 def forward(params, X, rounding: bool = True):
     """Network on raw features X (B,108) -> (y (B,3) FP64, cache).
 
-    The network is made odd, y(-x) = -y(x), as the exact boundary map is
-    linear: y = (f(xh) - f(-xh)) / 2 with f the 108-64-64-3 GELU network (a
+    The network is odd, y(-x) = -y(x), as the exact boundary map is linear:
+    y = L xf + (f(xh) - f(-xh)) / 2 with f the 108-64-64-3 GELU network (a
     constant or even part would pump a one-signed field into the grid through
-    the scale s).  With rounding=True the bf16 rounding points of the kernels
-    are applied (FP64 sums in between; bf16(-x) = -bf16(x)); with False
-    everything is FP64 (the exact function of the FP32 master weights)."""
+    the scale s).  L acts on the FP32 normalised inputs with FP32 weights.
+    With rounding=True the bf16 rounding points of the kernels are applied
+    (FP64 sums in between; bf16(-x) = -bf16(x)); with False everything is
+    FP64 (the exact function of the FP32 master weights)."""
     P = unpack(params)
     s, inv, xh = scale(X)
     r = bf16 if rounding else (lambda a: np.asarray(a, np.float64))
@@ -153,7 +168,8 @@ def forward(params, X, rounding: bool = True):
     xh = np.asarray(xh, np.float64)
     yp, cp = _branch(P, xh, r, rounding)
     ym, cm = _branch(P, -xh, r, rounding)
-    return 0.5 * (yp - ym), dict(s=s, inv=inv, plus=cp, minus=cm)
+    lin = normalised(X, rounding).astype(np.float64) @ P["L"].astype(np.float64).T
+    return lin + 0.5 * (yp - ym), dict(s=s, inv=inv, plus=cp, minus=cm)
 
 
 def base(S1) -> np.ndarray:
@@ -182,9 +198,10 @@ def loss_and_grad(params, X, T, S1=None, nrm: int | None = None, rounding: bool 
     respect to the flat FP32 parameters (straight through the bf16 weight
     rounding).  S1: age-1 samples for the baseline (default: the age-1
     features padded with zero outer ez).  The odd network's gradient is the
-    f(xh) branch's with dy/2 plus the f(-xh) branch's with -dy/2.
-    rounding=True rounds the backward MFMA operands (dy, dz2, dz1) to bf16 as
-    the kernels do."""
+    f(xh) branch's with dy/2 plus the f(-xh) branch's with -dy/2.  The linear
+    path is fitted by least squares (fit_linear), not by gradient: its
+    gradient entries are zero.  rounding=True rounds the backward MFMA
+    operands (dy, dz2, dz1) to bf16 as the kernels do."""
     y, c = forward(params, X, rounding)
     if S1 is None:
         S1 = pad_samples(np.asarray(X, np.float32)[:, NFEAT:2 * NFEAT])
@@ -209,10 +226,29 @@ def loss_and_grad(params, X, T, S1=None, nrm: int | None = None, rounding: bool 
         dz1r = r(dz1)
         gW1 = dz1r.T @ cc["xh"]
         gb1 = dz1.sum(0)
-        gL = dyr.T @ cc["xh"]
         g += np.concatenate([gW1.reshape(-1), gb1, gW2.reshape(-1), gb2, gW3.reshape(-1), gb3,
-                             gL.reshape(-1)])
+                             np.zeros(OUT * IN)])
     return loss, g
+
+
+def gram(X, T, S1, round_bf16: bool = False):
+    """Normal-equation sums of the linear path over rows with s > S_MIN:
+    G = sum xn xn^T (108 x 108), R = sum xn t^T (108 x 3), count; xn the
+    FP32 normalised inputs (bf16-rounded with round_bf16), t the normalised
+    residual targets (targets()); FP64 sums."""
+    xn = normalised(X).astype(np.float32)
+    if round_bf16:
+        xn = bf16(xn)
+    th, w = targets(X, T, S1)
+    xn = xn.astype(np.float64) * w[:, None]
+    th = th.astype(np.float64) * w[:, None]
+    return xn.T @ xn, xn.T @ th, float(w.sum())
+
+
+def fit_linear(G, R, count: float, rho: float) -> np.ndarray:
+    """Ridge solution L (3 x 108) of (G + rho * count * I) L^T = R."""
+    A = np.asarray(G, np.float64) + rho * count * np.eye(IN)
+    return np.linalg.solve(A, np.asarray(R, np.float64)).T
 
 
 def adam(params, m, v, g, step: int, lr: float, b1=0.9, b2=0.999, eps=1e-8):

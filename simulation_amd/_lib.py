@@ -148,6 +148,9 @@ SIGNATURES = {
     "lbc_train_grad": (_I, [_I, _P, _P, C.c_longlong, _P, _F, _P, _P, _I, _P, _P, _F, C.c_uint,
                             _P]),
     "lbc_adam": (_I, [_I, _P, _P, _P, _P, _I, _F, _F, _F, _F, _P, _P, _I, _I, _P]),
+    "lbc_gram_elems": (_I, []),
+    "lbc_gram": (_I, [_I, _P, _P, C.c_longlong, _I, _I, _P, _P]),
+    "lbc_fit_linear": (_I, [_I, _P, _P, C.c_double, _P, _P, _P]),
 }
 
 _lib = None

@@ -55,12 +55,9 @@ __global__ void __launch_bounds__(256) pack_kernel(const float *p, uint16_t *w)
     } else if (q < W_W3T) {
         int k = (q - W_W2T) / HID, j = (q - W_W2T) % HID;      // W2T[k][j] = W2[j][k]
         v = p[P_W2 + j * HID + k];
-    } else if (q < W_L) {
+    } else {
         int k = (q - W_W3T) / 32, j = (q - W_W3T) % 32;        // W3T[k][j] = W3[j][k]
         if (j < OUT) v = p[P_W3 + j * HID + k];
-    } else {
-        int j = (q - W_L) / K1, k = (q - W_L) % K1;
-        if (j < OUT && k < IN) v = p[P_L + j * IN + k];
     }
     w[q] = f2bf(v);
 }
@@ -90,6 +87,7 @@ __global__ void __launch_bounds__(256) forward_kernel(int n, const float *src,
     __shared__ __attribute__((aligned(16))) uint16_t Xh[64 * XS];
     __shared__ __attribute__((aligned(16))) uint16_t A1[64 * AS];
     __shared__ __attribute__((aligned(16))) uint16_t A2[64 * AS];
+    __shared__ float lin_s[32][OUT];
     const Net N{w, p};
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, c = lane & 15, g = lane >> 4;
     const int r0 = wave * 16;
@@ -101,7 +99,13 @@ __global__ void __launch_bounds__(256) forward_kernel(int n, const float *src,
         load_sample(src, offs, tstride, n, row, g, v);
         float s, inv;
         store_features(v, Xh, r0, s, inv);
-        if (g == 0 && c < 8 && row < n && s_out) s_out[row] = s;
+        float lin[OUT];
+        linear_out(p + P_L, v, inv, lin);
+        if (g == 0 && c < 8) {
+            if (row < n && s_out) s_out[row] = s;
+#pragma unroll
+            for (int d = 0; d < OUT; ++d) lin_s[wave * 8 + c][d] = lin[d];
+        }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -111,7 +115,7 @@ __global__ void __launch_bounds__(256) forward_kernel(int n, const float *src,
         for (int k = 0; k < 4; ++k) {
             const float yo = odd_out(yy[k]);
             const int rr = base + wave * 8 + g * 4 + k;
-            if (g < 2 && c < OUT && rr < n) y[(long)rr * OUT + c] = yo;
+            if (g < 2 && c < OUT && rr < n) y[(long)rr * OUT + c] = lin_s[wave * 8 + g * 4 + k][c] + yo;
         }
         __syncthreads();
     }
@@ -148,11 +152,10 @@ __global__ void __launch_bounds__(256) train_kernel(int n, const float *src,
     for (int q = tid; q < 64 * DS; q += 256) S.DY[q] = 0;
     __syncthreads();
 
-    f32x4 gw1[8], gw2[4], gw3, gl[2];
+    f32x4 gw1[8], gw2[4], gw3;
     for (int k = 0; k < 8; ++k) gw1[k] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
     for (int k = 0; k < 4; ++k) gw2[k] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
     gw3 = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-    gl[0] = gl[1] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
     float gb1[4] = {0, 0, 0, 0}, gb2[4] = {0, 0, 0, 0}, gb3 = 0.0f, lsum = 0.0f;
     const float sc = 2.0f / norm;
 
@@ -164,6 +167,9 @@ __global__ void __launch_bounds__(256) train_kernel(int n, const float *src,
         load_sample(src, offs, tstride, n, row, g, v);
         float s, inv;
         store_features(v, S.Xh, r0, s, inv, S.XhT, MS, noise, (uint32_t)row, seed);
+        // the hidden layers fit what the (fixed) linear path leaves
+        float lin[OUT];
+        linear_out(p + P_L, v, inv, lin);
         if (g == 0 && c < 8) {
             float wgt = (row < n && s > S_MIN) ? 1.0f : 0.0f;
 #pragma unroll
@@ -175,7 +181,7 @@ __global__ void __launch_bounds__(256) train_kernel(int n, const float *src,
                             : nxt[d == 2 ? S_EZ2 : (d == 1 ? S_EZ1 : S_EZ0)];
                     b = base_of(d, src + offs[row] - tstride);
                 }
-                S.th[si][d] = (T - b) * inv;
+                S.th[si][d] = (T - b) * inv - lin[d];
             }
             S.th[si][3] = wgt;
         }
@@ -254,10 +260,6 @@ __global__ void __launch_bounds__(256) train_kernel(int n, const float *src,
                 gw2[kt] = mfma(a2, ld8(S.A1T + (kt * 16 + c) * MS + ks * 32 + g * 8), gw2[kt]);
             const bf16x8 ay = ld8(S.DYT + c * MS + ks * 32 + g * 8);
             gw3 = mfma(ay, ld8(S.A2T + (wave * 16 + c) * MS + ks * 32 + g * 8), gw3);
-            // linear path: input tiles 2 wave and 2 wave + 1 of L
-#pragma unroll
-            for (int u = 0; u < 2; ++u)
-                gl[u] = mfma(ay, ld8(S.XhT + ((2 * wave + u) * 16 + c) * MS + ks * 32 + g * 8), gl[u]);
         }
         __syncthreads();
     }
@@ -278,12 +280,9 @@ __global__ void __launch_bounds__(256) train_kernel(int n, const float *src,
     for (int k = 0; k < 4; ++k) {
         const int j = g * 4 + k;
         if (j < OUT) P[P_W3 + j * HID + wave * 16 + c] = gw3[k];
-#pragma unroll
-        for (int u = 0; u < 2; ++u) {
-            const int kk = (2 * wave + u) * 16 + c;
-            if (j < OUT && kk < IN) P[P_L + j * IN + kk] = gl[u][k];
-        }
     }
+    // L is fitted by least squares (fit_kernel), not by gradient: zero
+    for (int q = tid; q < OUT * IN; q += 256) P[P_L + q] = 0.0f;
     // bias gradients and loss: lanes -> waves -> block, fixed order
     auto lane_sum = [](float x) {
         x += __shfl_xor(x, 16);
@@ -348,10 +347,182 @@ __global__ void __launch_bounds__(256) adam_kernel(int nblocks, const float *par
         const int j = (q - P_W3) / HID, k = (q - P_W3) % HID;
         w[W_W3 + j * HID + k] = h;
         w[W_W3T + k * 32 + j] = h;
-    } else if (q >= P_L) {
-        const int j = (q - P_L) / IN, k = (q - P_L) % IN;
-        w[W_L + j * K1 + k] = h;
     }
+}
+
+// ------------------------------------------------------------ linear path
+// Ridge least squares for the linear path L: with xh the normalised inputs
+// (bf16-rounded when round_bf16, as the bf16 operands see them) and
+// t = (target - base) / s the normalised residual target of a row,
+//   G = sum_rows w xh xh^T (IN x IN),  R = sum_rows w xh t^T (IN x OUT),
+//   L^T = (G + ridge I)^-1 R,  ridge = rho * sum_rows w,
+// w = 1 for rows with s > S_MIN.  Sums in FP64.  Each block accumulates
+// the IN x (IN + OUT) products plus the row count over its rows; the fit
+// kernel reduces the partials in a fixed order and solves by Cholesky.
+constexpr int GC = IN + OUT;            // columns of [G | R]
+constexpr int GSZ = IN * GC + 1;        // + row count
+constexpr int GPER = (IN * GC + 255) / 256;
+
+__global__ void __launch_bounds__(256) gram_kernel(int n, const float *src, const long long *offs,
+                                                   long long tstride, int round_bf16,
+                                                   double *part)
+{
+    __shared__ float X[32][GC + 1];
+    const int tid = threadIdx.x;
+    double acc[GPER];
+    int ia[GPER], ib[GPER];
+#pragma unroll
+    for (int j = 0; j < GPER; ++j) {
+        acc[j] = 0.0;
+        const int e = tid + 256 * j;
+        ia[j] = e < IN * GC ? e / GC : 0;
+        ib[j] = e < IN * GC ? e % GC : 0;
+    }
+    double cnt = 0.0;
+    // loader: thread t < 128 holds age t & 3 of row t >> 2 of the round
+    const int lr = tid >> 2, la = tid & 3;
+    for (int base = blockIdx.x * 32; base < n; base += gridDim.x * 32) {
+        const int row = base + lr;
+        if (tid < 128) {
+            float v[NF];
+            const bool ok = row < n;
+            if (ok) {
+                const float *q = src + offs[row] - (long long)la * tstride;
+#pragma unroll
+                for (int i = 0; i < NF; ++i) v[i] = q[i];
+            } else {
+#pragma unroll
+                for (int i = 0; i < NF; ++i) v[i] = 0.0f;
+            }
+            float mx = 0.0f;
+#pragma unroll
+            for (int i = 0; i < NF; ++i) mx = fmaxf(mx, fabsf(v[i]));
+            mx = fmaxf(mx, __shfl_xor(mx, 1));
+            mx = fmaxf(mx, __shfl_xor(mx, 2));
+            const bool use = ok && mx > S_MIN;
+            const float inv = use ? 1.0f / mx : 0.0f;
+#pragma unroll
+            for (int i = 0; i < NF; ++i) {
+                float x = v[i] * inv;
+                if (round_bf16) x = bf2f(f2bf(x));
+                X[lr][la * NF + i] = x;
+            }
+            if (la == 0) {
+#pragma unroll
+                for (int d = 0; d < OUT; ++d) {
+                    float t = 0.0f;
+                    if (use) {
+                        const float *nxt = src + offs[row] + tstride;
+                        const float T = nxt[d == 2 ? S_EZ2 : (d == 1 ? S_EZ1 : S_EZ0)];
+                        t = (T - base_of(d, src + offs[row] - tstride)) * inv;
+                    }
+                    X[lr][IN + d] = t;
+                }
+                if (use) cnt += 1.0;
+            }
+        }
+        __syncthreads();
+        for (int r = 0; r < 32; ++r) {
+#pragma unroll
+            for (int j = 0; j < GPER; ++j)
+                acc[j] += (double)X[r][ia[j]] * (double)X[r][ib[j]];
+        }
+        __syncthreads();
+    }
+    double *P = part + (long)blockIdx.x * GSZ;
+#pragma unroll
+    for (int j = 0; j < GPER; ++j) {
+        const int e = tid + 256 * j;
+        if (e < IN * GC) P[e] = acc[j];
+    }
+    // row count: loader threads (la == 0) -> block, fixed order
+    __shared__ double cs[32];
+    if (tid < 128 && la == 0) cs[lr] = cnt;
+    __syncthreads();
+    if (tid == 0) {
+        double c = 0.0;
+        for (int r = 0; r < 32; ++r) c += cs[r];
+        P[IN * GC] = c;
+    }
+}
+
+// Partials -> [G | R | count], fixed order over blocks.
+__global__ void __launch_bounds__(256) gram_reduce_kernel(int nblocks, const double *part,
+                                                          double *out)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= GSZ) return;
+    double s = 0.0;
+    for (int b = 0; b < nblocks; ++b) s += part[(long)b * GSZ + e];
+    out[e] = s;
+}
+
+// One workgroup: Cholesky of G + ridge I in LDS (FP64, packed lower
+// triangle), two triangular solves for the OUT right-hand sides, L written
+// into the parameters.  status[0] = 0 on success, 1 + k when pivot k is not
+// above 1e-13 of the largest diagonal entry (numerically singular: the
+// system is then left unsolved and L is not written).
+__device__ __forceinline__ int tri(int i, int j) { return i * (i + 1) / 2 + j; }
+
+__global__ void __launch_bounds__(256) fit_kernel(const double *gram, double rho, float *params,
+                                                  int *status)
+{
+    __shared__ double A[IN * (IN + 1) / 2];
+    __shared__ double B[IN][OUT];
+    __shared__ int bad;
+    const int tid = threadIdx.x;
+    const double ridge = rho * gram[IN * GC];
+    for (int i = 0; i < IN; ++i)
+        for (int j = tid; j <= i; j += 256) A[tri(i, j)] = gram[i * GC + j] + (i == j ? ridge : 0.0);
+    for (int e = tid; e < IN * OUT; e += 256) B[e / OUT][e % OUT] = gram[(e / OUT) * GC + IN + e % OUT];
+    __shared__ double floor_s;
+    if (tid == 0) {
+        bad = 0;
+        double m = 0.0;
+        for (int i = 0; i < IN; ++i) m = fmax(m, gram[i * GC + i] + ridge);
+        floor_s = 1e-13 * m;
+    }
+    __syncthreads();
+    // right-looking Cholesky: A = C C^T, C lower
+    for (int k = 0; k < IN; ++k) {
+        if (tid == 0) {
+            const double d = A[tri(k, k)];
+            const bool ok = d > floor_s && d > 0.0;
+            if (!ok && !bad) bad = 1 + k;
+            A[tri(k, k)] = ok ? sqrt(d) : 1.0;
+        }
+        __syncthreads();
+        const double dk = A[tri(k, k)];
+        for (int i = k + 1 + tid; i < IN; i += 256) A[tri(i, k)] /= dk;
+        __syncthreads();
+        const int m = IN - k - 1;
+        for (int e = tid; e < m * m; e += 256) {
+            const int i = k + 1 + e / m, j = k + 1 + e % m;
+            if (j <= i) A[tri(i, j)] -= A[tri(i, k)] * A[tri(j, k)];
+        }
+        __syncthreads();
+    }
+    if (bad) {
+        if (tid == 0) status[0] = bad;
+        return;
+    }
+    // C y = B, then C^T x = y (one thread per right-hand side)
+    if (tid < OUT) {
+        const int o = tid;
+        for (int i = 0; i < IN; ++i) {
+            double s = B[i][o];
+            for (int j = 0; j < i; ++j) s -= A[tri(i, j)] * B[j][o];
+            B[i][o] = s / A[tri(i, i)];
+        }
+        for (int i = IN - 1; i >= 0; --i) {
+            double s = B[i][o];
+            for (int j = i + 1; j < IN; ++j) s -= A[tri(j, i)] * B[j][o];
+            B[i][o] = s / A[tri(i, i)];
+        }
+    }
+    __syncthreads();
+    for (int e = tid; e < OUT * IN; e += 256) params[P_L + e] = (float)B[e % IN][e / IN];
+    if (tid == 0) status[0] = 0;
 }
 
 }  // namespace
@@ -440,6 +611,32 @@ int lbc_adam(int nblocks, const float *part, float *params, float *m, float *v, 
                        part, params, m, v, lr, beta1, beta2, eps, (float)c1, (float)c2, wbuf,
                        grad_out, q0, q1);
     return hip_check(hipGetLastError(), "lbc_adam");
+}
+
+int lbc_gram_elems(void) { return GSZ; }
+
+int lbc_gram(int n, const float *src, const long long *offs, long long tstride, int round_bf16,
+             int nblocks, double *part, void *stream)
+{
+    if (n <= 0 || nblocks <= 0) return fail("lbc_gram: n=%d nblocks=%d", n, nblocks);
+    if (!src || !offs || !part) return fail("lbc_gram: null pointer");
+    hipLaunchKernelGGL(gram_kernel, dim3(nblocks), dim3(256), 0, S(stream), n, src, offs, tstride,
+                       round_bf16, part);
+    return hip_check(hipGetLastError(), "lbc_gram");
+}
+
+int lbc_fit_linear(int nblocks, const double *part, double *gram, double rho, float *params,
+                   int *status, void *stream)
+{
+    if (nblocks <= 0 || !part || !gram) return fail("lbc_fit_linear: nblocks=%d", nblocks);
+    if (!(rho >= 0.0)) return fail("lbc_fit_linear: rho=%g", rho);
+    hipLaunchKernelGGL(gram_reduce_kernel, dim3((GSZ + 255) / 256), dim3(256), 0, S(stream),
+                       nblocks, part, gram);
+    int rc = hip_check(hipGetLastError(), "lbc_fit_linear: reduce");
+    if (rc || !params) return rc;
+    if (!status) return fail("lbc_fit_linear: status is required with params");
+    hipLaunchKernelGGL(fit_kernel, dim3(1), dim3(256), 0, S(stream), gram, rho, params, status);
+    return hip_check(hipGetLastError(), "lbc_fit_linear: solve");
 }
 
 }  // extern "C"

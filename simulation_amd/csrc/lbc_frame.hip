@@ -52,6 +52,7 @@ struct FrameSmem {
     uint16_t A1[64 * AS], A2[64 * AS];
     float pred[PMAX][OUT];
     float srow[32];
+    float lin[32][OUT];
     int pos[PMAX];          // edge << 24 | tau
     int own[PMAX];          // global history index, or -1 when not owned
     // where each sample value comes from: bits 0-11 region cell, 12-13 field
@@ -223,7 +224,13 @@ __global__ void __launch_bounds__(256) frame_kernel(LbcFrameParams P)
             }
             float s, inv;
             store_features(v, S.Xh, r0, s, inv);
-            if (g == 0 && c < 8) S.srow[wave * 8 + c] = s;
+            float lin[OUT];
+            linear_out(P.params + P_L, v, inv, lin);
+            if (g == 0 && c < 8) {
+                S.srow[wave * 8 + c] = s;
+#pragma unroll
+                for (int d = 0; d < OUT; ++d) S.lin[wave * 8 + c][d] = lin[d];
+            }
             __syncthreads();
             float z1[16], z2[16], y[4];
             forward_tile<false>(N, S.Xh, S.A1, S.A2, r0, z1, z2, y);
@@ -235,7 +242,7 @@ __global__ void __launch_bounds__(256) frame_kernel(LbcFrameParams P)
                 if (g < 2 && c < OUT && pp < np) {
                     const float sr = S.srow[q8];
                     const float b = base_of(c, S.ring[pp][(lev - 2) & 3]);
-                    S.pred[pp][c] = sr > S_MIN ? b + sr * yo : b;
+                    S.pred[pp][c] = sr > S_MIN ? b + sr * (S.lin[q8][c] + yo) : b;
                 }
             }
             __syncthreads();

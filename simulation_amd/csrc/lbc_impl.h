@@ -7,10 +7,14 @@
 // depths 2..4 x 3 tangential offsets) to the next-step ez of the outer 3
 // cells:
 //   s = max|x|, xh = bf16(x * (1/s))  (rows with s <= 1e-30 predict base),
-//   f(xh) = W3 bf16(gelu(W2 bf16(gelu(W1 xh + b1)) + b2)) + b3 + L xh,
+//   f(xh) = W3 bf16(gelu(W2 bf16(gelu(W1 xh + b1)) + b2)) + b3,
 //   gelu(z) = z sigma(1.702 z),
-//   y = (f(xh) - f(-xh)) / 2          (odd: the exact boundary map is linear),
+//   y = L xf + (f(xh) - f(-xh)) / 2   (odd: the exact boundary map is linear),
 //   ez_next[d] = base[d] + s * y[d],
+// with xf = x * (1/s) in FP32: the linear path L carries most of the
+// correction and runs in FP32 on the vector ALUs (its least-squares
+// coefficients cancel each other, and rounded to bf16 they turn the closed
+// loop unstable); the hidden layers only correct what L leaves.
 // base = the two-step absorbing boundary of fd1d/pycuda/fd1d_1_2.py:53-61
 // applied to the three outer layers (depth d takes depth d+1's ez of two
 // steps earlier): the network learns a correction to it, and a network with
@@ -41,11 +45,10 @@ constexpr int P_W1 = 0, P_B1 = P_W1 + HID * IN, P_W2 = P_B1 + HID, P_B2 = P_W2 +
               P_W3 = P_B2 + HID, P_B3 = P_W3 + OUT * HID, P_L = P_B3 + OUT;
 // bf16 operand buffer (uint16 elements): W1 [64][128] (k >= 108 zero),
 // W2 [64][64], W3 [16][64] (rows >= 3 zero), W2^T [64][64], W3^T [64][32]
-// (k >= 3 zero), L [16][128] (rows >= 3, k >= 108 zero)
+// (k >= 3 zero); L is read in FP32 from the parameters
 constexpr int K1 = 128;
 constexpr int W_W1 = 0, W_W2 = W_W1 + HID * K1, W_W3 = W_W2 + HID * HID,
-              W_W2T = W_W3 + 16 * HID, W_W3T = W_W2T + HID * HID, W_L = W_W3T + HID * 32,
-              WBUF = W_L + 16 * K1;
+              W_W2T = W_W3 + 16 * HID, W_W3T = W_W2T + HID * HID, WBUF = W_W3T + HID * 32;
 
 // LDS row strides (bf16 elements; multiples of 8 keep 16-byte alignment and
 // spread 16 rows over distinct bank groups)
@@ -180,8 +183,28 @@ __device__ __forceinline__ float odd_out(float yk)
 
 struct Net {
     const uint16_t *w;      // bf16 operand buffer (WBUF elements)
-    const float *p;         // FP32 parameters (biases read from here)
+    const float *p;         // FP32 parameters (biases and L read from here)
 };
+
+// Linear path of the sample in tile row r: lane (r, g) holds the raw
+// features of age g in v (scale 1/s in inv); every lane of the row gets
+// lin[o] = sum_k L[o][k] x[k] inv (FP32: per-lane sums over the lane's 27
+// features in order, then the four ages as (g0 + g1) + (g2 + g3)).
+__device__ __forceinline__ void linear_out(const float *Lp, const float (&v)[NF], float inv,
+                                           float (&lin)[OUT])
+{
+    const int g = (threadIdx.x & 63) >> 4;
+#pragma unroll
+    for (int o = 0; o < OUT; ++o) {
+        const float *w = Lp + o * IN + g * NF;
+        float a = 0.0f;
+#pragma unroll
+        for (int i = 0; i < NF; ++i) a = __builtin_fmaf(w[i], v[i] * inv, a);
+        a = a + __shfl_xor(a, 16);
+        a = a + __shfl_xor(a, 32);
+        lin[o] = a;
+    }
+}
 
 // Scales and stores one 16-row tile's features.  Lane (r = lane&15,
 // g = lane>>4) holds the 27 features of sample g of row r in v[]; writes xh
@@ -286,15 +309,12 @@ __device__ __forceinline__ void forward_tile(const Net &N, const uint16_t *Xh, u
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    // layer 3: [16 x 64] x [64 x 16] (outputs 3..15 are zero rows of W3),
-    // plus the linear path [16 x 128] x [128 x 16]
+    // layer 3: [16 x 64] x [64 x 16] (outputs 3..15 are zero rows of W3)
     f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
     for (int ks = 0; ks < 2; ++ks)
         acc = mfma(ld8(A2 + (r0 + c) * AS + ks * 32 + g * 8), ld8(W3 + c * HID + ks * 32 + g * 8),
                    acc);
-#pragma unroll
-    for (int ks = 0; ks < 4; ++ks) acc = mfma(xa[ks], ld8(N.w + W_L + c * K1 + ks * 32 + g * 8), acc);
     const float b3 = c < OUT ? N.p[P_B3 + c] : 0.0f;
 #pragma unroll
     for (int v = 0; v < 4; ++v) y[v] = acc[v] + b3;

@@ -87,6 +87,41 @@ class Net:
     def numpy(self) -> np.ndarray:
         return self.params.cpu().numpy()
 
+    def set_params(self, p) -> None:
+        """New FP32 parameters (Adam moments reset, operands repacked)."""
+        self.params.copy_(torch.from_numpy(np.asarray(p, np.float32)))
+        self.m.zero_()
+        self.v.zero_()
+        self.step = 0
+        self.pack()
+
+    def gram(self, src, offs, tstride, nblocks: int = 0, round_bf16: bool = False):
+        """Per-block normal-equation sums of the linear path over the rows
+        (lbc_gram): a (nblocks, lbc_gram_elems()) FP64 tensor for fit_linear."""
+        n = offs.numel()
+        nb = nblocks or max(1, min(2048, (n + 2047) // 2048))
+        L = _lib.lib()
+        part = torch.empty((nb, L.lbc_gram_elems()), dtype=torch.float64, device=self.device)
+        call("lbc_gram", n, ptr(src), offs.data_ptr(), int(tstride), int(round_bf16), nb,
+             ptr(part, torch.float64), stream())
+        return part
+
+    def fit_linear(self, part: torch.Tensor, rho: float) -> torch.Tensor:
+        """Ridge solution of the linear path from gram() sums (FP64 Cholesky
+        on the GPU), written into the parameters; returns [G | R | count]."""
+        L = _lib.lib()
+        g = torch.empty(L.lbc_gram_elems(), dtype=torch.float64, device=self.device)
+        st = torch.zeros(1, dtype=torch.int32, device=self.device)
+        call("lbc_fit_linear", part.shape[0], ptr(part, torch.float64), ptr(g, torch.float64),
+             float(rho), ptr(self.params), st.data_ptr(), stream())
+        bad = int(st.item())
+        if bad:
+            raise _lib.FdtdError("lbc_fit_linear", bad,
+                                 f"normal equations numerically singular at pivot {bad - 1}; "
+                                 f"raise rho")
+        self.pack()
+        return g
+
     def forward(self, src: torch.Tensor, offs: torch.Tensor, tstride: int):
         """Normalised outputs y (n, 3) and scales s (n) for rows whose sample k
         starts at src[offs[r] - k * tstride]."""
@@ -221,8 +256,11 @@ class TrainConfig:
     steps: int = 520
     margin: int = 32
     seed: int = 0
-    linear_steps: int = 4000      # phase 1: the linear path alone (network output held at 0)
-    train_steps: int = 8000       # phase 2: the hidden layers, linear path frozen
+    fit_rows: int = 4_000_000     # rows (random subset) of the linear path's normal equations
+    rho: tuple = (1e-6, 1e-5, 3e-5, 1e-4, 3e-4, 1e-3)   # ridge candidates (x row count)
+    select_n: int = 256           # closed-loop validation window for rho and the hidden scale
+    scales: tuple = (0.0, 0.25, 0.5, 1.0)   # hidden-layer output scales tried after training
+    train_steps: int = 8000       # Adam steps of the hidden layers (linear path fixed)
     batch: int = 8192
     lr: float = 2e-3
     lr_min: float = 2e-5
@@ -296,15 +334,35 @@ def training_rows(G: torch.Tensor, net: Net, s_floor: float) -> torch.Tensor:
     return torch.cat(keep)
 
 
+def validation_reflection(net: Net, n: int, device=None) -> float:
+    """Closed-loop reflection ratio of the learned boundary on a validation
+    case that differs from the evaluation defaults of reflection(): pulse
+    (width 2.5) at (0.3 n, 0.6 n), cylinder (radius n/8, eps_r 3, sigma 0.01)
+    at (0.65 n, 0.35 n)."""
+    r = reflection(net, n, start=("pulse", 0.3 * n, 0.6 * n, 2.5),
+                   cylinders=[(0.65 * n, 0.35 * n, n / 8, 3.0, 0.01)], methods=("learned",),
+                   device=device)
+    return r["ratio"]["learned"]
+
+
 def train(cfg: TrainConfig, G: torch.Tensor, net: Net | None = None, log=None) -> tuple:
-    """Adam on random batches of the recorded rows, cosine learning rate, in
-    two phases: the linear path alone (the hidden layers' output held at 0
-    by W3 = 0), then the hidden layers with the linear path frozen.
-    Returns (net, history [(phase, step, train loss, validation loss)])."""
+    """Three stages, all on the GPU:
+    1. the linear path by ridge least squares (lbc_gram / lbc_fit_linear on
+       a random subset of cfg.fit_rows rows), the ridge chosen among cfg.rho
+       by the closed-loop reflection of a validation case
+       (validation_reflection at cfg.select_n) -- the one-step loss does not
+       tell a stable boundary from an unstable one;
+    2. the hidden layers by Adam (lbc_train_grad / lbc_adam) on random
+       batches, cosine learning rate, the linear path fixed: they fit what
+       it leaves;
+    3. the hidden layers' output scale chosen among cfg.scales by the same
+       closed-loop validation (0 keeps the linear path alone).
+    Returns (net, history: dicts of the stages' figures)."""
     dev = G.device
     if net is None:
         p = init_params(cfg.seed)
         p[P_W3:P_L] = 0.0
+        p[P_L:] = 0.0
         net = Net(p, dev)
     ts = G.shape[2] * NSAMP
     rows = training_rows(G, net, cfg.s_floor)
@@ -313,27 +371,64 @@ def train(cfg: TrainConfig, G: torch.Tensor, net: Net | None = None, log=None) -
     perm = torch.randperm(rows.numel(), device=dev, generator=gen)
     nval = min(65536, rows.numel() // 10)
     val, trn = rows[perm[:nval]], rows[perm[nval:]]
-    nb = max(1, min(512, (cfg.batch + 255) // 256))
-    part = torch.empty((nb, net.nparam), dtype=torch.float32, device=dev)
-    ploss = torch.empty(nb, dtype=torch.float32, device=dev)
     hist = []
+    v0, _ = net.grad(G, val, ts)
+    hist.append({"stage": "start", "validation": v0})
     if log:
-        v0, _ = net.grad(G, val, ts)
         log(f"{rows.numel()} rows ({trn.numel()} train, {nval} validation); "
             f"validation loss at start (the two-step boundary): {v0:.3e}")
-    phases = (("linear", cfg.linear_steps, (P_L, OUT * IN)), ("hidden", cfg.train_steps, (0, P_L)))
-    for name, nsteps, only in phases:
-        for k in range(nsteps):
-            lr = cfg.lr_min + 0.5 * (cfg.lr - cfg.lr_min) * (1 + math.cos(math.pi * k / nsteps))
-            idx = torch.randint(0, trn.numel(), (cfg.batch,), device=dev, generator=gen)
-            pl = net.train_step(G, trn[idx], ts, lr, nblocks=nb, part=part, ploss=ploss,
-                                noise=cfg.noise, only=only)
-            if (k + 1) % 1000 == 0 or k == nsteps - 1:
-                tl = float(pl.double().sum()) / cfg.batch
-                vl, _ = net.grad(G, val, ts)
-                hist.append((name, k + 1, tl, vl))
-                if log:
-                    log(f"{name} step {k + 1}: train {tl:.3e} validation {vl:.3e} lr {lr:.2e}")
+    # 1. linear path
+    fit = trn[torch.randperm(trn.numel(), device=dev, generator=gen)[:cfg.fit_rows]]
+    part = net.gram(G, fit, ts)
+    best = None
+    for rho in cfg.rho:
+        net.fit_linear(part, rho)
+        vl, _ = net.grad(G, val, ts)
+        rr = validation_reflection(net, cfg.select_n, dev)
+        hist.append({"stage": "linear", "rho": rho, "validation": vl, "reflection": rr})
+        if log:
+            log(f"linear path rho {rho:.0e}: validation {vl:.3e} closed-loop reflection {rr:.2e}")
+        if math.isfinite(rr) and (best is None or rr < best[0]):
+            best = (rr, rho)
+    if best is None:
+        raise RuntimeError("no ridge value gave a finite closed-loop reflection")
+    rho = best[1]
+    net.fit_linear(part, rho)
+    del part
+    # 2. hidden layers
+    nb = max(1, min(512, (cfg.batch + 255) // 256))
+    gpart = torch.empty((nb, net.nparam), dtype=torch.float32, device=dev)
+    ploss = torch.empty(nb, dtype=torch.float32, device=dev)
+    p = net.numpy()
+    p[P_W3:P_L] = init_params(cfg.seed)[P_W3:P_L]
+    net.set_params(p)
+    for k in range(cfg.train_steps):
+        lr = cfg.lr_min + 0.5 * (cfg.lr - cfg.lr_min) * (1 + math.cos(math.pi * k / cfg.train_steps))
+        idx = torch.randint(0, trn.numel(), (cfg.batch,), device=dev, generator=gen)
+        pl = net.train_step(G, trn[idx], ts, lr, nblocks=nb, part=gpart, ploss=ploss,
+                            noise=cfg.noise, only=(0, P_L))
+        if (k + 1) % 1000 == 0 or k == cfg.train_steps - 1:
+            tl = float(pl.double().sum()) / cfg.batch
+            vl, _ = net.grad(G, val, ts)
+            hist.append({"stage": "hidden", "step": k + 1, "train": tl, "validation": vl})
+            if log:
+                log(f"hidden step {k + 1}: train {tl:.3e} validation {vl:.3e} lr {lr:.2e}")
+    # 3. hidden-layer scale
+    trained = net.numpy()
+    best = None
+    for lam in cfg.scales:
+        cand = scaled(net, lam)
+        vl, _ = cand.grad(G, val, ts)
+        rr = validation_reflection(cand, cfg.select_n, dev)
+        hist.append({"stage": "scale", "scale": lam, "validation": vl, "reflection": rr})
+        if log:
+            log(f"hidden scale {lam}: validation {vl:.3e} closed-loop reflection {rr:.2e}")
+        if math.isfinite(rr) and (best is None or rr < best[0]):
+            best = (rr, lam)
+    p = trained.copy()
+    p[P_W3:P_L] *= best[1] if best else 0.0
+    net.set_params(p)
+    hist.append({"stage": "chosen", "rho": rho, "scale": best[1] if best else 0.0})
     return net, hist
 
 
@@ -488,7 +583,6 @@ def main(argv=None) -> int:
     ap.add_argument("--runs", type=int, default=512)
     ap.add_argument("--window", type=int, default=160)
     ap.add_argument("--steps", type=int, default=520)
-    ap.add_argument("--linear-steps", type=int, default=4000)
     ap.add_argument("--train-steps", type=int, default=8000)
     ap.add_argument("--batch", type=int, default=8192)
     ap.add_argument("--lr", type=float, default=2e-3)
@@ -503,8 +597,7 @@ def main(argv=None) -> int:
     def log(m):
         print(f"[{time.perf_counter() - t0:7.1f}s] {m}", flush=True)
     cfg = TrainConfig(runs=a.runs, window=a.window, steps=a.steps, train_steps=a.train_steps,
-                      linear_steps=a.linear_steps, batch=a.batch, lr=a.lr, seed=a.seed,
-                      noise=a.noise)
+                      batch=a.batch, lr=a.lr, seed=a.seed, noise=a.noise)
     dev = require_gpu()
     G = generate(cfg, dev, log)
     log(f"samples {tuple(G.shape)} ({G.numel() * 4 / 2**30:.1f} GiB)")

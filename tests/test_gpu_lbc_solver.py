@@ -76,7 +76,8 @@ def test_generic_network_matches_oracle(gpu):
     nx, ny = 80, 72
     print("[lbc solver] generic net", flush=True)
     p = L.init_params(11)
-    p[L.NPARAM - 195:] *= 3.0
+    p[L.P_W3:L.P_L] *= 3.0
+    p[L.P_L:] = (np.random.default_rng(11).standard_normal(L.NPARAM - L.P_L) * 0.05).astype(np.float32)
     md = _medium(nx, ny, gpu)
     src, probe = (34, 36), (40, 36)
     runs = (56, 24)

@@ -28,7 +28,8 @@ def _problem(rng, B=24):
     X[3] = 0.0                                        # an all-zero row (weight 0)
     T = rng.standard_normal((B, L.OUT)).astype(np.float32) * np.float32(0.1)
     p = L.init_params(3)
-    p[L.NPARAM - 200:] *= 5.0                          # non-trivial output layer
+    p[L.P_W3:L.P_L] *= 5.0                             # non-trivial output layer
+    p[L.P_L:] = rng.standard_normal(L.NPARAM - L.P_L).astype(np.float32) * np.float32(0.2)
     return p.astype(np.float32), X, T
 
 
@@ -43,8 +44,10 @@ def test_backward_matches_central_differences():
         e = (y - th) * w[:, None]
         return float(np.sum(e * e) / len(X))
     assert abs(f(p) - loss) < 1e-12
-    idx = np.concatenate([rng.choice(L.NPARAM, 40, replace=False),
-                          [0, L.NPARAM - 1, L.NPARAM - 4, 64 * 108 + 5]])
+    # the linear path is fitted by least squares, not by gradient
+    assert not np.any(g[L.P_L:])
+    idx = np.concatenate([rng.choice(L.P_L, 40, replace=False),
+                          [0, L.P_L - 1, L.P_L - 3, 64 * 108 + 5]])
     p64 = p.astype(np.float64)
     for k in idx:
         h = 1e-3 * max(1.0, abs(p64[k]))
@@ -134,3 +137,64 @@ def test_adam_first_step_closed_form():
     L.adam(p, m, v, g, 1, 0.01)
     want = -0.01 * np.sign(g) * np.abs(g) / (np.abs(g) + 1e-8)
     assert np.allclose(p, want, rtol=1e-5, atol=1e-9)
+
+
+def _planted(rng, B=400):
+    """Rows with targets that the linear path reproduces exactly: T = base +
+    s * (L xn) for a known L."""
+    X = rng.standard_normal((B, L.IN)).astype(np.float32)
+    X[:, rng.choice(L.IN, 10, replace=False)] *= np.float32(5.0)
+    X[7] = 0.0
+    S1 = L.pad_samples(X[:, L.NFEAT:2 * L.NFEAT])
+    Lt = (rng.standard_normal((L.OUT, L.IN)) * 0.3).astype(np.float64)
+    s, _, _ = L.scale(X)
+    xn = L.normalised(X).astype(np.float64)
+    T = (L.base(S1).astype(np.float64) + s[:, None] * (xn @ Lt.T)).astype(np.float32)
+    return X, T, S1, Lt
+
+
+def test_fit_linear_recovers_a_planted_map():
+    """Normal equations + solve give back the L that generated the targets
+    (to the FP32 rounding of the targets); the all-zero row is left out."""
+    rng = np.random.default_rng(11)
+    X, T, S1, Lt = _planted(rng)
+    G, R, n = L.gram(X, T, S1)
+    assert n == len(X) - 1
+    Lf = L.fit_linear(G, R, n, 0.0)
+    assert np.abs(Lf - Lt).max() < 1e-4
+    p = np.zeros(L.NPARAM, np.float32)
+    p[L.P_L:] = Lf.astype(np.float32).reshape(-1)
+    th, w = L.targets(X, T, S1)
+    y, _ = L.forward(p, X)
+    assert np.abs((y - th) * w[:, None]).max() < 1e-4
+
+
+def test_ridge_fit_is_stationary():
+    """The ridge solution zeroes the gradient of
+    sum_rows |t - L xn|^2 + rho n |L|^2, evaluated row by row."""
+    rng = np.random.default_rng(12)
+    X, T, S1, _ = _planted(rng, 300)
+    T = T + rng.standard_normal(T.shape).astype(np.float32) * np.float32(0.05)
+    G, R, n = L.gram(X, T, S1)
+    rho = 1e-2
+    Lf = L.fit_linear(G, R, n, rho)
+    th, w = L.targets(X, T, S1)
+    xn = L.normalised(X).astype(np.float64)
+    grad = np.zeros_like(Lf)
+    for b in range(len(X)):
+        if w[b]:
+            grad += -2.0 * np.outer(th[b] - Lf @ xn[b], xn[b])
+    grad += 2.0 * rho * n * Lf
+    assert np.abs(grad).max() < 1e-8 * max(1.0, np.abs(G).max())
+
+
+def test_linear_path_is_scale_free_and_odd():
+    rng = np.random.default_rng(13)
+    X = rng.standard_normal((32, L.IN)).astype(np.float32)
+    p = np.zeros(L.NPARAM, np.float32)
+    p[L.P_L:] = rng.standard_normal(L.NPARAM - L.P_L).astype(np.float32)
+    y, _ = L.forward(p, X)
+    y4, _ = L.forward(p, X * np.float32(4.0))        # power-of-two scale: exact
+    ym, _ = L.forward(p, -X)
+    assert np.array_equal(y, y4)
+    assert np.array_equal(y, -ym)
