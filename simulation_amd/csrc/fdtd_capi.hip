@@ -138,7 +138,7 @@ int plan_tm(const fd2d_tm_desc *d, int nsteps, Plan *pl)
 // memory is involved).
 struct Aux {
     hipStream_t s = nullptr, s2 = nullptr;
-    hipEvent_t fork = nullptr, join = nullptr, join2 = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr, join2 = nullptr, join0 = nullptr;
 };
 
 int aux_for_current_device(Aux **out)
@@ -155,6 +155,7 @@ int aux_for_current_device(Aux **out)
         if (!rc) rc = hip_check(hipEventCreateWithFlags(&a.fork, hipEventDisableTiming), "aux event");
         if (!rc) rc = hip_check(hipEventCreateWithFlags(&a.join, hipEventDisableTiming), "aux event");
         if (!rc) rc = hip_check(hipEventCreateWithFlags(&a.join2, hipEventDisableTiming), "aux event");
+        if (!rc) rc = hip_check(hipEventCreateWithFlags(&a.join0, hipEventDisableTiming), "aux event");
         if (rc) return rc;
     }
     *out = &a;
@@ -643,7 +644,12 @@ int fd2d_lbc_run(const fd2d_tm_desc *d, const fd2d_lbc_desc *b, int t0, int nste
     p.nx = d->nx; p.ny = d->ny;
     p.naz = d->md.naz; p.nbz = d->md.nbz;
     p.halo = pl.halo; p.rows = pl.rows;
-    p.rlo = 1; p.rhi = d->nx - 1; p.clo = 1; p.chi = d->ny - 1;
+    // interior part: waves whose reads and writes stay DZ or more cells from
+    // every edge.  The frame kernel reads less than DZ = 3K+6 cells deep and
+    // writes less than K+3 deep, so the interior part of launch l+1 needs
+    // neither the frame of launch l nor conflicts with it.
+    const int DZ = 3 * pl.depth + 6;
+    p.rlo = DZ; p.rhi = d->nx - DZ; p.clo = DZ; p.chi = d->ny - DZ;
     p.nsrc = d->nsrc;
     for (int k = 0; k < d->nsrc; ++k) { p.src_i[k] = d->src_i[k]; p.src_j[k] = d->src_j[k]; }
     p.src_wave = d->src_wave; p.wave_len = d->wave_len;
@@ -656,9 +662,26 @@ int fd2d_lbc_run(const fd2d_tm_desc *d, const fd2d_lbc_desc *b, int t0, int nste
     Aux *ax = nullptr;
     rc = aux_for_current_device(&ax);
     if (rc) return rc;
+    // geometry tables of the (at most two) launch depths
+    const int kq = nsteps / L, kr = nsteps % L;
+    const void *tabq = nullptr, *tabr = nullptr;
+    rc = lbc_frame_tables(d->nx, d->ny, kq, &tabq);
+    if (!rc && kr) rc = lbc_frame_tables(d->nx, d->ny, kq + 1, &tabr);
+    if (rc) return rc;
+    // three streams: interior part P0 on the caller's, boundary part P1 on
+    // ax->s, frame F on ax->s2.  Launch l:
+    //   P0(l) after P0(l-1), P1(l-1)   (reads and overwrites what they used)
+    //   P1(l) after P0(l-1), P1(l-1), F(l-1)
+    //   F(l)  after P1(l-1), F(l-1)    (P0 stays DZ cells clear of the frame)
+    // events: join0 = last P0, join = last P1, join2 = last F.
+    hipStream_t s0 = S(stream);
+    rc = hip_check(hipEventRecord(ax->fork, s0), "fork event");
+    if (!rc) rc = hip_check(hipStreamWaitEvent(ax->s, ax->fork, 0), "fork wait");
+    if (!rc) rc = hip_check(hipStreamWaitEvent(ax->s2, ax->fork, 0), "fork wait");
+    if (rc) return rc;
     int done = 0;
     for (int l = 0; l < L; ++l) {
-        const int k = nsteps / L + (l < nsteps % L ? 1 : 0);
+        const int k = kq + (l < kr ? 1 : 0);
         float **in = (l % 2 == 0) ? A : B;
         float **out = (l % 2 == 0) ? B : A;
         p.dz_i = in[0]; p.iz_i = in[1]; p.hx_i = in[2]; p.hy_i = in[3];
@@ -668,10 +691,6 @@ int fd2d_lbc_run(const fd2d_tm_desc *d, const fd2d_lbc_desc *b, int t0, int nste
         p.ez_o = (l == L - 1) ? d->ez : nullptr;
         p.t_first = t0 + done + 1;
         p.skip_w = k + 3;          // the frame kernel writes the outer k+3 cells
-        // the frame of depth k+3, recomputed with the network: it reads the
-        // launch's input set only and writes cells the blocked launch leaves
-        // alone, so the three kernels run concurrently (frame and boundary
-        // part on two library streams, interior part on the caller's)
         LbcFrameParams f{};
         f.nx = d->nx; f.ny = d->ny; f.K = k;
         f.dz_i = in[0]; f.iz_i = lossy ? in[1] : nullptr; f.hx_i = in[2]; f.hy_i = in[3];
@@ -680,19 +699,28 @@ int fd2d_lbc_run(const fd2d_tm_desc *d, const fd2d_lbc_desc *b, int t0, int nste
         f.naz = d->md.naz; f.nbz = d->md.nbz;
         f.hist_in = H[l % 2]; f.hist_out = H[(l + 1) % 2];
         f.w = b->wbuf; f.params = b->params;
-        rc = hip_check(hipEventRecord(ax->fork, S(stream)), "fork event");
-        if (!rc) rc = hip_check(hipStreamWaitEvent(ax->s2, ax->fork, 0), "fork wait");
-        if (!rc) rc = hip_check(hipStreamWaitEvent(ax->s, ax->fork, 0), "fork wait");
-        if (!rc) rc = launch_lbc_frame(f, ax->s2);
+        f.tab = (l < kr) ? tabr : tabq;
+        if (l > 0) {
+            // waits on the events of launch l-1 (recorded below), issued
+            // before this launch records them again
+            rc = hip_check(hipStreamWaitEvent(ax->s, ax->join0, 0), "P1 wait P0");
+            if (!rc) rc = hip_check(hipStreamWaitEvent(ax->s, ax->join2, 0), "P1 wait F");
+            if (!rc) rc = hip_check(hipStreamWaitEvent(ax->s2, ax->join, 0), "F wait P1");
+            if (!rc) rc = hip_check(hipStreamWaitEvent(s0, ax->join, 0), "P0 wait P1");
+            if (rc) return rc;
+        }
+        rc = launch_lbc_frame(f, ax->s2);
+        if (!rc) rc = hip_check(hipEventRecord(ax->join2, ax->s2), "F event");
         if (!rc) rc = launch_fd2d_tb(p, k, pl.cols, lossy, false, 1, ax->s);
-        if (!rc) rc = launch_fd2d_tb(p, k, pl.cols, lossy, false, 0, S(stream));
-        if (!rc) rc = hip_check(hipEventRecord(ax->join, ax->s), "join event");
-        if (!rc) rc = hip_check(hipEventRecord(ax->join2, ax->s2), "join event");
-        if (!rc) rc = hip_check(hipStreamWaitEvent(S(stream), ax->join, 0), "join wait");
-        if (!rc) rc = hip_check(hipStreamWaitEvent(S(stream), ax->join2, 0), "join wait");
+        if (!rc) rc = hip_check(hipEventRecord(ax->join, ax->s), "P1 event");
+        if (!rc) rc = launch_fd2d_tb(p, k, pl.cols, lossy, false, 0, s0);
+        if (!rc) rc = hip_check(hipEventRecord(ax->join0, s0), "P0 event");
         if (rc) return rc;
         done += k;
     }
+    rc = hip_check(hipStreamWaitEvent(s0, ax->join, 0), "join wait");
+    if (!rc) rc = hip_check(hipStreamWaitEvent(s0, ax->join2, 0), "join wait");
+    if (rc) return rc;
     return 0;
 }
 

@@ -76,16 +76,20 @@ struct LbcFrameParams {
     const float *dz_i, *iz_i, *hx_i, *hy_i;     // state at the launch start
     float *dz_o, *iz_o, *hx_o, *hy_o, *ez_o;    // state after K steps (ez_o may be NULL)
     const float *naz, *nbz;
-    const float *hist_in;                       // [npos][4][27], newest sample first
+    const float *hist_in;                       // [npos][4][30], newest sample first
     float *hist_out;
     const uint16_t *w;                          // network: bf16 operands
-    const float *params;                        //          FP32 parameters (biases)
+    const float *params;                        //          FP32 parameters (biases, L)
+    const void *tab;                            // geometry tables (lbc_frame_tables)
 };
 int launch_fd2d_energy_diff(int wn, int wm, int nya, int ia0, int ja0, const float *eza,
                             const float *hxa, const float *hya, int nyb, int ib0, int jb0,
                             const float *ezb, const float *hxb, const float *hyb, double *work,
                             hipStream_t s);
 size_t lbc_frame_smem();
+// Geometry tables of the frame workgroups for an nx x ny grid and depth K on
+// the current device: built on first use (and kept until the process ends).
+int lbc_frame_tables(int nx, int ny, int K, const void **tab);
 int lbc_frame_blocks(int nx, int ny, int K);
 int launch_lbc_frame(const LbcFrameParams &p, hipStream_t s);
 

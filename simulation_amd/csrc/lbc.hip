@@ -55,9 +55,23 @@ __global__ void __launch_bounds__(256) pack_kernel(const float *p, uint16_t *w)
     } else if (q < W_W3T) {
         int k = (q - W_W2T) / HID, j = (q - W_W2T) % HID;      // W2T[k][j] = W2[j][k]
         v = p[P_W2 + j * HID + k];
-    } else {
+    } else if (q < W_FRM) {
         int k = (q - W_W3T) / 32, j = (q - W_W3T) % 32;        // W3T[k][j] = W3[j][k]
         if (j < OUT) v = p[P_W3 + j * HID + k];
+    } else {
+        // per-lane operands of the transposed network (inverse of frm_slot)
+        const int e = q - W_FRM, lane = e / (FRM_OPS * 8), op = (e / 8) % FRM_OPS, vv = e % 8;
+        const int c = lane & 15, g = lane >> 4;
+        const int u = (2 * ((op - 16) % 2) + vv / 4) * 16 + 4 * g + vv % 4;
+        if (op < 16) {
+            const int i = 8 * (op % 4) + vv;
+            if (i < NF) v = p[P_W1 + ((op / 4) * 16 + c) * IN + g * NF + i];
+        } else if (op < 24) {
+            v = p[P_W2 + (((op - 16) / 2) * 16 + c) * HID + u];
+        } else if (c < OUT) {
+            const int u3 = (2 * (op - 24) + vv / 4) * 16 + 4 * g + vv % 4;
+            v = p[P_W3 + c * HID + u3];
+        }
     }
     w[q] = f2bf(v);
 }
@@ -336,6 +350,8 @@ __global__ void __launch_bounds__(256) adam_kernel(int nblocks, const float *par
     const float val = p[q] - lr * (mm / c1) / (sqrtf(vv / c2) + eps);
     p[q] = val;
     const uint16_t h = f2bf(val);
+    const int fs = frm_slot(q);
+    if (fs >= 0) w[fs] = h;
     if (q < P_B1) {
         const int j = q / IN, k = q % IN;
         w[W_W1 + j * K1 + k] = h;

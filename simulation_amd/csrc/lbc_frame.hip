@@ -21,13 +21,27 @@
 // cells before and 2K+1 after the chunk.  Per step: network for every boundary position
 // whose cells lie in the region (features = the position's last four
 // canonical samples, kept in an LDS ring), D and E update, ez of the
-// predicted cells replaced, H update, new samples pushed.  The ring enters
+// predicted cells replaced, H update, new samples pushed.
+//
+// The network runs transposed, D^T = W X^T: the 16 MFMA columns are 8
+// positions and their negations (the odd network), the rows are hidden
+// units.  Lane (c, g) of a wave supplies column c's features of age g, so
+// the input operand is built from the lane's own ring reads, and a layer's
+// accumulator (hidden units 16 mt + 4 g + v of column c) is, after GELU and
+// rounding, the next layer's input operand once the next layer's weight
+// columns are permuted to match (unit (2 ks + v/4) 16 + 4 g + v%4 at K index
+// 32 ks + 8 g + v): no LDS between the layers.  The layer weights sit in
+// registers for the whole kernel; the FP32 linear path and the biases in
+// LDS.  b3 is left out: the odd combination cancels it.  The ring enters
 // and leaves through a global history buffer (four samples per position,
 // newest first; ping-ponged between launches).  Expressions follow
 // fd2d/program/fd2d_3_3.py:68-98 (free space) with the lossy E update of
 // fd2d/pycuda/fd2d_3_4.py:181-182, operand for operand, no contraction.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#include <mutex>
+#include <vector>
 
 #include "fdtd_internal.h"
 #include "lbc_impl.h"
@@ -38,6 +52,7 @@ namespace {
 
 constexpr int CH = 40;                        // chunk length along the edge
 constexpr int KMAX = 8;
+constexpr int NT = 512;                       // threads per workgroup (8 waves)
 // region cells: corner chunks (3K+6) x (CH + 2K + 1), others at most
 // (2K+6) x (CH + 3K + 2) (30 x 57 and 22 x 66 at K = 8)
 constexpr int RMAX_C = (3 * KMAX + 6) * (CH + 2 * KMAX + 1);
@@ -48,11 +63,9 @@ constexpr int PMAX = 128;                     // positions per workgroup
 struct FrameSmem {
     float dz[RMAX], iz[RMAX], ez[RMAX], hx[RMAX], hy[RMAX], na[RMAX], nb[RMAX];
     float ring[PMAX][HIST][NS];
-    uint16_t Xh[64 * XS];
-    uint16_t A1[64 * AS], A2[64 * AS];
     float pred[PMAX][OUT];
-    float srow[32];
-    float lin[32][OUT];
+    float L[OUT][IN];       // linear path (FP32)
+    float b1[HID], b2[HID];
     int pos[PMAX];          // edge << 24 | tau
     int own[PMAX];          // global history index, or -1 when not owned
     // where each sample value comes from: bits 0-11 region cell, 12-13 field
@@ -61,6 +74,16 @@ struct FrameSmem {
 };
 
 static_assert(sizeof(FrameSmem) + 64 <= 160 * 1024, "frame kernel LDS above 160 KiB");
+
+// A workgroup's positions and sample sources depend only on the grid shape
+// and the depth: built once (frame_setup_kernel) into global memory.
+struct FrameTab {
+    int np, pad[3];
+    int nact[KMAX + 4];     // nact[l]: positions needed at level l (a prefix)
+    int pos[PMAX], own[PMAX];
+    uint16_t src[PMAX][NS];
+};
+static_assert(sizeof(FrameTab) % 16 == 0, "table stride");
 
 struct Geo {
     int ri0, rj0, RH, RW;           // region origin and size (global)
@@ -77,25 +100,63 @@ __device__ inline int gidx(int e, int t, int nx, int ny)
     }
 }
 
-__global__ void __launch_bounds__(256) frame_kernel(LbcFrameParams P)
+// Layer weights of one lane in MFMA A-operand form (W_FRM of the bf16
+// buffer, in frm_slot's order).
+struct LaneW {
+    bf16x8 w1[4][4];        // hidden tile mt, K step ks
+    bf16x8 w2[4][2];
+    bf16x8 w3[2];
+};
+
+__device__ __forceinline__ void load_lane_w(LaneW &W, const uint16_t *wbuf, int lane)
 {
-    extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
-    FrameSmem &S = *reinterpret_cast<FrameSmem *>(smem_raw);
-    const int nx = P.nx, ny = P.ny, K = P.K, W = K + 3;
-    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, c = lane & 15, g = lane >> 4;
+    const uint16_t *b = wbuf + W_FRM + lane * FRM_OPS * 8;
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+        for (int ks = 0; ks < 4; ++ks) W.w1[mt][ks] = ld8(b + (mt * 4 + ks) * 8);
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+        for (int ks = 0; ks < 2; ++ks) W.w2[mt][ks] = ld8(b + (16 + mt * 2 + ks) * 8);
+#pragma unroll
+    for (int ks = 0; ks < 2; ++ks) W.w3[ks] = ld8(b + (24 + ks) * 8);
+}
+
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+
+// GELU of the accumulators of 4 hidden tiles (+ bias), rounded to bf16 and
+// packed as the next layer's two K steps.
+__device__ __forceinline__ void act_pack(const f32x4 (&z)[4], const float *bias, int g,
+                                         bf16x8 (&out)[2])
+{
+#pragma unroll
+    for (int ks = 0; ks < 2; ++ks) {
+        bf16x2 h[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int mt = 2 * ks + (u >> 1), v0 = 2 * (u & 1);
+            f32x2 a;
+            a.x = gelu(z[mt][v0] + bias[mt * 16 + 4 * g + v0]);
+            a.y = gelu(z[mt][v0 + 1] + bias[mt * 16 + 4 * g + v0 + 1]);
+            h[u] = __builtin_convertvector(a, bf16x2);
+        }
+        out[ks] = __builtin_bit_cast(bf16x8, h);
+    }
+}
+
+// Region and owned cells of workgroup b (kind: 0 top, 1 bottom, 2 left,
+// 3 right chunk; q: chunk index along its edge).
+__device__ inline void block_geo(int b, int nx, int ny, int K, Geo &G, int &kind, int &q)
+{
+    const int W = K + 3;
     const int nct = (ny + CH - 1) / CH;
     const int ncl = (nx - 2 * W + CH - 1) / CH;
-
-    // ---------------------------------------------------------- geometry
-    Geo G;
-    int kind, q;                  // 0 top, 1 bottom, 2 left, 3 right
-    {
-        int b = blockIdx.x;
-        if (b < nct) { kind = 0; q = b; }
-        else if (b < 2 * nct) { kind = 1; q = b - nct; }
-        else if (b < 2 * nct + ncl) { kind = 2; q = b - 2 * nct; }
-        else { kind = 3; q = b - 2 * nct - ncl; }
-    }
+    if (b < nct) { kind = 0; q = b; }
+    else if (b < 2 * nct) { kind = 1; q = b - nct; }
+    else if (b < 2 * nct + ncl) { kind = 2; q = b - 2 * nct; }
+    else { kind = 3; q = b - 2 * nct - ncl; }
     int R0 = 2 * K + 6;
     if (kind < 2) {
         const int c0 = q * CH, c1 = min(ny, c0 + CH);
@@ -117,11 +178,23 @@ __global__ void __launch_bounds__(256) frame_kernel(LbcFrameParams P)
         G.oj0 = kind == 2 ? 0 : ny - W;
         G.oj1 = kind == 2 ? W : ny;
     }
-    const int RH = G.RH, RW = G.RW, NC = RH * RW;
+}
 
+// One workgroup per frame workgroup: its position list (thread 0; at most
+// CH + 3K + 2 + 2 (3K + 3) = 120 positions) and sample-source table.
+__global__ void __launch_bounds__(NT) frame_setup_kernel(int nx, int ny, int K, FrameTab *tabs)
+{
+    __shared__ FrameTab S;
+    FrameTab &T = tabs[blockIdx.x];
+    const int tid = threadIdx.x;
+    const int W = K + 3;
+    const int nct = (ny + CH - 1) / CH;
+    Geo G;
+    int kind, q;
+    block_geo(blockIdx.x, nx, ny, K, G, kind, q);
+    const int RH = G.RH, RW = G.RW;
     // ---------------------------------------------------------- positions
     // (thread 0 builds the list; at most CH + 3K + 2 + 2 (3K + 3) = 120)
-    __shared__ int np_s;
     if (tid == 0) {
         int n = 0;
         auto add = [&](int e, int t0, int t1, int w0, int w1) {
@@ -147,37 +220,55 @@ __global__ void __launch_bounds__(256) frame_kernel(LbcFrameParams P)
             if (ilo <= 2) add(0, t0, t1, 0, -1);
             if (ihi >= nx - 2) add(1, t0, t1, 0, -1);
         }
-        np_s = n;
+        S.np = n;
     }
 
-    // ---------------------------------------------------------- load
-    for (int k = tid; k < RH * RW; k += 256) {
-        const int r = k / RW, cc = k % RW;
-        const long o = (long)(G.ri0 + r) * ny + (G.rj0 + cc);
-        S.dz[k] = P.dz_i[o];
-        S.iz[k] = P.iz_i ? P.iz_i[o] : 0.0f;
-        S.hx[k] = P.hx_i[o];
-        S.hy[k] = P.hy_i[o];
-        S.na[k] = P.naz[o];
-        S.nb[k] = P.nbz ? P.nbz[o] : 0.0f;
-        S.ez[k] = 0.0f;
+    __syncthreads();
+    const int np = S.np;
+    // Levels at which a position is needed.  Along an edge the valid part of
+    // a region shrinks per step by one cell at a cut on the low side and by
+    // two at a cut on the high side, so position t (region range [lo, hi)
+    // in its edge's coordinate; a side at the grid's edge is no cut) has a
+    // valid prediction up to level min(t - lo, (hi - t) / 2); it is
+    // evaluated one level beyond (margin).  Positions sorted by that level,
+    // descending (counting sort, stable): level l works on a prefix.
+    __shared__ int key[PMAX], cnt[KMAX + 2], at[KMAX + 2];
+    __shared__ int tpos[PMAX], town[PMAX];
+    for (int k = tid; k < np; k += NT) {
+        const int e = S.pos[k] >> 24, t = S.pos[k] & 0xFFFFFF;
+        const int lo = e < 2 ? G.rj0 : G.ri0, hi = e < 2 ? G.rj0 + RW : G.ri0 + RH;
+        const int Lg = e < 2 ? ny : nx;
+        int lm = K;
+        if (lo > 0) lm = min(lm, t - lo + 1);
+        if (hi < Lg) lm = min(lm, (hi - t) / 2 + 1);
+        key[k] = max(1, lm);
+        tpos[k] = S.pos[k];
+        town[k] = S.own[k];
     }
     __syncthreads();
-    const int np = np_s;
-    // history ring: sample of level l (l = -3..K) in slot l & 3
-    for (int k = tid; k < np * HIST * NS; k += 256) {
-        const int p = k / (HIST * NS), a = (k / NS) % HIST, i = k % NS;
-        int e = S.pos[p] >> 24, t = S.pos[p] & 0xFFFFFF;
-        S.ring[p][(-a) & 3][i] = P.hist_in[((long)gidx(e, t, nx, ny) * HIST + a) * NS + i];
+    if (tid == 0) {
+        for (int l = 0; l <= K + 1; ++l) cnt[l] = 0;
+        for (int k = 0; k < np; ++k) ++cnt[key[k]];
+        int o = 0;
+        for (int l = K; l >= 1; --l) { at[l] = o; o += cnt[l]; }
+        for (int k = 0; k < np; ++k) {
+            const int d = at[key[k]]++;
+            S.pos[d] = tpos[k];
+            S.own[d] = town[k];
+        }
+        int n = 0;
+        for (int l = K; l >= 1; --l) {
+            n += cnt[l];
+            T.nact[l] = n;
+        }
+        T.nact[0] = np;
     }
     __syncthreads();
-
-    const Net N{P.w, P.params};
     // sample sources, resolved once: canon() run on an accessor that returns
     // the address instead of the value (clamped into the region: positions
     // near a cut side read stale neighbours, within the part of the region
     // that is no longer valid anyway)
-    for (int k = tid; k < np * NS; k += 256) {
+    for (int k = tid; k < np * NS; k += NT) {
         const int p = k / NS, i = k % NS;
         const int e = S.pos[p] >> 24, t = S.pos[p] & 0xFFFFFF;
         int code = -1;                       // set by the accessor
@@ -198,57 +289,172 @@ __global__ void __launch_bounds__(256) frame_kernel(LbcFrameParams P)
         uint16_t v;
         if (code < 0 || sg == 0.0f) v = (uint16_t)(3 << 12);
         else v = (uint16_t)(code | (sg < 0.0f ? 0x8000 : 0));
-        S.src[p][i] = v;
+        T.src[p][i] = v;
     }
+    for (int k = tid; k < np; k += NT) {
+        T.pos[k] = S.pos[k];
+        T.own[k] = S.own[k];
+    }
+    if (tid == 0) T.np = np;
+}
+
+__global__ void __launch_bounds__(NT) frame_kernel(LbcFrameParams P)
+{
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+    FrameSmem &S = *reinterpret_cast<FrameSmem *>(smem_raw);
+    const int nx = P.nx, ny = P.ny, K = P.K, W = K + 3;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, c = lane & 15, g = lane >> 4;
+    constexpr int NW = NT / 64;
+
+    Geo G;
+    int kind, q;
+    block_geo(blockIdx.x, nx, ny, K, G, kind, q);
+    const FrameTab &T = reinterpret_cast<const FrameTab *>(P.tab)[blockIdx.x];
+    const int RH = G.RH, RW = G.RW, NC = RH * RW;
+
+    // ---------------------------------------------------------- tables
+    const int np = T.np;
+    __shared__ int nact_s[KMAX + 4];
+    if (tid <= K) nact_s[tid] = T.nact[tid];
+    for (int k = tid; k < np; k += NT) {
+        S.pos[k] = T.pos[k];
+        S.own[k] = T.own[k];
+    }
+    {
+        const uint32_t *s32 = reinterpret_cast<const uint32_t *>(&T.src[0][0]);
+        uint32_t *d32 = reinterpret_cast<uint32_t *>(&S.src[0][0]);
+        for (int k = tid; k < np * NS / 2; k += NT) d32[k] = s32[k];
+    }
+    // ---------------------------------------------------------- load
+    for (int k = tid; k < RH * RW; k += NT) {
+        const int r = k / RW, cc = k % RW;
+        const long o = (long)(G.ri0 + r) * ny + (G.rj0 + cc);
+        S.dz[k] = P.dz_i[o];
+        S.iz[k] = P.iz_i ? P.iz_i[o] : 0.0f;
+        S.hx[k] = P.hx_i[o];
+        S.hy[k] = P.hy_i[o];
+        S.na[k] = P.naz[o];
+        S.nb[k] = P.nbz ? P.nbz[o] : 0.0f;
+        S.ez[k] = 0.0f;
+    }
+    __syncthreads();
+    // history ring: sample of level l (l = -3..K) in slot l & 3
+    for (int k = tid; k < np * HIST * NS; k += NT) {
+        const int p = k / (HIST * NS), a = (k / NS) % HIST, i = k % NS;
+        int e = S.pos[p] >> 24, t = S.pos[p] & 0xFFFFFF;
+        S.ring[p][(-a) & 3][i] = P.hist_in[((long)gidx(e, t, nx, ny) * HIST + a) * NS + i];
+    }
+    __syncthreads();
+
+    for (int k = tid; k < OUT * IN; k += NT) S.L[k / IN][k % IN] = P.params[P_L + k];
+    for (int k = tid; k < HID; k += NT) {
+        S.b1[k] = P.params[P_B1 + k];
+        S.b2[k] = P.params[P_B2 + k];
+    }
+    LaneW LW;
+    load_lane_w(LW, P.w, lane);
     // cell loops without divisions: thread tid starts at cell tid and steps
-    // by 256 = DR rows + DC columns
-    const int DR = 256 / RW, DC = 256 % RW;
+    // by NT = DR rows + DC columns
+    const int DR = NT / RW, DC = NT % RW;
     const int r_start = tid / RW, c_start = tid % RW;
     __syncthreads();
 
     for (int lev = 1; lev <= K; ++lev) {
         // ------------------------------------------------------ network
-        // 32 positions per round: tile rows c and c + 8 hold position
-        // base + 8 wave + (c & 7) and its negation (odd network)
-        for (int base = 0; base < np; base += 32) {
-            const int r0 = wave * 16;
-            const int p = base + wave * 8 + (c & 7);
-            float v[NF];
-            if (p < np) {
-                const float *src = S.ring[p][(lev - 1 - g) & 3];
+        // tiles of 8 positions, tile q on wave q % NW; no barrier before the
+        // D/E update, which touches neither the ring nor pred
+        const int na = nact_s[lev];
+        for (int tb = wave * 8; tb < na; tb += NW * 8) {
+            const int q = tb + (c & 7);
+            float x[NF];
+            if (q < na) {
+                const float *src = S.ring[q][(lev - 1 - g) & 3];
 #pragma unroll
-                for (int i = 0; i < NF; ++i) v[i] = src[i];
+                for (int i = 0; i < NF; ++i) x[i] = src[i];
             } else {
 #pragma unroll
-                for (int i = 0; i < NF; ++i) v[i] = 0.0f;
+                for (int i = 0; i < NF; ++i) x[i] = 0.0f;
             }
-            float s, inv;
-            store_features(v, S.Xh, r0, s, inv);
+            float mx = 0.0f;
+#pragma unroll
+            for (int i = 0; i < NF; ++i) mx = fmaxf(mx, fabsf(x[i]));
+            mx = fmaxf(mx, __shfl_xor(mx, 16));
+            mx = fmaxf(mx, __shfl_xor(mx, 32));
+            const float inv = mx > S_MIN ? 1.0f / mx : 0.0f;
+            // FP32 linear path: lanes c and c + 8 hold the same position and
+            // split its 27 features of age g (0..13 and 14..26)
             float lin[OUT];
-            linear_out(P.params + P_L, v, inv, lin);
-            if (g == 0 && c < 8) {
-                S.srow[wave * 8 + c] = s;
+            {
+                const int h0 = c < 8 ? 0 : 14;
+                float xs[14];
 #pragma unroll
-                for (int d = 0; d < OUT; ++d) S.lin[wave * 8 + c][d] = lin[d];
-            }
-            __syncthreads();
-            float z1[16], z2[16], y[4];
-            forward_tile<false>(N, S.Xh, S.A1, S.A2, r0, z1, z2, y);
+                for (int i = 0; i < 14; ++i) {
+                    const float lo = x[i], hi = i + 14 < NF ? x[i + 14] : 0.0f;
+                    xs[i] = (c < 8 ? lo : hi) * inv;
+                }
 #pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const float yo = odd_out(y[k]);
-                const int q8 = wave * 8 + g * 4 + k;          // valid for g < 2
-                const int pp = base + q8;
-                if (g < 2 && c < OUT && pp < np) {
-                    const float sr = S.srow[q8];
-                    const float b = base_of(c, S.ring[pp][(lev - 2) & 3]);
-                    S.pred[pp][c] = sr > S_MIN ? b + sr * (S.lin[q8][c] + yo) : b;
+                for (int o = 0; o < OUT; ++o) {
+                    const float *w = &S.L[o][g * NF + h0];
+                    float a = 0.0f;
+#pragma unroll
+                    for (int i = 0; i < 14; ++i)
+                        if (i < 13 || c < 8) a = __builtin_fmaf(w[i], xs[i], a);
+                    a = a + __shfl_xor(a, 8);
+                    a = a + __shfl_xor(a, 16);
+                    a = a + __shfl_xor(a, 32);
+                    lin[o] = a;
                 }
             }
-            __syncthreads();
+            // layer 1 input: column c < 8 the features, c >= 8 their negation
+            const float sc = c < 8 ? inv : -inv;
+            bf16x8 xb[4];
+#pragma unroll
+            for (int ks = 0; ks < 4; ++ks) {
+                bf16x2 h[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int i0 = 8 * ks + 2 * u;
+                    f32x2 a;
+                    a.x = i0 < NF ? x[i0] * sc : 0.0f;
+                    a.y = i0 + 1 < NF ? x[i0 + 1] * sc : 0.0f;
+                    h[u] = __builtin_convertvector(a, bf16x2);
+                }
+                xb[ks] = __builtin_bit_cast(bf16x8, h);
+            }
+            f32x4 z[4];
+#pragma unroll
+            for (int mt = 0; mt < 4; ++mt) {
+                z[mt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+                for (int ks = 0; ks < 4; ++ks) z[mt] = mfma(LW.w1[mt][ks], xb[ks], z[mt]);
+            }
+            bf16x8 a1[2];
+            act_pack(z, S.b1, g, a1);
+#pragma unroll
+            for (int mt = 0; mt < 4; ++mt) {
+                z[mt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+                for (int ks = 0; ks < 2; ++ks) z[mt] = mfma(LW.w2[mt][ks], a1[ks], z[mt]);
+            }
+            bf16x8 a2[2];
+            act_pack(z, S.b2, g, a2);
+            f32x4 y = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int ks = 0; ks < 2; ++ks) y = mfma(LW.w3[ks], a2[ks], y);
+            // outputs v = 0..2 of column c in lanes g = 0: odd part from the
+            // column pair (c, c + 8)
+#pragma unroll
+            for (int v = 0; v < OUT; ++v) {
+                const float other = __shfl_xor(y[v], 8);
+                if (g == 0 && c < 8 && q < na) {
+                    const float yo = 0.5f * (y[v] - other);
+                    const float b = base_of(v, S.ring[q][(lev - 2) & 3]);
+                    S.pred[q][v] = mx > S_MIN ? b + mx * (lin[v] + yo) : b;
+                }
+            }
         }
         // ------------------------------------------------------ D, E update
-        for (int k = tid, r = r_start, cc = c_start; k < NC; k += 256) {
+        for (int k = tid, r = r_start, cc = c_start; k < NC; k += NT) {
             const int i = G.ri0 + r, j = G.rj0 + cc;
             float dz = S.dz[k];
             if (i >= 1 && j >= 1 && r >= 1 && cc >= 1) {
@@ -269,7 +475,7 @@ __global__ void __launch_bounds__(256) frame_kernel(LbcFrameParams P)
         }
         __syncthreads();
         // ------------------------------------------------------ learned cells
-        for (int k = tid; k < np * OUT; k += 256) {
+        for (int k = tid; k < na * OUT; k += NT) {
             const int p = k / OUT, d = k % OUT;
             const int e = S.pos[p] >> 24, t = S.pos[p] & 0xFFFFFF;
             int i, j;
@@ -279,7 +485,7 @@ __global__ void __launch_bounds__(256) frame_kernel(LbcFrameParams P)
         }
         __syncthreads();
         // ------------------------------------------------------ H update
-        for (int k = tid, r = r_start, cc = c_start; k < NC; k += 256) {
+        for (int k = tid, r = r_start, cc = c_start; k < NC; k += NT) {
             const int i = G.ri0 + r, j = G.rj0 + cc;
             if (i < nx - 1 && j < ny - 1 && r + 1 < RH && cc + 1 < RW) {
                 const float ez = S.ez[k];
@@ -291,7 +497,7 @@ __global__ void __launch_bounds__(256) frame_kernel(LbcFrameParams P)
         }
         __syncthreads();
         // ------------------------------------------------------ new samples
-        for (int k = tid; k < np * NS; k += 256) {
+        for (int k = tid; k < na * NS; k += NT) {
             const int p = k / NS, i = k % NS;
             const unsigned sv = S.src[p][i];
             const unsigned f = (sv >> 12) & 3u, cell = sv & 0xFFFu;
@@ -303,7 +509,7 @@ __global__ void __launch_bounds__(256) frame_kernel(LbcFrameParams P)
     }
 
     // ---------------------------------------------------------- store
-    for (int k = tid; k < NC; k += 256) {
+    for (int k = tid; k < NC; k += NT) {
         const int r = k / RW, cc = k % RW;
         const int i = G.ri0 + r, j = G.rj0 + cc;
         if (i < G.oi0 || i >= G.oi1 || j < G.oj0 || j >= G.oj1) continue;
@@ -314,7 +520,7 @@ __global__ void __launch_bounds__(256) frame_kernel(LbcFrameParams P)
         P.hy_o[o] = S.hy[k];
         if (P.ez_o) P.ez_o[o] = S.ez[k];
     }
-    for (int k = tid; k < np * HIST * NS; k += 256) {
+    for (int k = tid; k < np * HIST * NS; k += NT) {
         const int p = k / (HIST * NS), a = (k / NS) % HIST, i = k % NS;
         if (S.own[p] < 0) continue;
         P.hist_out[((long)S.own[p] * HIST + a) * NS + i] = S.ring[p][(K - a) & 3][i];
@@ -325,6 +531,46 @@ __global__ void __launch_bounds__(256) frame_kernel(LbcFrameParams P)
 }  // namespace lbc
 
 size_t lbc_frame_smem() { return sizeof(lbc::FrameSmem); }
+
+int lbc_frame_tables(int nx, int ny, int K, const void **tab)
+{
+    struct Entry {
+        int dev, nx, ny, K;
+        lbc::FrameTab *p;
+    };
+    static std::mutex mu;
+    static std::vector<Entry> cache;
+    int dev = 0;
+    int rc = hip_check(hipGetDevice(&dev), "hipGetDevice");
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lk(mu);
+    for (const Entry &e : cache)
+        if (e.dev == dev && e.nx == nx && e.ny == ny && e.K == K) {
+            *tab = e.p;
+            return 0;
+        }
+    const int nb = lbc_frame_blocks(nx, ny, K);
+    lbc::FrameTab *p = nullptr;
+    rc = hip_check(hipMalloc((void **)&p, (size_t)nb * sizeof(lbc::FrameTab)), "lbc frame tables");
+    if (rc) return rc;
+    // built on a stream of its own and waited for: the tables are then
+    // ready for every stream and thread that uses them
+    hipStream_t s = nullptr;
+    rc = hip_check(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "lbc frame tables: stream");
+    if (!rc) {
+        hipLaunchKernelGGL(lbc::frame_setup_kernel, dim3(nb), dim3(lbc::NT), 0, s, nx, ny, K, p);
+        rc = hip_check(hipGetLastError(), "lbc frame setup kernel");
+        if (!rc) rc = hip_check(hipStreamSynchronize(s), "lbc frame setup kernel");
+        hipStreamDestroy(s);
+    }
+    if (rc) {
+        hipFree(p);
+        return rc;
+    }
+    cache.push_back({dev, nx, ny, K, p});
+    *tab = p;
+    return 0;
+}
 
 int lbc_frame_blocks(int nx, int ny, int K)
 {
@@ -343,7 +589,7 @@ int launch_lbc_frame(const LbcFrameParams &p, hipStream_t s)
         if (rc) return rc;
         attr = true;
     }
-    hipLaunchKernelGGL(lbc::frame_kernel, dim3(lbc_frame_blocks(p.nx, p.ny, p.K)), dim3(256),
+    hipLaunchKernelGGL(lbc::frame_kernel, dim3(lbc_frame_blocks(p.nx, p.ny, p.K)), dim3(lbc::NT),
                        sizeof(lbc::FrameSmem), s, p);
     return hip_check(hipGetLastError(), "lbc frame kernel");
 }

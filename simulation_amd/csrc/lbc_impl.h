@@ -45,10 +45,40 @@ constexpr int P_W1 = 0, P_B1 = P_W1 + HID * IN, P_W2 = P_B1 + HID, P_B2 = P_W2 +
               P_W3 = P_B2 + HID, P_B3 = P_W3 + OUT * HID, P_L = P_B3 + OUT;
 // bf16 operand buffer (uint16 elements): W1 [64][128] (k >= 108 zero),
 // W2 [64][64], W3 [16][64] (rows >= 3 zero), W2^T [64][64], W3^T [64][32]
-// (k >= 3 zero); L is read in FP32 from the parameters
+// (k >= 3 zero), then the per-lane operands of the transposed network of the
+// time-stepping kernel (frm_slot); L is read in FP32 from the parameters
 constexpr int K1 = 128;
+constexpr int FRM_OPS = 26;     // per lane: W1 16 (tile mt, K step ks), W2 8, W3 2
 constexpr int W_W1 = 0, W_W2 = W_W1 + HID * K1, W_W3 = W_W2 + HID * HID,
-              W_W2T = W_W3 + 16 * HID, W_W3T = W_W2T + HID * HID, WBUF = W_W3T + HID * 32;
+              W_W2T = W_W3 + 16 * HID, W_W3T = W_W2T + HID * HID, W_FRM = W_W3T + HID * 32,
+              WBUF = W_FRM + 64 * FRM_OPS * 8;
+
+// Transposed network (positions as MFMA columns, hidden units as rows): lane
+// (c = lane & 15, g = lane >> 4) holds, as A operands,
+//   op mt*4 + ks      : W1[16 mt + c][27 g + 8 ks + v]        (8 ks + v < 27)
+//   op 16 + mt*2 + ks : W2[16 mt + c][u(ks, g, v)]
+//   op 24 + ks        : W3[c][u(ks, g, v)]                    (c < 3)
+// v = 0..7, u(ks, g, v) = (2 ks + v / 4) 16 + 4 g + v % 4: the hidden unit a
+// layer's accumulator leaves in lane (c, g) at K index 32 ks + 8 g + v of
+// the next layer.  frm_slot maps a flat parameter index to its element of
+// W_FRM (or -1: a bias, L, or W1's padding).
+__host__ __device__ inline int frm_slot(int q)
+{
+    int lane, op, v;
+    if (q < P_B1) {
+        const int j = q / IN, k = q % IN, g = k / NF, i = k % NF;
+        lane = (j % 16) + 16 * g; op = (j / 16) * 4 + i / 8; v = i % 8;
+    } else if (q >= P_W2 && q < P_B2) {
+        const int m = (q - P_W2) / HID, u = (q - P_W2) % HID, b = u / 16, r = u % 16;
+        lane = (m % 16) + 16 * (r / 4); op = 16 + (m / 16) * 2 + b / 2; v = (b % 2) * 4 + r % 4;
+    } else if (q >= P_W3 && q < P_B3) {
+        const int o = (q - P_W3) / HID, u = (q - P_W3) % HID, b = u / 16, r = u % 16;
+        lane = o + 16 * (r / 4); op = 24 + b / 2; v = (b % 2) * 4 + r % 4;
+    } else {
+        return -1;
+    }
+    return W_FRM + (lane * FRM_OPS + op) * 8 + v;
+}
 
 // LDS row strides (bf16 elements; multiples of 8 keep 16-byte alignment and
 // spread 16 rows over distinct bank groups)

@@ -229,3 +229,46 @@ def test_training_reduces_loss(gpu):
     print(f"[lbc train] loss after the linear fit {l0:.3e} -> {l1:.3e}", flush=True)
     assert l1 < 0.1 * l0
     assert np.array_equal(net.numpy()[L.P_L:], fitted)
+
+
+def _frame_operands(p):
+    """Per-lane A operands of the time-stepping kernel's transposed network,
+    restated from their definition: lane (c, g), op mt*4+ks -> W1[16mt+c]
+    [27g + 8ks + v] (zero past 27), op 16+2mt+ks -> W2[16mt+c][u], op 24+ks ->
+    W3[c][u] (zero for c >= 3), u = (2ks + v//4)*16 + 4g + v%4."""
+    P = L.unpack(p)
+    out = np.zeros((64, 26, 8), np.float32)
+    for lane in range(64):
+        c, g = lane % 16, lane // 16
+        for v in range(8):
+            for ks in range(4):
+                i = 8 * ks + v
+                for mt in range(4):
+                    if i < L.NFEAT:
+                        out[lane, mt * 4 + ks, v] = P["W1"][16 * mt + c, 27 * g + i]
+            for ks in range(2):
+                u = (2 * ks + v // 4) * 16 + 4 * g + v % 4
+                for mt in range(4):
+                    out[lane, 16 + 2 * mt + ks, v] = P["W2"][16 * mt + c, u]
+                if c < L.OUT:
+                    out[lane, 24 + ks, v] = P["W3"][c, u]
+    return L.bf16(out).reshape(-1)
+
+
+def test_frame_operand_layout(gpu):
+    """The bf16 buffer's per-lane operands: written by lbc_pack, kept current
+    by lbc_adam (the whole buffer equals a fresh pack after a step)."""
+    from simulation_amd import learned
+    p, X, T = _problem(9, 256)
+    net = learned.Net(p, gpu)
+    w = net.wbuf.cpu().numpy().view(np.uint16)
+    nfrm = 64 * 26 * 8
+    assert w.size == net.wbuf.numel() and w.size >= nfrm
+    got = (w[-nfrm:].astype(np.uint32) << 16).view(np.float32)
+    assert np.array_equal(got, _frame_operands(p))
+    src, offs, ts = _rows(X, gpu)
+    net.train_step(src, offs, ts, 1e-2, tgt=torch.from_numpy(T).to(gpu), nblocks=3)
+    fresh = learned.Net(net.numpy(), gpu)
+    assert torch.equal(net.wbuf, fresh.wbuf)
+    got = (net.wbuf.cpu().numpy().view(np.uint16)[-nfrm:].astype(np.uint32) << 16).view(np.float32)
+    assert np.array_equal(got, _frame_operands(net.numpy()))
