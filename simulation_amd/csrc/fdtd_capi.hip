@@ -152,10 +152,13 @@ int aux_for_current_device(Aux **out)
     if (!a.s) {
         rc = hip_check(hipStreamCreateWithFlags(&a.s, hipStreamNonBlocking), "aux stream");
         if (!rc) rc = hip_check(hipStreamCreateWithFlags(&a.s2, hipStreamNonBlocking), "aux stream");
-        if (!rc) rc = hip_check(hipEventCreateWithFlags(&a.fork, hipEventDisableTiming), "aux event");
-        if (!rc) rc = hip_check(hipEventCreateWithFlags(&a.join, hipEventDisableTiming), "aux event");
-        if (!rc) rc = hip_check(hipEventCreateWithFlags(&a.join2, hipEventDisableTiming), "aux event");
-        if (!rc) rc = hip_check(hipEventCreateWithFlags(&a.join0, hipEventDisableTiming), "aux event");
+        // events order work between streams of this device only: a
+        // device-scope release is enough (no system-scope fence)
+        const unsigned ef = hipEventDisableTiming | hipEventReleaseToDevice;
+        if (!rc) rc = hip_check(hipEventCreateWithFlags(&a.fork, ef), "aux event");
+        if (!rc) rc = hip_check(hipEventCreateWithFlags(&a.join, ef), "aux event");
+        if (!rc) rc = hip_check(hipEventCreateWithFlags(&a.join2, ef), "aux event");
+        if (!rc) rc = hip_check(hipEventCreateWithFlags(&a.join0, ef), "aux event");
         if (rc) return rc;
     }
     *out = &a;

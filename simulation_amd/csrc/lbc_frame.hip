@@ -57,7 +57,7 @@ constexpr int NT = 512;                       // threads per workgroup (8 waves)
 // (2K+6) x (CH + 3K + 2) (30 x 57 and 22 x 66 at K = 8)
 constexpr int RMAX_C = (3 * KMAX + 6) * (CH + 2 * KMAX + 1);
 constexpr int RMAX_E = (2 * KMAX + 6) * (CH + 3 * KMAX + 2);
-constexpr int RMAX = RMAX_C > RMAX_E ? RMAX_C : RMAX_E;
+constexpr int RMAX = ((RMAX_C > RMAX_E ? RMAX_C : RMAX_E) + 3) / 4 * 4;   // keeps the ring 16-byte aligned
 constexpr int PMAX = 128;                     // positions per workgroup
 
 struct FrameSmem {
@@ -337,14 +337,16 @@ __global__ void __launch_bounds__(NT) frame_kernel(LbcFrameParams P)
         S.nb[k] = P.nbz ? P.nbz[o] : 0.0f;
         S.ez[k] = 0.0f;
     }
-    __syncthreads();
-    // history ring: sample of level l (l = -3..K) in slot l & 3
-    for (int k = tid; k < np * HIST * NS; k += NT) {
-        const int p = k / (HIST * NS), a = (k / NS) % HIST, i = k % NS;
-        int e = S.pos[p] >> 24, t = S.pos[p] & 0xFFFFFF;
-        S.ring[p][(-a) & 3][i] = P.hist_in[((long)gidx(e, t, nx, ny) * HIST + a) * NS + i];
+    // history ring: the sample of level l (l = -3..K) in slot (-l) & 3, so
+    // the history's ages 0..3 (levels 0, -1, -2, -3) land in slots 0..3: one
+    // contiguous copy of 4 x NS floats per position (positions read from the
+    // table in global memory: no barrier between the loads)
+    for (int k = tid; k < np * (HIST * NS / 4); k += NT) {
+        const int p = k / (HIST * NS / 4), j = k % (HIST * NS / 4);
+        const int ps = T.pos[p], e = ps >> 24, t = ps & 0xFFFFFF;
+        const float4 v = reinterpret_cast<const float4 *>(P.hist_in + (long)gidx(e, t, nx, ny) * HIST * NS)[j];
+        reinterpret_cast<float4 *>(&S.ring[p][0][0])[j] = v;
     }
-    __syncthreads();
 
     for (int k = tid; k < OUT * IN; k += NT) S.L[k / IN][k % IN] = P.params[P_L + k];
     for (int k = tid; k < HID; k += NT) {
@@ -368,7 +370,7 @@ __global__ void __launch_bounds__(NT) frame_kernel(LbcFrameParams P)
             const int q = tb + (c & 7);
             float x[NF];
             if (q < na) {
-                const float *src = S.ring[q][(lev - 1 - g) & 3];
+                const float *src = S.ring[q][(g + 1 - lev) & 3];   // level lev - 1 - g
 #pragma unroll
                 for (int i = 0; i < NF; ++i) x[i] = src[i];
             } else {
@@ -448,7 +450,7 @@ __global__ void __launch_bounds__(NT) frame_kernel(LbcFrameParams P)
                 const float other = __shfl_xor(y[v], 8);
                 if (g == 0 && c < 8 && q < na) {
                     const float yo = 0.5f * (y[v] - other);
-                    const float b = base_of(v, S.ring[q][(lev - 2) & 3]);
+                    const float b = base_of(v, S.ring[q][(2 - lev) & 3]);
                     S.pred[q][v] = mx > S_MIN ? b + mx * (lin[v] + yo) : b;
                 }
             }
@@ -503,7 +505,7 @@ __global__ void __launch_bounds__(NT) frame_kernel(LbcFrameParams P)
             const unsigned f = (sv >> 12) & 3u, cell = sv & 0xFFFu;
             float v = f == 0 ? S.ez[cell] : (f == 1 ? S.hx[cell] : (f == 2 ? S.hy[cell] : 0.0f));
             if (sv & 0x8000u) v = -v;
-            S.ring[p][lev & 3][i] = v;
+            S.ring[p][(-lev) & 3][i] = v;
         }
         __syncthreads();
     }
@@ -523,7 +525,7 @@ __global__ void __launch_bounds__(NT) frame_kernel(LbcFrameParams P)
     for (int k = tid; k < np * HIST * NS; k += NT) {
         const int p = k / (HIST * NS), a = (k / NS) % HIST, i = k % NS;
         if (S.own[p] < 0) continue;
-        P.hist_out[((long)S.own[p] * HIST + a) * NS + i] = S.ring[p][(K - a) & 3][i];
+        P.hist_out[((long)S.own[p] * HIST + a) * NS + i] = S.ring[p][(a - K) & 3][i];
     }
 }
 
