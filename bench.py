@@ -277,7 +277,7 @@ def bench_learned(a):
     n = a.size if a.size != 4096 else 2048
     K = a.steps if a.steps != 2000 else 1600
     W = a.warmup
-    wpath = a.weights or os.path.join(ROOT, "simulation_amd", "data", "lbc_weights.npz")
+    wpath = a.weights or LB.WEIGHTS
     if os.path.exists(wpath):
         net, wname = LB.load(wpath, dev), os.path.relpath(wpath, ROOT)
     else:
@@ -326,6 +326,11 @@ def bench_learned(a):
                      "avg_launch_ms": round(launch_ms, 4), "alg_bytes_per_launch": int(bpl)},
         "cpu_baseline": None,
     }
+    # reflected energy of the same two forms (FP64 accounting, reflection()
+    # case: point pulse and a lossy cylinder, 1.6 n + 400 steps)
+    rr = LB.reflection(net, n, methods=("learned", "pml32"), device=dev)
+    rec["config"]["reflection"] = {m: float(f"{v:.3e}") for m, v in rr["ratio"].items()}
+    rec["config"]["reflection_steps"] = rr["steps"]
     print(json.dumps(rec), flush=True)
 
 

@@ -391,14 +391,18 @@ int lbc_train_grad(int n, const float *src, const long long *offs, long long tst
  * the network applied every step (the blocked launch is exact deeper in).
  * hist, hist2: [lbc_hist_elems(nx, ny)] floats each (npos x 4 x 30), the last four boundary
  * samples of every position (zero for fields at rest); the run leaves them
- * current in hist.  nsteps >= 2 (an even number of launches).               */
+ * current in hist.  work: lbc_work_bytes(nx, ny) bytes of device memory for
+ * the frame kernel's geometry tables (rewritten by every call; one run at a
+ * time per work buffer).  nsteps >= 2 (an even number of launches).        */
 #define FDTD_LBC_MARGIN 32
 typedef struct fd2d_lbc_desc {
     const float *params;        /* lbc_param_count() FP32 parameters         */
     const uint16_t *wbuf;       /* bf16 operands (lbc_pack)                  */
     float *hist, *hist2;        /* boundary-sample history, ping-pong pair   */
+    void *work;                 /* lbc_work_bytes(nx, ny) bytes               */
 } fd2d_lbc_desc;
 size_t lbc_hist_elems(int nx, int ny);
+size_t lbc_work_bytes(int nx, int ny);
 int fd2d_lbc_margin(void);
 int fd2d_lbc_run(const fd2d_tm_desc *d, const fd2d_lbc_desc *b, int t0, int nsteps,
                  void *stream);
@@ -412,19 +416,22 @@ int lbc_adam(int nblocks, const float *part, float *params, float *m, float *v, 
              int first, int count, void *stream);
 
 /* Ridge least squares for the linear path.  lbc_gram accumulates, over the
- * rows (as lbc_train_grad, targets from the next sample) with s > 0, the FP64
- * sums G = sum xn xn^T (108 x 108) and R = sum xn t^T (108 x 3) of the
+ * rows (as lbc_train_grad: targets tgt[r][3], or from the next sample when
+ * tgt is NULL) with s > 0, the FP64
+ * sums G = sum w xn xn^T (108 x 108) and R = sum w xn t^T (108 x 3) of the
  * normalised inputs xn (bf16-rounded when round_bf16) and residual targets
- * t = (T - base) / s, plus the row count: nblocks workgroups write
+ * t = (T - base) / s with row weights w = s^wpow (0 <= wpow <= 4; 0: all
+ * rows alike, 2: least squares on the raw values), plus the weight sum
+ * (the count for wpow = 0): nblocks workgroups write
  * part[block][lbc_gram_elems()] (row-major [G | R], then the count).
  * lbc_fit_linear reduces the partials in a fixed order into gram, then (if
- * params is not NULL) solves (G + rho * count * I) L^T = R by Cholesky in
+ * params is not NULL) solves (G + rho * wsum * I) L^T = R by Cholesky in
  * FP64 and writes L into params; status[0] = 0, or 1 + k if pivot k is not
  * positive (L then unchanged).  Call lbc_pack afterwards only if the bf16
  * buffer must follow (L is read in FP32).                                   */
 int lbc_gram_elems(void);
-int lbc_gram(int n, const float *src, const long long *offs, long long tstride, int round_bf16,
-             int nblocks, double *part, void *stream);
+int lbc_gram(int n, const float *src, const long long *offs, long long tstride, const float *tgt,
+             int round_bf16, float wpow, int nblocks, double *part, void *stream);
 int lbc_fit_linear(int nblocks, const double *part, double *gram, double rho, float *params,
                    int *status, void *stream);
 

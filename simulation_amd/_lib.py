@@ -82,7 +82,7 @@ class Fd2dGrid(C.Structure):
 class Fd2dLbcDesc(C.Structure):
     """fd2d_lbc_desc: network and boundary-sample history (include/fdtd.h)."""
     _fields_ = [("params", C.c_void_p), ("wbuf", C.c_void_p), ("hist", C.c_void_p),
-                ("hist2", C.c_void_p)]
+                ("hist2", C.c_void_p), ("work", C.c_void_p)]
 
 
 class Fd2dBatchDesc(C.Structure):
@@ -137,6 +137,7 @@ SIGNATURES = {
     "fdtd_graph_launch": (_I, [_P, _P]),
     "fdtd_graph_destroy": (_I, [_P]),
     "lbc_hist_elems": (C.c_size_t, [_I, _I]),
+    "lbc_work_bytes": (C.c_size_t, [_I, _I]),
     "fd2d_lbc_margin": (_I, []),
     "fd2d_lbc_run": (_I, [C.POINTER(Fd2dTmDesc), C.POINTER(Fd2dLbcDesc), _I, _I, _P]),
     "lbc_param_count": (_I, []),
@@ -149,7 +150,7 @@ SIGNATURES = {
                             _P]),
     "lbc_adam": (_I, [_I, _P, _P, _P, _P, _I, _F, _F, _F, _F, _P, _P, _I, _I, _P]),
     "lbc_gram_elems": (_I, []),
-    "lbc_gram": (_I, [_I, _P, _P, C.c_longlong, _I, _I, _P, _P]),
+    "lbc_gram": (_I, [_I, _P, _P, C.c_longlong, _P, _I, _F, _I, _P, _P]),
     "lbc_fit_linear": (_I, [_I, _P, _P, C.c_double, _P, _P, _P]),
 }
 

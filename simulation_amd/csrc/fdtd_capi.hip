@@ -613,6 +613,12 @@ size_t lbc_hist_elems(int nx, int ny)
     return (size_t)lbc::npos(nx, ny) * lbc::HIST * lbc::NS;
 }
 
+size_t lbc_work_bytes(int nx, int ny)
+{
+    if (nx < 64 || ny < 64) return 0;
+    return lbc_frame_table_offset(nx, ny, 9);
+}
+
 int fd2d_lbc_margin(void) { return FDTD_LBC_MARGIN; }
 
 int fd2d_lbc_run(const fd2d_tm_desc *d, const fd2d_lbc_desc *b, int t0, int nsteps, void *stream)
@@ -621,8 +627,8 @@ int fd2d_lbc_run(const fd2d_tm_desc *d, const fd2d_lbc_desc *b, int t0, int nste
     int rc = tm_validate(d, t0, nsteps);
     if (rc) return rc;
     if (!pml_none(&d->pml)) return fail("fd2d_lbc_run: the learned boundary replaces the pml (pass none)");
-    if (!b || !b->params || !b->wbuf || !b->hist || !b->hist2)
-        return fail("fd2d_lbc_run: params, wbuf, hist and hist2 are required");
+    if (!b || !b->params || !b->wbuf || !b->hist || !b->hist2 || !b->work)
+        return fail("fd2d_lbc_run: params, wbuf, hist, hist2 and work are required");
     if (d->nx < 64 || d->ny < 64) return fail("fd2d_lbc_run: grid %dx%d below 64x64", d->nx, d->ny);
     const int M = FDTD_LBC_MARGIN;
     for (int k = 0; k < d->nsrc; ++k)
@@ -665,12 +671,12 @@ int fd2d_lbc_run(const fd2d_tm_desc *d, const fd2d_lbc_desc *b, int t0, int nste
     Aux *ax = nullptr;
     rc = aux_for_current_device(&ax);
     if (rc) return rc;
-    // geometry tables of the (at most two) launch depths
+    // geometry tables of the (at most two) launch depths, built into the
+    // caller's work buffer on the frame stream, ahead of the frame kernels
     const int kq = nsteps / L, kr = nsteps % L;
-    const void *tabq = nullptr, *tabr = nullptr;
-    rc = lbc_frame_tables(d->nx, d->ny, kq, &tabq);
-    if (!rc && kr) rc = lbc_frame_tables(d->nx, d->ny, kq + 1, &tabr);
-    if (rc) return rc;
+    char *wk = static_cast<char *>(b->work);
+    void *tabq = wk + lbc_frame_table_offset(d->nx, d->ny, kq);
+    void *tabr = kr ? wk + lbc_frame_table_offset(d->nx, d->ny, kq + 1) : nullptr;
     // three streams: interior part P0 on the caller's, boundary part P1 on
     // ax->s, frame F on ax->s2.  Launch l:
     //   P0(l) after P0(l-1), P1(l-1)   (reads and overwrites what they used)
@@ -681,6 +687,8 @@ int fd2d_lbc_run(const fd2d_tm_desc *d, const fd2d_lbc_desc *b, int t0, int nste
     rc = hip_check(hipEventRecord(ax->fork, s0), "fork event");
     if (!rc) rc = hip_check(hipStreamWaitEvent(ax->s, ax->fork, 0), "fork wait");
     if (!rc) rc = hip_check(hipStreamWaitEvent(ax->s2, ax->fork, 0), "fork wait");
+    if (!rc) rc = launch_lbc_frame_setup(d->nx, d->ny, kq, tabq, ax->s2);
+    if (!rc && kr) rc = launch_lbc_frame_setup(d->nx, d->ny, kq + 1, tabr, ax->s2);
     if (rc) return rc;
     int done = 0;
     for (int l = 0; l < L; ++l) {

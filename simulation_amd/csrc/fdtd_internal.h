@@ -80,16 +80,18 @@ struct LbcFrameParams {
     float *hist_out;
     const uint16_t *w;                          // network: bf16 operands
     const float *params;                        //          FP32 parameters (biases, L)
-    const void *tab;                            // geometry tables (lbc_frame_tables)
+    const void *tab;                            // geometry tables (launch_lbc_frame_setup)
 };
 int launch_fd2d_energy_diff(int wn, int wm, int nya, int ia0, int ja0, const float *eza,
                             const float *hxa, const float *hya, int nyb, int ib0, int jb0,
                             const float *ezb, const float *hxb, const float *hyb, double *work,
                             hipStream_t s);
 size_t lbc_frame_smem();
-// Geometry tables of the frame workgroups for an nx x ny grid and depth K on
-// the current device: built on first use (and kept until the process ends).
-int lbc_frame_tables(int nx, int ny, int K, const void **tab);
+// Geometry tables of the frame workgroups (per depth K = 1..8, one after the
+// other in the caller's work buffer; built by launch_lbc_frame_setup).
+size_t lbc_frame_table_bytes(int nx, int ny, int K);
+size_t lbc_frame_table_offset(int nx, int ny, int K);
+int launch_lbc_frame_setup(int nx, int ny, int K, void *tab, hipStream_t s);
 int lbc_frame_blocks(int nx, int ny, int K);
 int launch_lbc_frame(const LbcFrameParams &p, hipStream_t s);
 

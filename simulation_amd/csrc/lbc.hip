@@ -380,10 +380,11 @@ constexpr int GSZ = IN * GC + 1;        // + row count
 constexpr int GPER = (IN * GC + 255) / 256;
 
 __global__ void __launch_bounds__(256) gram_kernel(int n, const float *src, const long long *offs,
-                                                   long long tstride, int round_bf16,
-                                                   double *part)
+                                                   long long tstride, const float *tgt,
+                                                   int round_bf16, float wpow, double *part)
 {
     __shared__ float X[32][GC + 1];
+    __shared__ float RW[32];            // sqrt of the row weight s^wpow
     const int tid = threadIdx.x;
     double acc[GPER];
     int ia[GPER], ib[GPER];
@@ -424,24 +425,27 @@ __global__ void __launch_bounds__(256) gram_kernel(int n, const float *src, cons
                 X[lr][la * NF + i] = x;
             }
             if (la == 0) {
+                RW[lr] = use ? (wpow == 0.0f ? 1.0f : powf(mx, 0.5f * wpow)) : 0.0f;
 #pragma unroll
                 for (int d = 0; d < OUT; ++d) {
                     float t = 0.0f;
                     if (use) {
                         const float *nxt = src + offs[row] + tstride;
-                        const float T = nxt[d == 2 ? S_EZ2 : (d == 1 ? S_EZ1 : S_EZ0)];
+                        const float T = tgt ? tgt[(long)row * OUT + d]
+                                            : nxt[d == 2 ? S_EZ2 : (d == 1 ? S_EZ1 : S_EZ0)];
                         t = (T - base_of(d, src + offs[row] - tstride)) * inv;
                     }
                     X[lr][IN + d] = t;
                 }
-                if (use) cnt += 1.0;
+                if (use) cnt += (double)RW[lr] * (double)RW[lr];
             }
         }
         __syncthreads();
         for (int r = 0; r < 32; ++r) {
+            const double wr = (double)RW[r] * (double)RW[r];
 #pragma unroll
             for (int j = 0; j < GPER; ++j)
-                acc[j] += (double)X[r][ia[j]] * (double)X[r][ib[j]];
+                acc[j] += wr * ((double)X[r][ia[j]] * (double)X[r][ib[j]]);
         }
         __syncthreads();
     }
@@ -631,13 +635,14 @@ int lbc_adam(int nblocks, const float *part, float *params, float *m, float *v, 
 
 int lbc_gram_elems(void) { return GSZ; }
 
-int lbc_gram(int n, const float *src, const long long *offs, long long tstride, int round_bf16,
-             int nblocks, double *part, void *stream)
+int lbc_gram(int n, const float *src, const long long *offs, long long tstride, const float *tgt,
+             int round_bf16, float wpow, int nblocks, double *part, void *stream)
 {
     if (n <= 0 || nblocks <= 0) return fail("lbc_gram: n=%d nblocks=%d", n, nblocks);
     if (!src || !offs || !part) return fail("lbc_gram: null pointer");
+    if (!(wpow >= 0.0f && wpow <= 4.0f)) return fail("lbc_gram: wpow=%g (0..4)", wpow);
     hipLaunchKernelGGL(gram_kernel, dim3(nblocks), dim3(256), 0, S(stream), n, src, offs, tstride,
-                       round_bf16, part);
+                       tgt, round_bf16, wpow, part);
     return hip_check(hipGetLastError(), "lbc_gram");
 }
 

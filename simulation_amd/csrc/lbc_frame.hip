@@ -40,9 +40,6 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
-#include <mutex>
-#include <vector>
-
 #include "fdtd_internal.h"
 #include "lbc_impl.h"
 
@@ -534,44 +531,23 @@ __global__ void __launch_bounds__(NT) frame_kernel(LbcFrameParams P)
 
 size_t lbc_frame_smem() { return sizeof(lbc::FrameSmem); }
 
-int lbc_frame_tables(int nx, int ny, int K, const void **tab)
+size_t lbc_frame_table_bytes(int nx, int ny, int K)
 {
-    struct Entry {
-        int dev, nx, ny, K;
-        lbc::FrameTab *p;
-    };
-    static std::mutex mu;
-    static std::vector<Entry> cache;
-    int dev = 0;
-    int rc = hip_check(hipGetDevice(&dev), "hipGetDevice");
-    if (rc) return rc;
-    std::lock_guard<std::mutex> lk(mu);
-    for (const Entry &e : cache)
-        if (e.dev == dev && e.nx == nx && e.ny == ny && e.K == K) {
-            *tab = e.p;
-            return 0;
-        }
-    const int nb = lbc_frame_blocks(nx, ny, K);
-    lbc::FrameTab *p = nullptr;
-    rc = hip_check(hipMalloc((void **)&p, (size_t)nb * sizeof(lbc::FrameTab)), "lbc frame tables");
-    if (rc) return rc;
-    // built on a stream of its own and waited for: the tables are then
-    // ready for every stream and thread that uses them
-    hipStream_t s = nullptr;
-    rc = hip_check(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "lbc frame tables: stream");
-    if (!rc) {
-        hipLaunchKernelGGL(lbc::frame_setup_kernel, dim3(nb), dim3(lbc::NT), 0, s, nx, ny, K, p);
-        rc = hip_check(hipGetLastError(), "lbc frame setup kernel");
-        if (!rc) rc = hip_check(hipStreamSynchronize(s), "lbc frame setup kernel");
-        hipStreamDestroy(s);
-    }
-    if (rc) {
-        hipFree(p);
-        return rc;
-    }
-    cache.push_back({dev, nx, ny, K, p});
-    *tab = p;
-    return 0;
+    return (size_t)lbc_frame_blocks(nx, ny, K) * sizeof(lbc::FrameTab);
+}
+
+size_t lbc_frame_table_offset(int nx, int ny, int K)
+{
+    size_t o = 0;
+    for (int k = 1; k < K; ++k) o += lbc_frame_table_bytes(nx, ny, k);
+    return o;
+}
+
+int launch_lbc_frame_setup(int nx, int ny, int K, void *tab, hipStream_t s)
+{
+    hipLaunchKernelGGL(lbc::frame_setup_kernel, dim3(lbc_frame_blocks(nx, ny, K)), dim3(lbc::NT), 0,
+                       s, nx, ny, K, reinterpret_cast<lbc::FrameTab *>(tab));
+    return hip_check(hipGetLastError(), "lbc frame setup kernel");
 }
 
 int lbc_frame_blocks(int nx, int ny, int K)

@@ -22,6 +22,7 @@ from __future__ import annotations
 
 import ctypes as C
 import math
+import os
 from dataclasses import dataclass
 
 import numpy as np
@@ -95,15 +96,19 @@ class Net:
         self.step = 0
         self.pack()
 
-    def gram(self, src, offs, tstride, nblocks: int = 0, round_bf16: bool = False):
+    def gram(self, src, offs, tstride, nblocks: int = 0, round_bf16: bool = False, tgt=None,
+             wpow: float = 0.0):
         """Per-block normal-equation sums of the linear path over the rows
-        (lbc_gram): a (nblocks, lbc_gram_elems()) FP64 tensor for fit_linear."""
+        (lbc_gram; targets tgt (n, 3) or the next samples; row weights
+        s^wpow): a (nblocks, lbc_gram_elems()) FP64 tensor for fit_linear."""
         n = offs.numel()
         nb = nblocks or max(1, min(2048, (n + 2047) // 2048))
         L = _lib.lib()
         part = torch.empty((nb, L.lbc_gram_elems()), dtype=torch.float64, device=self.device)
-        call("lbc_gram", n, ptr(src), offs.data_ptr(), int(tstride), int(round_bf16), nb,
-             ptr(part, torch.float64), stream())
+        call("lbc_gram", n, ptr(src), offs.data_ptr(), int(tstride),
+             None if tgt is None else ptr(tgt), int(round_bf16), float(wpow), nb,
+             ptr(part, torch.float64),
+             stream())
         return part
 
     def fit_linear(self, part: torch.Tensor, rho: float) -> torch.Tensor:
@@ -202,6 +207,8 @@ class LearnedTMSolver:
         dev = self.tm.device
         self.hist = torch.zeros(n, dtype=torch.float32, device=dev)
         self.hist2 = torch.zeros(n, dtype=torch.float32, device=dev)
+        wb = _lib.lib().lbc_work_bytes(nx, ny)
+        self.work = torch.empty(max(1, wb), dtype=torch.uint8, device=dev)
 
     @property
     def t(self) -> int:
@@ -209,7 +216,8 @@ class LearnedTMSolver:
 
     def desc(self):
         return _lib.Fd2dLbcDesc(self.net.params.data_ptr(), self.net.wbuf.data_ptr(),
-                                self.hist.data_ptr(), self.hist2.data_ptr())
+                                self.hist.data_ptr(), self.hist2.data_ptr(),
+                                self.work.data_ptr())
 
     def run(self, nsteps: int) -> None:
         d = self.tm.desc()
@@ -249,16 +257,17 @@ class TrainConfig:
     it, a field that never radiates and whose far part depends on how the grid
     is closed -- not what an absorbing boundary is for.  The window's boundary
     samples are recorded every step."""
-    runs: int = 512
+    runs: int = 256
     window: int = 160
     npml: int = 64
     gap: int = 8
     steps: int = 520
     margin: int = 32
     seed: int = 0
-    fit_rows: int = 4_000_000     # rows (random subset) of the linear path's normal equations
-    rho: tuple = (1e-6, 1e-5, 3e-5, 1e-4, 3e-4, 1e-3)   # ridge candidates (x row count)
-    select_n: int = 256           # closed-loop validation window for rho and the hidden scale
+    fit_rows: int = 6_000_000     # rows (random subset) of the linear path's normal equations
+    wpow: tuple = (0.0, 1.0, 2.0)            # row-weight exponents s^wpow tried for the fit
+    rho: tuple = (1e-7, 3e-7, 1e-6, 1e-5)    # ridge candidates (x weight sum)
+    select_n: int = 2048          # closed-loop validation window for the fit and the hidden scale
     scales: tuple = (0.0, 0.25, 0.5, 1.0)   # hidden-layer output scales tried after training
     train_steps: int = 8000       # Adam steps of the hidden layers (linear path fixed)
     batch: int = 8192
@@ -347,8 +356,9 @@ def validation_reflection(net: Net, n: int, device=None) -> float:
 
 def train(cfg: TrainConfig, G: torch.Tensor, net: Net | None = None, log=None) -> tuple:
     """Three stages, all on the GPU:
-    1. the linear path by ridge least squares (lbc_gram / lbc_fit_linear on
-       a random subset of cfg.fit_rows rows), the ridge chosen among cfg.rho
+    1. the linear path by weighted ridge least squares (lbc_gram /
+       lbc_fit_linear on a random subset of cfg.fit_rows rows, row weights
+       s^wpow), the weight exponent and ridge chosen among cfg.wpow x cfg.rho
        by the closed-loop reflection of a validation case
        (validation_reflection at cfg.select_n) -- the one-step loss does not
        tell a stable boundary from an unstable one;
@@ -379,22 +389,31 @@ def train(cfg: TrainConfig, G: torch.Tensor, net: Net | None = None, log=None) -
             f"validation loss at start (the two-step boundary): {v0:.3e}")
     # 1. linear path
     fit = trn[torch.randperm(trn.numel(), device=dev, generator=gen)[:cfg.fit_rows]]
-    part = net.gram(G, fit, ts)
     best = None
-    for rho in cfg.rho:
-        net.fit_linear(part, rho)
-        vl, _ = net.grad(G, val, ts)
-        rr = validation_reflection(net, cfg.select_n, dev)
-        hist.append({"stage": "linear", "rho": rho, "validation": vl, "reflection": rr})
-        if log:
-            log(f"linear path rho {rho:.0e}: validation {vl:.3e} closed-loop reflection {rr:.2e}")
-        if math.isfinite(rr) and (best is None or rr < best[0]):
-            best = (rr, rho)
+    for wp in cfg.wpow:
+        part = net.gram(G, fit, ts, wpow=wp)
+        for rho in cfg.rho:
+            try:
+                net.fit_linear(part, rho)
+            except _lib.FdtdError as e:
+                hist.append({"stage": "linear", "wpow": wp, "rho": rho, "error": str(e)})
+                continue
+            vl, _ = net.grad(G, val, ts)
+            rr = validation_reflection(net, cfg.select_n, dev)
+            hist.append({"stage": "linear", "wpow": wp, "rho": rho, "validation": vl,
+                         "reflection": rr})
+            if log:
+                log(f"linear path wpow {wp} rho {rho:.0e}: validation {vl:.3e} "
+                    f"closed-loop reflection {rr:.2e}")
+            if math.isfinite(rr) and (best is None or rr < best[0]):
+                best = (rr, wp, rho, net.numpy()[P_L:].copy())
+        del part
     if best is None:
-        raise RuntimeError("no ridge value gave a finite closed-loop reflection")
-    rho = best[1]
-    net.fit_linear(part, rho)
-    del part
+        raise RuntimeError("no linear fit gave a finite closed-loop reflection")
+    _, wp, rho, lbest = best
+    p = net.numpy()
+    p[P_L:] = lbest
+    net.set_params(p)
     # 2. hidden layers
     nb = max(1, min(512, (cfg.batch + 255) // 256))
     gpart = torch.empty((nb, net.nparam), dtype=torch.float32, device=dev)
@@ -428,7 +447,7 @@ def train(cfg: TrainConfig, G: torch.Tensor, net: Net | None = None, log=None) -
     p = trained.copy()
     p[P_W3:P_L] *= best[1] if best else 0.0
     net.set_params(p)
-    hist.append({"stage": "chosen", "rho": rho, "scale": best[1] if best else 0.0})
+    hist.append({"stage": "chosen", "wpow": wp, "rho": rho, "scale": best[1] if best else 0.0})
     return net, hist
 
 
@@ -567,7 +586,12 @@ def save(net: Net, path: str, meta: dict | None = None) -> None:
     np.savez(path, params=net.numpy(), meta=np.array(json.dumps(meta or {})))
 
 
-def load(path: str, device=None) -> Net:
+# the network trained by `python -m simulation_amd.learned` (shipped with the
+# package: parameters and training record, no pickled objects)
+WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "lbc_weights.npz")
+
+
+def load(path: str = WEIGHTS, device=None) -> Net:
     d = np.load(path, allow_pickle=False)
     return Net(d["params"], device)
 
@@ -580,7 +604,7 @@ def main(argv=None) -> int:
     import json
     import time
     ap = argparse.ArgumentParser()
-    ap.add_argument("--runs", type=int, default=512)
+    ap.add_argument("--runs", type=int, default=256)
     ap.add_argument("--window", type=int, default=160)
     ap.add_argument("--steps", type=int, default=520)
     ap.add_argument("--train-steps", type=int, default=8000)
@@ -589,6 +613,8 @@ def main(argv=None) -> int:
     ap.add_argument("--noise", type=float, default=0.0)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--eval", type=int, nargs="*", default=[512])
+    ap.add_argument("--long", type=int, nargs=2, default=[0, 0], metavar=("N", "STEPS"),
+                    help="long-run check: an N x N window for STEPS steps")
     ap.add_argument("--out", default="")
     ap.add_argument("--report", default="")
     a = ap.parse_args(argv)
@@ -608,6 +634,11 @@ def main(argv=None) -> int:
     for n in a.eval:
         r = reflection(net, n, device=dev, log=log)
         rep["reflection"][str(n)] = {k: r[k] for k in ("n", "steps", "ratio", "final_ratio")}
+    if a.long[0]:
+        r = reflection(net, a.long[0], steps=a.long[1], chunk=500,
+                       methods=("learned", "abc", "pml32"), device=dev, log=log)
+        rep["long"] = {k: r[k] for k in ("n", "steps", "ratio", "final_ratio", "curve_t", "curve")}
+        log("long run final " + " ".join(f"{m}={v:.2e}" for m, v in r["final_ratio"].items()))
     if a.out:
         save(net, a.out, {"config": cfg.__dict__, "train": hist})
         log(f"saved {a.out}")

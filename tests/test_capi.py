@@ -129,3 +129,16 @@ def test_lbc_sizes_match_the_oracle_layout():
     for nx, ny in ((64, 64), (96, 112), (2048, 2048)):
         assert L.lbc_hist_elems(nx, ny) == len(LO.positions(nx, ny)) * LO.HIST * LO.NSAMP
     assert L.fd2d_lbc_margin() >= 3 * 8 + 6      # deepest frame region (3K+6 at K=8)
+
+
+def test_lbc_work_and_operand_buffer_sizes():
+    """The frame tables (one set per depth 1..8) need a work buffer that grows
+    with the perimeter; grids below 64 x 64 are rejected (size 0).  The bf16
+    buffer holds the row-major operands (W1 128-wide, W2, W3 16 rows, W2^T,
+    W3^T 32-wide) and 64 lanes x 26 per-lane operands of 8 values."""
+    from simulation_amd import _lib
+    L = _lib.lib()
+    assert L.lbc_work_bytes(32, 32) == 0
+    a, b = L.lbc_work_bytes(512, 512), L.lbc_work_bytes(2048, 2048)
+    assert 0 < a < b < 8 * a
+    assert L.lbc_wbuf_elems() == 64 * 128 + 64 * 64 + 16 * 64 + 64 * 64 + 64 * 32 + 64 * 26 * 8
