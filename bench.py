@@ -55,11 +55,13 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=("tm", "large", "batch", "learned"), default="tm",
+    ap.add_argument("--workload", choices=("tm", "large", "batch", "learned", "tfsf"),
+                    default="tm",
                     help="tm: the N x N single-cylinder configuration (default, N=4096); "
                          "large: 16384^2, 16 cylinders, 4 sources, 5000 steps; "
                          "batch: 256 uneven grids; learned: 2048^2 learned boundary against "
-                         "npml=32 PML")
+                         "npml=32 PML; tfsf: the plane-wave program (1024^2, npml=80, TF/SF "
+                         "plane wave, lossy cylinder, running DFT at 3 frequencies)")
     ap.add_argument("--weights", default="",
                     help="learned: network parameters (.npz from simulation_amd.learned); "
                          "default: the committed weights, else the zero network")
@@ -334,6 +336,131 @@ def bench_learned(a):
     print(json.dumps(rec), flush=True)
 
 
+def bench_tfsf(a):
+    """The reference's plane-wave program (fd2d/clang/test_3_4.c main, FP32):
+    1024^2, npml=80, cylinder of radius 1.5 m (eps_r 30, sigma 0.3), plane
+    wave through the TF/SF box, running DFT at 50, 300 and 700 MHz.  One
+    step = ezinct, D update, inctdz, E update, fourier, hxinct, H update,
+    incthx, incthy (fd2d_tfsf_run).  Algorithmic bytes per cell-update: the
+    lossy PML state (60 B) plus R+W of the 2 x 3 DFT planes (48 B)."""
+    import numpy as np
+    import torch
+    from simulation_amd import fd2d
+    dev = torch.device("cuda", 0)
+    n = a.size if a.size != 4096 else 1024
+    K = a.steps if a.steps != 2000 else 1000
+    W = a.warmup
+    npml, freq = 80, (50e6, 300e6, 700e6)
+    dt = fd2d.time_step()
+    rgrid = int(1.5 / 0.01 - 1)
+    pml = fd2d.new_pmlayer(n, n, dev)
+    fd2d.pmlparam(n, n, npml, pml)
+    md = fd2d.dielectric(n, n, npml, rgrid, dt, 30.0, 0.30, dev)
+    s = fd2d.PlaneWaveSolver(n, n, npml, pml, md, W + K, freq=freq, dt=dt, device=dev)
+    s.run(W)
+    torch.cuda.synchronize()
+    st = torch.cuda.current_stream()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record(st)
+    s.run(K)
+    e1.record(st)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    step_ms = e0.elapsed_time(e1) / K
+    b_alg = 60 + 16 * len(freq)
+    achieved = b_alg * n * n / (step_ms * 1e-3) / 1e9
+    cpu = None if a.no_cpu else cpu_baseline_tfsf(n, a.cpu_seconds, npml, rgrid, freq)
+    rec = {
+        "metric": METRIC, "value": round(n * n * K / wall / 1e6, 1), "unit": "Mcell-updates/s",
+        "n_gpus": 1, "steps": K, "warmup": W, "ms_per_step": round(wall * 1e3 / K, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (fields start at zero; Gaussian plane wave t0=20 spread=8)",
+        "config": {"workload": f"2-D TM plane wave {n}x{n} FP32, npml={npml} PML, TF/SF box, "
+                               f"cylinder r={rgrid} eps_r=30 sigma=0.3, DFT at 3 frequencies "
+                               f"(fd2d/clang/test_3_4.c)", "grid": [n, n], "npml": npml,
+                   "freq": list(freq)},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4), "traffic": None,
+                     "kernel": "per-step kernels of fd2d_tfsf_run (one step)",
+                     "avg_launch_ms": round(step_ms, 4), "alg_bytes_per_launch": b_alg * n * n},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(rec), flush=True)
+
+
+def cpu_baseline_tfsf(n: int, budget_s: float, npml: int, rgrid: int, freq) -> dict | None:
+    """The reference's own plane-wave step (fd2d/clang/test_3_4.c:223-233,
+    compiled under oracle/_ref) on the host cores for a bounded sample."""
+    import numpy as np
+    ref = os.path.join(ROOT, "oracle", "_ref", "libref_test_3_4.so")
+    ref64 = os.path.join(ROOT, "oracle", "_ref", "libref_fd2d_3_4.so")
+    if not (os.path.exists(ref) and os.path.exists(ref64)):
+        return None
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    L, L64 = C.CDLL(ref), C.CDLL(ref64)
+    F = C.POINTER(C.c_float)
+    P = lambda a: a.ctypes.data_as(F)  # noqa: E731
+    keys = ("fx1", "fx2", "fx3", "fy1", "fy2", "fy3", "gx2", "gx3", "gy2", "gy3")
+
+    class Pml(C.Structure):
+        _fields_ = [(k, F) for k in keys]
+
+    class Med(C.Structure):
+        _fields_ = [("naz", F), ("nbz", F)]
+
+    class Ft(C.Structure):
+        _fields_ = [(k, F) for k in ("r_pt", "i_pt", "r_in", "i_in")]
+
+    class Med64(C.Structure):
+        _fields_ = [("naz", C.POINTER(C.c_double)), ("nbz", C.POINTER(C.c_double))]
+    prof = {k: (np.zeros(n, np.float32) if k.endswith("1") else np.ones(n, np.float32))
+            for k in keys}
+    pml = Pml(*[P(prof[k]) for k in keys])
+    L.pmlparam(n, n, npml, C.byref(pml))
+    dt = np.float32(np.float32(0.01) / np.float32(6e8))
+    L64.dielectric.restype = Med64
+    m64 = L64.dielectric(n, n, npml, rgrid, C.c_double(float(dt)), C.c_double(30.0), C.c_double(0.3))
+    naz = np.ctypeslib.as_array(m64.naz, shape=(n, n)).astype(np.float32)
+    nbz = np.ctypeslib.as_array(m64.nbz, shape=(n, n)).astype(np.float32)
+    md = Med(P(naz), P(nbz))
+    f = {k: np.zeros((n, n), np.float32) for k in ("dz", "iz", "ez", "hx", "hy", "ihx", "ihy")}
+    nf = len(freq)
+    fr = np.asarray(freq, np.float32)
+    ftv = {"r_pt": np.zeros((nf, n, n), np.float32), "i_pt": np.zeros((nf, n, n), np.float32),
+           "r_in": np.zeros(nf, np.float32), "i_in": np.zeros(nf, np.float32)}
+    ft = Ft(*[P(ftv[k]) for k in ("r_pt", "i_pt", "r_in", "i_in")])
+    ezi, hxi, bc = np.zeros(n, np.float32), np.zeros(n, np.float32), np.zeros(4, np.float32)
+    I = C.c_int
+
+    def step(t):
+        L.ezinct(I(n), P(ezi), P(hxi), P(bc))
+        L.dfield(I(t), I(n), I(n), C.byref(pml), P(ezi), P(f["dz"]), P(f["hx"]), P(f["hy"]))
+        L.inctdz(I(n), I(n), I(npml), P(hxi), P(f["dz"]))
+        L.efield(I(n), I(n), C.byref(md), P(f["dz"]), P(f["iz"]), P(f["ez"]))
+        L.fourier(I(t), I(nf), I(n), I(n), C.c_float(dt), P(fr), P(ezi), P(f["ez"]), C.byref(ft))
+        L.hxinct(I(n), P(ezi), P(hxi))
+        L.hfield(I(n), I(n), C.byref(pml), P(f["ez"]), P(f["ihx"]), P(f["ihy"]), P(f["hx"]),
+                 P(f["hy"]))
+        L.incthx(I(n), I(n), I(npml), P(ezi), P(f["hx"]))
+        L.incthy(I(n), I(n), I(npml), P(ezi), P(f["hy"]))
+    step(1)
+    t0 = time.perf_counter()
+    steps = 0
+    while True:
+        step(steps + 2)
+        steps += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s:
+            break
+    return {"value": round(n * n * steps / el / 1e6, 2), "unit": "Mcell-updates/s",
+            "cores": threads, "kind": "reference",
+            "sample": f"{steps} steps of the {n}x{n} FP32 plane-wave step of "
+                      f"fd2d/clang/test_3_4.c (TF/SF, PML, lossy, DFT at {nf} frequencies), "
+                      f"{el:.1f} s, OMP_NUM_THREADS={threads}"}
+
+
 def main():
     a = parse()
     if a.record_traffic:
@@ -349,6 +476,8 @@ def main():
         return bench_batch(a)
     if a.workload == "learned":
         return bench_learned(a)
+    if a.workload == "tfsf":
+        return bench_tfsf(a)
     import torch
     from simulation_amd import fd2d
 

@@ -415,6 +415,64 @@ int lbc_adam(int nblocks, const float *part, float *params, float *m, float *v, 
              float lr, float beta1, float beta2, float eps, uint16_t *wbuf, float *grad_out,
              int first, int count, void *stream);
 
+/* ------------------------------------------ plane wave (TF/SF) and DFT
+ * The incident plane wave of fd2d/pycuda/fd2d_3_3.py:80-169 and
+ * fd2d_3_4.py:121-226: a 1-D grid along y (ezi, hxi: ny cells) with the
+ * two-step boundary (bc[4]) at both ends, and the total-field/scattered-
+ * field corrections on the box of rows and columns npml-1 .. n-npml.
+ * Serial semantics of fd2d/clang/test_3_4.c:38-147, FP32, bit-exact.       */
+#define FDTD_MAX_FREQ 16
+/* Replaces the device struct `ftrans` (fd2d/pycuda/fd2d_3_4.py:100-103):
+ * r_pt/i_pt are nf planes of nx*ny (plane n at n*nx*ny), r_in/i_in nf.     */
+typedef struct fdtd_ftrans {
+    float *r_pt, *i_pt;
+    float *r_in, *i_in;
+} fdtd_ftrans;
+
+/* `ezinct(ny, ezi, hxi, bc)` (fd2d_3_4.py:140-148): ezi update, then the
+ * boundary at both ends (one workgroup: every update precedes them).        */
+int fd2d_ezinct(int ny, float *ezi, const float *hxi, float *bc, void *stream);
+/* `dfield(t, nx, ny, pml, ezi, dz, hx, hy)` of fd2d_3_4.py:151-161: the D
+ * update (as fd2d_dfield) and the incident source ezi[inc_pos] = inc_val
+ * (the reference's ezi[3] = gaussian(t, 20, 8)).                            */
+int fd2d_dfield_inc(int t, int nx, int ny, const fdtd_pml *pml, float *ezi, float *dz,
+                    const float *hx, const float *hy, int inc_pos, float inc_val, void *stream);
+/* `inctdz(nx, ny, npml, hxi, dz)` (fd2d_3_4.py:164-170).                   */
+int fd2d_inctdz(int nx, int ny, int npml, const float *hxi, float *dz, void *stream);
+/* `hxinct(ny, ezi, hxi)` (fd2d_3_4.py:186-191).                             */
+int fd2d_hxinct(int ny, const float *ezi, float *hxi, void *stream);
+/* `incthx(nx, ny, npml, ezi, hx)` and `incthy(nx, ny, npml, ezi, hy)`
+ * (fd2d_3_4.py:211-226).                                                    */
+int fd2d_incthx(int nx, int ny, int npml, const float *ezi, float *hx, void *stream);
+int fd2d_incthy(int nx, int ny, int npml, const float *ezi, float *hy, void *stream);
+/* `fourier(t, nf, nx, ny, dt, freq, ezi, ez, ft)` (fd2d_3_4.py:121-136):
+ * freq is a host array of nf <= FDTD_MAX_FREQ frequencies; the twiddles
+ * cos/sin(2*pi*freq*dt*t) are evaluated on the host (double product,
+ * narrowed to float, cosf/sinf) as fd2d/clang/test_3_4.c:42-49 evaluates
+ * them; the incident probe is ezi[6], as in the reference.                  */
+int fd2d_fourier(int t, int nf, int nx, int ny, float dt, const float *freq, const float *ezi,
+                 const float *ez, const fdtd_ftrans *ft, void *stream);
+
+/* A whole plane-wave run: per step t = t0+1 .. t0+nsteps the sequence of
+ * fd2d/clang/test_3_4.c:223-233 (ezinct with ezi[inc_pos] = inc_wave[t-1];
+ * D update with d's PML and hard sources; inctdz; E update with d->md;
+ * fourier if nf > 0; hxinct; H update; incthx; incthy; d's probes).  Uses
+ * the caller's state set of d (no partner set needed).                     */
+typedef struct fd2d_tfsf_desc {
+    int npml;                   /* box rows/columns npml-1 .. n-npml (>= 2)  */
+    float *ezi, *hxi, *bc;      /* incident grid, ny cells each; bc[4]       */
+    int inc_pos;                /* incident source cell (3 in the reference) */
+    const float *inc_wave;      /* host: inc_wave[t-1] for step t            */
+    int inc_len;
+    int nf;                     /* DFT frequencies (0: no DFT)               */
+    const float *freq;          /* host, nf values                           */
+    float dt;
+    fdtd_ftrans ft;
+    int dft_probe;              /* incident probe cell (6 in the reference)  */
+} fd2d_tfsf_desc;
+int fd2d_tfsf_run(const fd2d_tm_desc *d, const fd2d_tfsf_desc *b, int t0, int nsteps,
+                  void *stream);
+
 /* Ridge least squares for the linear path.  lbc_gram accumulates, over the
  * rows (as lbc_train_grad: targets tgt[r][3], or from the next sample when
  * tgt is NULL) with s > 0, the FP64

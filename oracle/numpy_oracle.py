@@ -64,3 +64,102 @@ def tm_run(state: dict, nsteps: int, pml, naz, nbz, sources=(), src_wave=None, p
         for p, (i, j) in enumerate(probes):
             out[p, s] = state["ez"][i, j]
     return out
+
+
+# ------------------------------------------------ plane wave (TF/SF) and DFT
+# fd2d/clang/test_3_4.c:38-147 (FP32) and fd2d/clang/fd2d_3_4.c:38-147
+# (FP64): serial semantics of the reference's plane-wave program.
+
+def ezinct(ezi, hxi, bc):
+    """fd2d/clang/test_3_4.c:56-65: update, then the two-step boundary at
+    both ends."""
+    h = ezi.dtype.type(0.5)
+    ny = len(ezi)
+    ezi[1:ny] += h * (hxi[0:ny-1] - hxi[1:ny])
+    ezi[0] = bc[0]; bc[0] = bc[1]; bc[1] = ezi[1]
+    ezi[ny-1] = bc[3]; bc[3] = bc[2]; bc[2] = ezi[ny-2]
+
+
+def hxinct(ezi, hxi):
+    """fd2d/clang/test_3_4.c:105-111."""
+    h = ezi.dtype.type(0.5)
+    ny = len(ezi)
+    hxi[0:ny-1] += h * (ezi[0:ny-1] - ezi[1:ny])
+
+
+def inctdz(nx, ny, npml, hxi, dz):
+    """fd2d/clang/test_3_4.c:82-89: column npml-1 gains, column ny-npml
+    loses half the incident Hx beside the box (in this order)."""
+    h = dz.dtype.type(0.5)
+    r = slice(npml - 1, nx - npml + 1)
+    dz[r, npml - 1] += h * hxi[npml - 2]
+    dz[r, ny - npml] -= h * hxi[ny - npml]
+
+
+def incthx(nx, ny, npml, ezi, hx):
+    """fd2d/clang/test_3_4.c:129-136."""
+    h = hx.dtype.type(0.5)
+    r = slice(npml - 1, nx - npml + 1)
+    hx[r, npml - 2] += h * ezi[npml - 1]
+    hx[r, ny - npml] -= h * ezi[ny - npml]
+
+
+def incthy(nx, ny, npml, ezi, hy):
+    """fd2d/clang/test_3_4.c:139-146."""
+    h = hy.dtype.type(0.5)
+    c = slice(npml - 1, ny - npml + 1)
+    hy[npml - 2, c] -= h * ezi[c]
+    hy[nx - npml, c] += h * ezi[c]
+
+
+_libm = None
+
+
+def twiddle(t, freq, dt, dtype):
+    """cos, sin of 2*pi*freq*dt*t as the C programs evaluate them: the
+    product in double, then cos/sin in double (FP64) or narrowed to float and
+    cosf/sinf of the host libm (FP32; fd2d/clang/test_3_4.c:42-43)."""
+    import ctypes
+    import math
+    global _libm
+    a = 2 * math.pi * float(freq) * float(dt) * t
+    if np.dtype(dtype) == np.float64:
+        return np.float64(math.cos(a)), np.float64(math.sin(a))
+    if _libm is None:
+        _libm = ctypes.CDLL("libm.so.6")
+        for f in (_libm.cosf, _libm.sinf):
+            f.restype, f.argtypes = ctypes.c_float, [ctypes.c_float]
+    af = float(np.float32(a))
+    return np.float32(_libm.cosf(af)), np.float32(_libm.sinf(af))
+
+
+def fourier(t, freq, dt, ezi, ez, ft, probe=6):
+    """fd2d/clang/test_3_4.c:38-53: running DFT of ez (planes ft["r_pt"][n],
+    ft["i_pt"][n]) and of the incident probe ezi[probe]."""
+    for n, f in enumerate(freq):
+        c, s = twiddle(t, f, dt, ez.dtype)
+        ft["r_in"][n] += c * ezi[probe]
+        ft["i_in"][n] -= s * ezi[probe]
+        ft["r_pt"][n] += c * ez
+        ft["i_pt"][n] -= s * ez
+
+
+def tfsf_run(state, inc, nsteps, npml, pml, naz, nbz, inc_wave, freq=(), dt=0.0, ft=None,
+             t0=0, inc_pos=3):
+    """The main loop of fd2d/clang/test_3_4.c:223-233 for nsteps from t0:
+    state {dz, iz, ez, hx, hy, ihx, ihy}, inc {ezi, hxi, bc}; inc_wave[t-1]
+    is the incident source of step t (the program's gaussian(t, 20, 8))."""
+    nx, ny = state["dz"].shape
+    for s in range(nsteps):
+        t = t0 + s + 1
+        ezinct(inc["ezi"], inc["hxi"], inc["bc"])
+        dfield(nx, ny, pml, state["dz"], state["hx"], state["hy"])
+        inc["ezi"][inc_pos] = inc_wave[t - 1]
+        inctdz(nx, ny, npml, inc["hxi"], state["dz"])
+        efield(naz, nbz, state["dz"], state["iz"], state["ez"])
+        if len(freq):
+            fourier(t, freq, dt, inc["ezi"], state["ez"], ft)
+        hxinct(inc["ezi"], inc["hxi"])
+        hfield(nx, ny, pml, state["ez"], state["ihx"], state["ihy"], state["hx"], state["hy"])
+        incthx(nx, ny, npml, inc["ezi"], state["hx"])
+        incthy(nx, ny, npml, inc["ezi"], state["hy"])

@@ -19,6 +19,7 @@ FDTD_EINVAL = -1
 MAX_SOURCES = 16
 MAX_PROBES = 16
 MAX_CYLINDERS = 64
+MAX_FREQ = 16
 ENERGY_WORK = 1025
 SRC_NONE, SRC_HARD, SRC_ADD = 0, 1, 2
 
@@ -79,6 +80,22 @@ class Fd2dGrid(C.Structure):
     ]
 
 
+class FdtdFtrans(C.Structure):
+    """fdtd_ftrans: the DFT planes (fd2d/pycuda/fd2d_3_4.py:100-103)."""
+    _fields_ = [(k, _f) for k in ("r_pt", "i_pt", "r_in", "i_in")]
+
+
+class Fd2dTfsfDesc(C.Structure):
+    """fd2d_tfsf_desc: incident plane wave and DFT of a fd2d_tfsf_run."""
+    _fields_ = [
+        ("npml", C.c_int),
+        ("ezi", _f), ("hxi", _f), ("bc", _f),
+        ("inc_pos", C.c_int), ("inc_wave", _f), ("inc_len", C.c_int),
+        ("nf", C.c_int), ("freq", _f), ("dt", C.c_float),
+        ("ft", FdtdFtrans), ("dft_probe", C.c_int),
+    ]
+
+
 class Fd2dLbcDesc(C.Structure):
     """fd2d_lbc_desc: network and boundary-sample history (include/fdtd.h)."""
     _fields_ = [("params", C.c_void_p), ("wbuf", C.c_void_p), ("hist", C.c_void_p),
@@ -136,6 +153,14 @@ SIGNATURES = {
     "fd2d_batch_graph": (_I, [C.POINTER(Fd2dBatchDesc), _I, _I, _I, _I, _P, C.POINTER(C.c_void_p)]),
     "fdtd_graph_launch": (_I, [_P, _P]),
     "fdtd_graph_destroy": (_I, [_P]),
+    "fd2d_ezinct": (_I, [_I, _P, _P, _P, _P]),
+    "fd2d_dfield_inc": (_I, [_I, _I, _I, C.POINTER(FdtdPml), _P, _P, _P, _P, _I, _F, _P]),
+    "fd2d_inctdz": (_I, [_I, _I, _I, _P, _P, _P]),
+    "fd2d_hxinct": (_I, [_I, _P, _P, _P]),
+    "fd2d_incthx": (_I, [_I, _I, _I, _P, _P, _P]),
+    "fd2d_incthy": (_I, [_I, _I, _I, _P, _P, _P]),
+    "fd2d_fourier": (_I, [_I, _I, _I, _I, _F, C.POINTER(C.c_float), _P, _P, C.POINTER(FdtdFtrans), _P]),
+    "fd2d_tfsf_run": (_I, [C.POINTER(Fd2dTmDesc), C.POINTER(Fd2dTfsfDesc), _I, _I, _P]),
     "lbc_hist_elems": (C.c_size_t, [_I, _I]),
     "lbc_work_bytes": (C.c_size_t, [_I, _I]),
     "fd2d_lbc_margin": (_I, []),

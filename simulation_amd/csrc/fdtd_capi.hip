@@ -605,6 +605,136 @@ int fd2d_tm_run(const fd2d_tm_desc *d, int t0, int nsteps, void *stream)
     return 0;
 }
 
+// ------------------------------------------------ plane wave (TF/SF) and DFT
+
+static int tfsf_box_check(const char *fn, int nx, int ny, int npml)
+{
+    if (npml < 2 || npml - 1 > nx - npml || npml - 1 > ny - npml)
+        return fail("%s: npml=%d gives no TF/SF box in %dx%d", fn, npml, nx, ny);
+    return 0;
+}
+
+int fd2d_ezinct(int ny, float *ezi, const float *hxi, float *bc, void *stream)
+{
+    clear_error();
+    if (ny < 3) return fail("fd2d_ezinct: ny=%d", ny);
+    if (!ezi || !hxi || !bc) return fail("fd2d_ezinct: ezi, hxi, bc are required");
+    return launch_ezinct(ny, ezi, hxi, bc, 0, 0, 0.0f, S(stream));
+}
+
+int fd2d_dfield_inc(int t, int nx, int ny, const fdtd_pml *pml, float *ezi, float *dz,
+                    const float *hx, const float *hy, int inc_pos, float inc_val, void *stream)
+{
+    clear_error();
+    if (!ezi) return fail("fd2d_dfield_inc: ezi is required");
+    if (inc_pos < 0 || inc_pos >= ny) return fail("fd2d_dfield_inc: inc_pos=%d outside [0,%d)", inc_pos, ny);
+    int rc = fd2d_dfield(t, nx, ny, pml, dz, hx, hy, 0, nullptr, nullptr, nullptr, stream);
+    if (rc) return rc;
+    return launch_set(ezi, inc_pos, inc_val, S(stream));
+}
+
+int fd2d_inctdz(int nx, int ny, int npml, const float *hxi, float *dz, void *stream)
+{
+    clear_error();
+    int rc = tfsf_box_check("fd2d_inctdz", nx, ny, npml);
+    if (rc) return rc;
+    if (!hxi || !dz) return fail("fd2d_inctdz: hxi and dz are required");
+    return launch_inctdz(nx, ny, npml, hxi, dz, S(stream));
+}
+
+int fd2d_hxinct(int ny, const float *ezi, float *hxi, void *stream)
+{
+    clear_error();
+    if (ny < 2) return fail("fd2d_hxinct: ny=%d", ny);
+    if (!ezi || !hxi) return fail("fd2d_hxinct: ezi and hxi are required");
+    return launch_hxinct(ny, ezi, hxi, S(stream));
+}
+
+int fd2d_incthx(int nx, int ny, int npml, const float *ezi, float *hx, void *stream)
+{
+    clear_error();
+    int rc = tfsf_box_check("fd2d_incthx", nx, ny, npml);
+    if (rc) return rc;
+    if (!ezi || !hx) return fail("fd2d_incthx: ezi and hx are required");
+    return launch_incthx(nx, ny, npml, ezi, hx, S(stream));
+}
+
+int fd2d_incthy(int nx, int ny, int npml, const float *ezi, float *hy, void *stream)
+{
+    clear_error();
+    int rc = tfsf_box_check("fd2d_incthy", nx, ny, npml);
+    if (rc) return rc;
+    if (!ezi || !hy) return fail("fd2d_incthy: ezi and hy are required");
+    return launch_incthy(nx, ny, npml, ezi, hy, S(stream));
+}
+
+static int fourier_check(const char *fn, int nf, int ny, const float *freq, const float *ezi,
+                         int probe, const fdtd_ftrans *ft)
+{
+    if (nf < 1 || nf > FDTD_MAX_FREQ) return fail("%s: nf=%d outside [1,%d]", fn, nf, FDTD_MAX_FREQ);
+    if (!freq) return fail("%s: freq (host array) is required", fn);
+    if (!ft || !ft->r_pt || !ft->i_pt) return fail("%s: ft->r_pt and ft->i_pt are required", fn);
+    if ((ft->r_in != nullptr) != (ft->i_in != nullptr)) return fail("%s: r_in and i_in go together", fn);
+    if (ft->r_in && (!ezi || probe < 0 || probe >= ny))
+        return fail("%s: the incident transform needs ezi with %d cells", fn, probe + 1);
+    return 0;
+}
+
+int fd2d_fourier(int t, int nf, int nx, int ny, float dt, const float *freq, const float *ezi,
+                 const float *ez, const fdtd_ftrans *ft, void *stream)
+{
+    clear_error();
+    if (nx < 1 || ny < 1 || !ez) return fail("fd2d_fourier: grid %dx%d, ez required", nx, ny);
+    int rc = fourier_check("fd2d_fourier", nf, ny, freq, ezi, 6, ft);
+    if (rc) return rc;
+    return launch_fourier(t, nf, nx, ny, dt, freq, ezi, 6, ez, *ft, S(stream));
+}
+
+int fd2d_tfsf_run(const fd2d_tm_desc *d, const fd2d_tfsf_desc *b, int t0, int nsteps, void *stream)
+{
+    clear_error();
+    int rc = tm_validate(d, t0, nsteps);
+    if (rc) return rc;
+    if (!b) return fail("fd2d_tfsf_run: NULL plane-wave descriptor");
+    rc = tfsf_box_check("fd2d_tfsf_run", d->nx, d->ny, b->npml);
+    if (rc) return rc;
+    if (!b->ezi || !b->hxi || !b->bc) return fail("fd2d_tfsf_run: ezi, hxi, bc are required");
+    if (b->inc_pos < 0 || b->inc_pos >= d->ny) return fail("fd2d_tfsf_run: inc_pos=%d", b->inc_pos);
+    if (!b->inc_wave || b->inc_len < t0 + nsteps)
+        return fail("fd2d_tfsf_run: inc_wave (host) must cover step %d", t0 + nsteps);
+    if (b->nf) {
+        rc = fourier_check("fd2d_tfsf_run", b->nf, d->ny, b->freq, b->ezi, b->dft_probe, &b->ft);
+        if (rc) return rc;
+    }
+    const bool pml = pml_all(&d->pml);
+    PmlPtrs pp = to_ptrs(d->pml);
+    hipStream_t s = S(stream);
+    for (int k = 0; k < nsteps; ++k) {
+        const int t = t0 + k + 1;
+        SrcList src{};
+        src.n = d->nsrc;
+        for (int q = 0; q < d->nsrc; ++q) { src.i[q] = d->src_i[q]; src.j[q] = d->src_j[q]; }
+        src.wave = d->src_wave; src.wave_len = d->wave_len; src.t = t;
+        rc = launch_ezinct(d->ny, b->ezi, b->hxi, b->bc, 1, b->inc_pos, b->inc_wave[t - 1], s);
+        if (!rc) rc = launch_fd2d_dfield(d->nx, d->ny, pml ? &pp : nullptr, d->dz, d->hx, d->hy, src, s);
+        if (!rc) rc = launch_inctdz(d->nx, d->ny, b->npml, b->hxi, d->dz, s);
+        if (!rc) rc = launch_fd2d_efield(d->nx, d->ny, d->md.naz, d->md.nbz, d->dz, d->iz, d->ez, s);
+        if (!rc && b->nf)
+            rc = launch_fourier(t, b->nf, d->nx, d->ny, b->dt, b->freq, b->ezi, b->dft_probe, d->ez,
+                                b->ft, s);
+        if (!rc) rc = launch_hxinct(d->ny, b->ezi, b->hxi, s);
+        if (!rc) rc = launch_fd2d_hfield(d->nx, d->ny, pml ? &pp : nullptr, d->ez, d->ihx, d->ihy,
+                                         d->hx, d->hy, s);
+        if (!rc) rc = launch_incthx(d->nx, d->ny, b->npml, b->ezi, d->hx, s);
+        if (!rc) rc = launch_incthy(d->nx, d->ny, b->npml, b->ezi, d->hy, s);
+        if (!rc && d->nprobe)
+            rc = launch_fd2d_probe(d->ny, d->ez, d->nprobe, d->probe_i, d->probe_j, d->probe_out,
+                                   d->probe_len, t - 1, s);
+        if (rc) return rc;
+    }
+    return 0;
+}
+
 // ------------------------------------------------ learned boundary
 
 size_t lbc_hist_elems(int nx, int ny)

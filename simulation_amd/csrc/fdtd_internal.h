@@ -86,6 +86,18 @@ int launch_fd2d_energy_diff(int wn, int wm, int nya, int ia0, int ja0, const flo
                             const float *hxa, const float *hya, int nyb, int ib0, int jb0,
                             const float *ezb, const float *hxb, const float *hyb, double *work,
                             hipStream_t s);
+// plane wave (tfsf.hip)
+void dft_twiddle(int t, float freq, float dt, float *c, float *s);
+int launch_ezinct(int ny, float *ezi, const float *hxi, float *bc, int set_inc, int inc_pos,
+                  float inc_val, hipStream_t s);
+int launch_set(float *p, int pos, float v, hipStream_t s);
+int launch_hxinct(int ny, const float *ezi, float *hxi, hipStream_t s);
+int launch_inctdz(int nx, int ny, int npml, const float *hxi, float *dz, hipStream_t s);
+int launch_incthx(int nx, int ny, int npml, const float *ezi, float *hx, hipStream_t s);
+int launch_incthy(int nx, int ny, int npml, const float *ezi, float *hy, hipStream_t s);
+int launch_fourier(int t, int nf, int nx, int ny, float dt, const float *freq, const float *ezi,
+                   int probe, const float *ez, const fdtd_ftrans &ft, hipStream_t s);
+
 size_t lbc_frame_smem();
 // Geometry tables of the frame workgroups (per depth K = 1..8, one after the
 // other in the caller's work buffer; built by launch_lbc_frame_setup).

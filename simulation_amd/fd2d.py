@@ -30,7 +30,9 @@ pmlayer = namedtuple("pmlayer", ("fx1", "fx2", "fx3", "fy1", "fy2", "fy3",
 medium = namedtuple("medium", ("naz", "nbz"))
 
 __all__ = ["pmlayer", "medium", "new_pmlayer", "pmlparam", "dielectric", "dfield", "efield",
-           "hfield", "energy", "TMSolver", "TMConfig", "simulate", "c2_config", "time_step"]
+           "hfield", "energy", "TMSolver", "TMConfig", "simulate", "c2_config", "time_step",
+           "ftrans", "ezinct", "dfield_inc", "inctdz", "hxinct", "incthx", "incthy", "fourier",
+           "new_ftrans", "amplitude_phase", "PlaneWaveSolver"]
 
 
 def time_step(ds: float = 0.01) -> float:
@@ -113,6 +115,134 @@ def energy(ez, hx, hy, window=None, work: torch.Tensor | None = None) -> float:
 
 
 STATE = ("dz", "iz", "ez", "hx", "hy", "ihx", "ihy")
+
+
+# ------------------------------------------------ plane wave (TF/SF) and DFT
+
+ftrans = namedtuple("ftrans", ("r_pt", "i_pt", "r_in", "i_in"))
+
+
+def ezinct(ny: int, ezi, hxi, bc) -> None:
+    """Incident Ez and its two-step boundary (fd2d/pycuda/fd2d_3_4.py:140-148)."""
+    call("fd2d_ezinct", int(ny), ptr(ezi), ptr(hxi), ptr(bc), stream())
+
+
+def dfield_inc(t: int, nx: int, ny: int, pml: pmlayer | None, ezi, dz, hx, hy,
+               inc_pos: int = 3, inc_val: float | None = None) -> None:
+    """D update plus the incident source ezi[inc_pos] = inc_val, default
+    gaussian(t, 20, 8) at 3 (fd2d/pycuda/fd2d_3_4.py:151-161)."""
+    from .fd1d import gaussian
+    v = gaussian(t, 20, 8.0) if inc_val is None else inc_val
+    call("fd2d_dfield_inc", int(t), int(nx), int(ny), _pml_struct(pml), ptr(ezi), ptr(dz),
+         ptr(hx), ptr(hy), int(inc_pos), float(v), stream())
+
+
+def inctdz(nx: int, ny: int, npml: int, hxi, dz) -> None:
+    """TF/SF correction of Dz (fd2d/pycuda/fd2d_3_4.py:164-170)."""
+    call("fd2d_inctdz", int(nx), int(ny), int(npml), ptr(hxi), ptr(dz), stream())
+
+
+def hxinct(ny: int, ezi, hxi) -> None:
+    """Incident Hx (fd2d/pycuda/fd2d_3_4.py:186-191)."""
+    call("fd2d_hxinct", int(ny), ptr(ezi), ptr(hxi), stream())
+
+
+def incthx(nx: int, ny: int, npml: int, ezi, hx) -> None:
+    """TF/SF correction of Hx (fd2d/pycuda/fd2d_3_4.py:211-217)."""
+    call("fd2d_incthx", int(nx), int(ny), int(npml), ptr(ezi), ptr(hx), stream())
+
+
+def incthy(nx: int, ny: int, npml: int, ezi, hy) -> None:
+    """TF/SF correction of Hy (fd2d/pycuda/fd2d_3_4.py:220-226)."""
+    call("fd2d_incthy", int(nx), int(ny), int(npml), ptr(ezi), ptr(hy), stream())
+
+
+def _ft_struct(ft: ftrans) -> _lib.FdtdFtrans:
+    return _lib.FdtdFtrans(fptr(ft.r_pt), fptr(ft.i_pt), fptr(ft.r_in), fptr(ft.i_in))
+
+
+def fourier(t: int, nf: int, nx: int, ny: int, dt: float, freq, ezi, ez, ft: ftrans) -> None:
+    """Running DFT of ez and of the incident probe ezi[6]
+    (fd2d/pycuda/fd2d_3_4.py:121-136); freq: host frequencies."""
+    fr = (C.c_float * nf)(*[float(f) for f in freq])
+    call("fd2d_fourier", int(t), int(nf), int(nx), int(ny), float(dt), fr, ptr(ezi), ptr(ez),
+         C.byref(_ft_struct(ft)), stream())
+
+
+def new_ftrans(nf: int, nx: int, ny: int, device=None) -> ftrans:
+    dev = device or require_gpu()
+    z = lambda *s: torch.zeros(s, dtype=torch.float32, device=dev)  # noqa: E731
+    return ftrans(z(nf, nx, ny), z(nf, nx, ny), z(nf), z(nf))
+
+
+def amplitude_phase(ft: ftrans, nx: int, ny: int, npml: int):
+    """Amplitude and phase at each frequency along row nx/2-1, as the
+    reference's post-processing (fd2d/pycuda/fd2d_3_4.py:378-385): host
+    NumPy on the FP32 transforms.  Returns (amplt, phase), (nf, ny) arrays
+    set on columns npml-1 .. ny-npml."""
+    r_pt, i_pt = ft.r_pt.cpu().numpy(), ft.i_pt.cpu().numpy()
+    r_in, i_in = ft.r_in.cpu().numpy(), ft.i_in.cpu().numpy()
+    nf = r_in.shape[0]
+    amplt = np.zeros((nf, ny), np.float32)
+    phase = np.zeros((nf, ny), np.float32)
+    j = slice(npml - 1, ny - npml + 1)
+    for n in range(nf):
+        rp, ip = r_pt[n, nx // 2 - 1, j], i_pt[n, nx // 2 - 1, j]
+        amplt[n, j] = 1 / np.hypot(r_in[n], i_in[n]) * np.hypot(rp, ip)
+        phase[n, j] = np.arctan2(ip, rp) - np.arctan2(i_in[n], r_in[n])
+    return amplt, phase
+
+
+class PlaneWaveSolver:
+    """A plane wave striking the grid through the TF/SF box, with the running
+    DFT (fd2d/pycuda/fd2d_3_4.py:261-375): fd2d_tfsf_run over a TMSolver's
+    state set.  inc_wave: host values of the incident source ezi[inc_pos]
+    for steps 1.. (default gaussian(t, 20, 8) at 3, as the reference); freq:
+    DFT frequencies (none: no DFT)."""
+
+    def __init__(self, nx, ny, npml, pml, md, nsteps_max, freq=(), dt=None, inc_pos=3,
+                 inc_wave=None, probes=(), device=None):
+        from .fd1d import gaussian
+        self.tm = TMSolver(nx, ny, pml, md, (), probes, nsteps_max, device=device)
+        dev = self.tm.device
+        self.npml = int(npml)
+        self.ezi = torch.zeros(ny, dtype=torch.float32, device=dev)
+        self.hxi = torch.zeros(ny, dtype=torch.float32, device=dev)
+        self.bc = torch.zeros(4, dtype=torch.float32, device=dev)
+        if inc_wave is None:
+            inc_wave = [gaussian(t, 20, 8.0) for t in range(1, nsteps_max + 1)]
+        self.inc_wave = np.ascontiguousarray(np.asarray(inc_wave, np.float32))
+        self.inc_pos = int(inc_pos)
+        self.freq = np.ascontiguousarray(np.asarray(freq, np.float32))
+        self.dt = float(time_step() if dt is None else dt)
+        self.ft = new_ftrans(len(self.freq), nx, ny, dev) if len(self.freq) else None
+
+    @property
+    def t(self) -> int:
+        return self.tm.t
+
+    def desc(self) -> _lib.Fd2dTfsfDesc:
+        b = _lib.Fd2dTfsfDesc()
+        b.npml = self.npml
+        b.ezi, b.hxi, b.bc = fptr(self.ezi), fptr(self.hxi), fptr(self.bc)
+        b.inc_pos = self.inc_pos
+        b.inc_wave = self.inc_wave.ctypes.data_as(C.POINTER(C.c_float))
+        b.inc_len = len(self.inc_wave)
+        b.nf = len(self.freq)
+        b.freq = self.freq.ctypes.data_as(C.POINTER(C.c_float)) if b.nf else None
+        b.dt = self.dt
+        if self.ft is not None:
+            b.ft = _ft_struct(self.ft)
+        b.dft_probe = 6
+        return b
+
+    def run(self, nsteps: int) -> None:
+        d, b = self.tm.desc(), self.desc()
+        call("fd2d_tfsf_run", C.byref(d), C.byref(b), int(self.tm.t), int(nsteps), stream())
+        self.tm.t += int(nsteps)
+
+    def state(self) -> dict:
+        return self.tm.state()
 
 
 class TMSolver:

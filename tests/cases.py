@@ -44,3 +44,21 @@ def fd2d_case(name: str, waveform):
     return dict(nx=nx, ny=ny, ns=ns, dtype=dt, pml=pml, naz=a["in_naz"],
                 nbz=a.get("in_nbz"), sources=[(src["i"], src["j"])], wave=w,
                 probes=[tuple(p) for p in meta["probes"]], meta=meta, exp=a)
+
+
+TFSF_CASES = ("fd2d_tfsf_f32", "fd2d_tfsf_f64")
+
+
+def tfsf_case(name: str, gaussian):
+    """A plane-wave fixture (fd2d/clang/test_3_4.c / fd2d_3_4.c main loop):
+    coefficients, DFT frequencies, the incident source values gaussian(t,
+    20, 8) of steps 1..ns, and the expected outputs."""
+    meta, a = golden(name)
+    dt = np.float32 if meta["dtype"] == "float32" else np.float64
+    inc = meta["incident"]
+    wave = np.array([gaussian(t, inc["t0"], inc["spread"], dt) for t in range(1, meta["ns"] + 1)],
+                    dt)
+    pml = {k: a["in_" + k] for k in PML_KEYS}
+    return dict(nx=meta["nx"], ny=meta["ny"], ns=meta["ns"], npml=meta["npml"], dtype=dt,
+                pml=pml, naz=a["in_naz"], nbz=a["in_nbz"], freq=a["in_freq"],
+                dt=dt(meta["dt"]), inc_pos=inc["pos"], wave=wave, meta=meta, exp=a)

@@ -25,6 +25,9 @@ and non-square grids):
   fd2d_c2_{f32,f64}    test_3_4.c / fd2d_3_4.c   97x131, npml=8, 150 steps: PML dfield/hfield,
                                                   lossy efield (naz, nbz, iz), hard Gaussian
                                                   (t0=20, spread 6) at (nx/2, ny/2), 4 probes
+  fd2d_tfsf_{f32,f64}  test_3_4.c / fd2d_3_4.c   70x90, npml=8, 160 steps: plane wave through the
+                                                  TF/SF box (incident grid + two-step boundary),
+                                                  PML, lossy cylinder, running DFT at 3 frequencies
   fd2d_dielectric_c    fd2d_3_4.c dielectric (FP64, integer sub-sampling), 97x131
   fd2d_pmlparam_c      fd2d_3_2.c / test_3_2.c pmlparam (FP64 / FP32) for npml 8, 32, 64
 """
@@ -238,6 +241,62 @@ def case_fd2d_c2(tag, prog, prog64, dt, nx, ny, ns, npml, rgrid, epsr, sigma, pr
          **{f"out_{k}": v for k, v in f.items()}, out_probe=probe_out)
 
 
+def case_fd2d_tfsf(tag, prog, prog64, dt, nx, ny, ns, npml, rgrid, epsr, sigma, freqs):
+    """The plane-wave program's main loop (ezinct, dfield with the incident
+    source ezi[3] = gaussian(t, 20, 8), inctdz, efield, fourier, hxinct,
+    hfield, incthx, incthy), PML + lossy cylinder + running DFT, with the
+    program's own functions in the order of its main()."""
+    lib = load(prog)
+    lib64 = load(prog64)
+    pml = {k: v.astype(dt) for k, v in ref_pml(lib64, np.float64, nx, ny, npml).items()}
+    M64 = medium_struct(np.float64)
+    lib64.dielectric.restype = M64
+    if dt == np.float32:
+        dtv = np.float32(np.float32(0.01) / np.float32(6e8))
+    else:
+        dtv = np.float64(0.01 / 6e8)
+    md = lib64.dielectric(C.c_int(nx), C.c_int(ny), C.c_int(npml), C.c_int(rgrid),
+                          C.c_double(float(dtv)), C.c_double(epsr), C.c_double(sigma))
+    naz = np.ctypeslib.as_array(md.naz, shape=(nx, ny)).copy().astype(dt)
+    nbz = np.ctypeslib.as_array(md.nbz, shape=(nx, ny)).copy().astype(dt)
+    S = pml_struct(dt)
+    s = S(*[P(pml[k]) for k in PML_KEYS])
+    M = medium_struct(dt)
+    m = M(P(naz), P(nbz))
+    R = real(dt)
+    ptr = C.POINTER(R)
+
+    class Ftrans(C.Structure):
+        _fields_ = [(k, ptr) for k in ("r_pt", "i_pt", "r_in", "i_in")]
+    nf = len(freqs)
+    freq = np.asarray(freqs, dt)
+    ft = {"r_pt": np.zeros((nf, nx, ny), dt), "i_pt": np.zeros((nf, nx, ny), dt),
+          "r_in": np.zeros(nf, dt), "i_in": np.zeros(nf, dt)}
+    fts = Ftrans(*[P(ft[k]) for k in ("r_pt", "i_pt", "r_in", "i_in")])
+    f = {k: np.zeros((nx, ny), dt) for k in ("dz", "iz", "ez", "hx", "hy", "ihx", "ihy")}
+    ezi = np.zeros(ny, dt)
+    hxi = np.zeros(ny, dt)
+    bc = np.zeros(4, dt)
+    I = C.c_int
+    for t in range(1, ns + 1):
+        lib.ezinct(I(ny), P(ezi), P(hxi), P(bc))
+        lib.dfield(I(t), I(nx), I(ny), C.byref(s), P(ezi), P(f["dz"]), P(f["hx"]), P(f["hy"]))
+        lib.inctdz(I(nx), I(ny), I(npml), P(hxi), P(f["dz"]))
+        lib.efield(I(nx), I(ny), C.byref(m), P(f["dz"]), P(f["iz"]), P(f["ez"]))
+        lib.fourier(I(t), I(nf), I(nx), I(ny), R(dtv), P(freq), P(ezi), P(f["ez"]), C.byref(fts))
+        lib.hxinct(I(ny), P(ezi), P(hxi))
+        lib.hfield(I(nx), I(ny), C.byref(s), P(f["ez"]), P(f["ihx"]), P(f["ihy"]), P(f["hx"]),
+                   P(f["hy"]))
+        lib.incthx(I(nx), I(ny), I(npml), P(ezi), P(f["hx"]))
+        lib.incthy(I(nx), I(ny), I(npml), P(ezi), P(f["hy"]))
+    meta = dict(program=prog, coef_program=prog64, nx=nx, ny=ny, ns=ns, npml=npml, rgrid=rgrid,
+                epsr=epsr, sigma=sigma, dt=float(dtv), dtype=str(np.dtype(dt)), freqs=list(freqs),
+                incident=dict(pos=3, t0=20, spread=8.0), dft_probe=6)
+    save(tag, meta, in_naz=naz, in_nbz=nbz, in_freq=freq, **{f"in_{k}": v for k, v in pml.items()},
+         **{f"out_{k}": v for k, v in f.items()}, out_ezi=ezi, out_hxi=hxi, out_bc=bc,
+         **{f"out_ft_{k}": v for k, v in ft.items()})
+
+
 def case_dielectric_c(tag, nx, ny, npml, rgrid, epsr, sigma):
     lib = load("fd2d_3_4")
     M64 = medium_struct(np.float64)
@@ -275,6 +334,8 @@ def main():
                       probes=[(30, 35), (10, 20), (45, 60)])
         case_fd2d_3_2(f"fd2d_3_2_{suf}", f"{q}_3_2", dt, 64, 80, 100, 8,
                       probes=[(27, 35), (5, 5), (60, 70)])
+        case_fd2d_tfsf(f"fd2d_tfsf_{suf}", f"{q}_3_4", "fd2d_3_4", dt, 70, 90, 160, 8, 12,
+                       30.0, 0.30, (50e6, 300e6, 700e6))
         case_fd2d_c2(f"fd2d_c2_{suf}", f"{q}_3_4", "fd2d_3_4", dt, 97, 131, 150, 8, 30, 30.0, 0.30,
                      probes=[(48, 75), (48, 100), (70, 65), (10, 65)])
     case_dielectric_c("fd2d_dielectric_c", 97, 131, 8, 30, 30.0, 0.30)

@@ -1,0 +1,105 @@
+"""Plane wave (TF/SF) and running DFT on the GPU against the reference's own
+plane-wave program (fixture tests/golden/fd2d_tfsf_f32.npz, written from
+fd2d/clang/test_3_4.c's functions in its main-loop order).
+
+Bar: bit-exact (np.array_equal) for every state array, the incident grid and
+its boundary state, and every DFT plane.  The DFT twiddles are evaluated on
+the host exactly as the C program evaluates them, so the device sums see
+the same FP32 operands."""
+import numpy as np
+import pytest
+import torch
+
+from cases import PML_KEYS, tfsf_case
+
+pytestmark = pytest.mark.gpu
+
+
+def _gauss(t, t0, spread, dt):
+    from simulation_amd import fd1d
+    return fd1d.gaussian(t, t0, spread, dt)
+
+
+def _t(a, dev):
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float32)).to(dev)
+
+
+def _solver(c, dev):
+    from simulation_amd import fd2d
+    pml = fd2d.pmlayer(**{k: _t(c["pml"][k], dev) for k in PML_KEYS})
+    md = fd2d.medium(_t(c["naz"], dev), _t(c["nbz"], dev))
+    s = fd2d.PlaneWaveSolver(c["nx"], c["ny"], c["npml"], pml, md, c["ns"], freq=c["freq"],
+                             dt=float(c["dt"]), inc_pos=c["inc_pos"], inc_wave=c["wave"],
+                             device=dev)
+    return s, pml, md
+
+
+def _check(c, s):
+    e = c["exp"]
+    st = s.state()
+    for k in ("dz", "iz", "ez", "hx", "hy", "ihx", "ihy"):
+        assert np.array_equal(st[k], e["out_" + k]), k
+    for k in ("ezi", "hxi", "bc"):
+        assert np.array_equal(getattr(s, k).cpu().numpy(), e["out_" + k]), k
+    for k in ("r_pt", "i_pt", "r_in", "i_in"):
+        assert np.array_equal(getattr(s.ft, k).cpu().numpy(), e["out_ft_" + k]), k
+
+
+@pytest.mark.parametrize("chunks", [(160,), (1, 63, 96)])
+def test_plane_wave_run_bit_exact(gpu, chunks):
+    c = tfsf_case("fd2d_tfsf_f32", _gauss)
+    print(f"[tfsf] run {c['nx']}x{c['ny']} npml={c['npml']} chunks={chunks}", flush=True)
+    s, _, _ = _solver(c, gpu)
+    for n in chunks:
+        s.run(n)
+    _check(c, s)
+
+
+def test_plane_wave_step_kernels_bit_exact(gpu):
+    """The per-step drop-ins, called in the reference's order."""
+    from simulation_amd import fd2d
+    c = tfsf_case("fd2d_tfsf_f32", _gauss)
+    print("[tfsf] per-step kernels", flush=True)
+    s, pml, md = _solver(c, gpu)
+    tm, nx, ny, npml = s.tm, c["nx"], c["ny"], c["npml"]
+    for t in range(1, c["ns"] + 1):
+        fd2d.ezinct(ny, s.ezi, s.hxi, s.bc)
+        fd2d.dfield_inc(t, nx, ny, pml, s.ezi, tm.dz, tm.hx, tm.hy, c["inc_pos"],
+                        float(c["wave"][t - 1]))
+        fd2d.inctdz(nx, ny, npml, s.hxi, tm.dz)
+        fd2d.efield(nx, ny, md, tm.dz, tm.iz, tm.ez)
+        fd2d.fourier(t, len(c["freq"]), nx, ny, float(c["dt"]), c["freq"], s.ezi, tm.ez, s.ft)
+        fd2d.hxinct(ny, s.ezi, s.hxi)
+        fd2d.hfield(nx, ny, pml, tm.ez, tm.ihx, tm.ihy, tm.hx, tm.hy)
+        fd2d.incthx(nx, ny, npml, s.ezi, tm.hx)
+        fd2d.incthy(nx, ny, npml, s.ezi, tm.hy)
+    _check(c, s)
+
+
+def test_amplitude_phase_postprocessing(gpu):
+    """The reference's amplitude/phase along row nx/2-1 (host NumPy, as the
+    PyCUDA script): finite on the box columns, zero outside, amplitude of
+    the incident transform itself is 1 where the field equals it."""
+    from simulation_amd import fd2d
+    c = tfsf_case("fd2d_tfsf_f32", _gauss)
+    s, _, _ = _solver(c, gpu)
+    s.run(c["ns"])
+    a, p = fd2d.amplitude_phase(s.ft, c["nx"], c["ny"], c["npml"])
+    e = c["exp"]
+    j = slice(c["npml"] - 1, c["ny"] - c["npml"] + 1)
+    r_in, i_in = e["out_ft_r_in"], e["out_ft_i_in"]
+    rp, ip = e["out_ft_r_pt"][:, c["nx"] // 2 - 1, j], e["out_ft_i_pt"][:, c["nx"] // 2 - 1, j]
+    want = (1 / np.hypot(r_in, i_in))[:, None] * np.hypot(rp, ip)
+    assert np.allclose(a[:, j], want, rtol=1e-6)
+    assert np.all(np.isfinite(p[:, j])) and not np.any(a[:, : c["npml"] - 1])
+
+
+def test_plane_wave_rejects_bad_setups(gpu):
+    from simulation_amd import _lib
+    c = tfsf_case("fd2d_tfsf_f32", _gauss)
+    s, _, _ = _solver(c, gpu)
+    with pytest.raises(_lib.FdtdError, match="inc_wave"):
+        s.run(c["ns"] + 1)
+    s.npml = 1
+    with pytest.raises(_lib.FdtdError, match="npml"):
+        s.run(2)

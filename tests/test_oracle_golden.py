@@ -3,7 +3,7 @@ programs (tests/golden/make_golden.py): bit-exact, FP32 and FP64."""
 import numpy as np
 import pytest
 
-from cases import FD1D_CASES, FD2D_CASES, fd1d_case, fd2d_case
+from cases import FD1D_CASES, FD2D_CASES, TFSF_CASES, fd1d_case, fd2d_case, tfsf_case
 
 
 def _wave(O):
@@ -86,3 +86,26 @@ def test_pmlparam_matches_reference_after_fp32_storage(oracle, npml):
         ulp = np.spacing(np.abs(ref64))
         assert np.all(np.abs(p[k] - ref64) <= 2 * ulp), k
         assert np.allclose(p[k].astype(np.float32), ref32, rtol=2e-7, atol=0), k
+
+
+@pytest.mark.parametrize("name", TFSF_CASES)
+def test_tfsf_oracle_matches_reference(oracle, name):
+    """Plane wave through the TF/SF box with the running DFT: the NumPy
+    restatement against the reference program's own functions, bit-exact
+    (fields, incident grid and its boundary state, all DFT planes)."""
+    from oracle import numpy_oracle as NO
+    c = tfsf_case(name, oracle.gaussian)
+    dt, nx, ny, nf = c["dtype"], c["nx"], c["ny"], len(c["freq"])
+    st = {k: np.zeros((nx, ny), dt) for k in ("dz", "iz", "ez", "hx", "hy", "ihx", "ihy")}
+    inc = {"ezi": np.zeros(ny, dt), "hxi": np.zeros(ny, dt), "bc": np.zeros(4, dt)}
+    ft = {"r_pt": np.zeros((nf, nx, ny), dt), "i_pt": np.zeros((nf, nx, ny), dt),
+          "r_in": np.zeros(nf, dt), "i_in": np.zeros(nf, dt)}
+    NO.tfsf_run(st, inc, c["ns"], c["npml"], c["pml"], c["naz"], c["nbz"], c["wave"],
+                c["freq"], c["dt"], ft, inc_pos=c["inc_pos"])
+    e = c["exp"]
+    for k, v in list(st.items()) + list(inc.items()):
+        assert np.array_equal(v, e["out_" + k]), k
+    for k, v in ft.items():
+        assert np.array_equal(v, e["out_ft_" + k]), k
+    # the plane wave crossed the box and the cylinder scattered it
+    assert np.abs(e["out_ez"]).max() > 0.1 and np.abs(e["out_ft_r_pt"]).max() > 1.0
