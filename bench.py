@@ -55,13 +55,14 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=("tm", "large", "batch", "learned", "tfsf"),
+    ap.add_argument("--workload", choices=("tm", "large", "batch", "learned", "tfsf", "dform"),
                     default="tm",
                     help="tm: the N x N single-cylinder configuration (default, N=4096); "
                          "large: 16384^2, 16 cylinders, 4 sources, 5000 steps; "
                          "batch: 256 uneven grids; learned: 2048^2 learned boundary against "
                          "npml=32 PML; tfsf: the plane-wave program (1024^2, npml=80, TF/SF "
-                         "plane wave, lossy cylinder, running DFT at 3 frequencies)")
+                         "plane wave, lossy cylinder, running DFT at 3 frequencies); dform: "
+                         "the 1-D Debye program (38000 cells, DFT at 3 frequencies)")
     ap.add_argument("--weights", default="",
                     help="learned: network parameters (.npz from simulation_amd.learned); "
                          "default: the committed weights, else the zero network")
@@ -390,6 +391,105 @@ def bench_tfsf(a):
     print(json.dumps(rec), flush=True)
 
 
+def bench_dform(a):
+    """The reference's 1-D Debye program (fd1d/clang/test_2_3.c main): 38000
+    cells, Debye half space, additive Gaussian at 1, DFT at 50/200/500 MHz;
+    one step = dxfield, exfield, fourier, hyfield (fd1d_dform_run).
+    Algorithmic bytes per cell-update: R+W dx ix sx hy (32), R nax..ndx
+    (16), W ex (4), R+W 6 DFT planes (48)."""
+    import numpy as np
+    import torch
+    from simulation_amd import fd1d
+    dev = torch.device("cuda", 0)
+    nx = a.size if a.size != 4096 else 38000
+    K = a.steps if a.steps != 2000 else 8000
+    W = a.warmup
+    freq = (50e6, 200e6, 500e6)
+    dt = float(np.float32(np.float32(0.01) / np.float32(6e8)))
+    md = fd1d.dielectric_dform(nx, dt, 2.0, 0.001e-6, 2.0, 0.01, debye=True, device=dev)
+    s = fd1d.DFormSolver(nx, md, W + K, freq=freq, dt=dt, device=dev)
+    s.run(W)
+    torch.cuda.synchronize()
+    st = torch.cuda.current_stream()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record(st)
+    s.run(K)
+    e1.record(st)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    step_ms = e0.elapsed_time(e1) / K
+    b_alg = 52 + 16 * len(freq)
+    achieved = b_alg * nx / (step_ms * 1e-3) / 1e9
+    cpu = None if a.no_cpu else cpu_baseline_dform(nx, a.cpu_seconds, freq)
+    rec = {
+        "metric": METRIC, "value": round(nx * K / wall / 1e6, 2), "unit": "Mcell-updates/s",
+        "n_gpus": 1, "steps": K, "warmup": W, "ms_per_step": round(wall * 1e3 / K, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (fields start at zero; additive Gaussian t0=50 spread=10 at cell 1)",
+        "config": {"workload": f"1-D D/E form {nx} cells FP32, Debye half space (chi 2, tau 1 ns, "
+                               f"eps_r 2, sigma 0.01), DFT at 3 frequencies "
+                               f"(fd1d/clang/test_2_3.c)", "grid": [nx], "freq": list(freq)},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 5), "traffic": None,
+                     "kernel": "per-step kernels of fd1d_dform_run (one step)",
+                     "avg_launch_ms": round(step_ms, 4), "alg_bytes_per_launch": b_alg * nx},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(rec), flush=True)
+
+
+def cpu_baseline_dform(nx: int, budget_s: float, freq) -> dict | None:
+    """The reference's own 1-D Debye step (fd1d/clang/test_2_3.c:123-128, a
+    single-threaded C program, compiled under oracle/_ref) for a bounded
+    sample."""
+    import numpy as np
+    ref = os.path.join(ROOT, "oracle", "_ref", "libref_test_2_3.so")
+    if not os.path.exists(ref):
+        return None
+    L = C.CDLL(ref)
+    F = C.POINTER(C.c_float)
+    P = lambda a: a.ctypes.data_as(F)  # noqa: E731
+
+    class Med(C.Structure):
+        _fields_ = [(k, F) for k in ("nax", "nbx", "ncx", "ndx")]
+
+    class Ft(C.Structure):
+        _fields_ = [(k, F) for k in ("r_pt", "i_pt", "r_in", "i_in")]
+    dt = np.float32(np.float32(0.01) / np.float32(6e8))
+    L.dielectric.restype = Med
+    md = L.dielectric(C.c_int(nx), C.c_float(dt), C.c_float(2.0), C.c_float(0.001e-6),
+                      C.c_float(2.0), C.c_float(0.01))
+    f = {k: np.zeros(nx, np.float32) for k in ("dx", "ix", "sx", "ex", "hy")}
+    bc = np.zeros(4, np.float32)
+    nf = len(freq)
+    fr = np.asarray(freq, np.float32)
+    ftv = {"r_pt": np.zeros((nf, nx), np.float32), "i_pt": np.zeros((nf, nx), np.float32),
+           "r_in": np.zeros(nf, np.float32), "i_in": np.zeros(nf, np.float32)}
+    ft = Ft(*[P(ftv[k]) for k in ("r_pt", "i_pt", "r_in", "i_in")])
+    I = C.c_int
+
+    def step(t):
+        L.dxfield(I(t), I(nx), P(f["dx"]), P(f["hy"]))
+        L.exfield(I(nx), C.byref(md), P(f["dx"]), P(f["ix"]), P(f["sx"]), P(f["ex"]))
+        L.fourier(I(t), I(nf), I(nx), C.c_float(dt), P(fr), P(f["ex"]), C.byref(ft))
+        L.hyfield(I(nx), P(f["ex"]), P(f["hy"]), P(bc))
+    step(1)
+    t0 = time.perf_counter()
+    steps = 0
+    while True:
+        step(steps + 2)
+        steps += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s:
+            break
+    return {"value": round(nx * steps / el / 1e6, 2), "unit": "Mcell-updates/s", "cores": 1,
+            "kind": "reference",
+            "sample": f"{steps} steps of the {nx}-cell FP32 Debye step of fd1d/clang/test_2_3.c "
+                      f"(single-threaded program), {el:.1f} s"}
+
+
 def cpu_baseline_tfsf(n: int, budget_s: float, npml: int, rgrid: int, freq) -> dict | None:
     """The reference's own plane-wave step (fd2d/clang/test_3_4.c:223-233,
     compiled under oracle/_ref) on the host cores for a bounded sample."""
@@ -478,6 +578,8 @@ def main():
         return bench_learned(a)
     if a.workload == "tfsf":
         return bench_tfsf(a)
+    if a.workload == "dform":
+        return bench_dform(a)
     import torch
     from simulation_amd import fd2d
 

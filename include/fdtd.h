@@ -415,19 +415,74 @@ int lbc_adam(int nblocks, const float *part, float *params, float *m, float *v, 
              float lr, float beta1, float beta2, float eps, uint16_t *wbuf, float *grad_out,
              int first, int count, void *stream);
 
+/* ------------------------------------------------- running DFT planes
+ * Replaces the device struct `ftrans` (fd2d/pycuda/fd2d_3_4.py:100-103,
+ * fd1d/pycuda/fd1d_2_3.py:74-77): r_pt/i_pt are nf planes of the grid
+ * (plane n at n * cells), r_in/i_in the nf transforms of the source probe. */
+#define FDTD_MAX_FREQ 16
+typedef struct fdtd_ftrans {
+    float *r_pt, *i_pt;
+    float *r_in, *i_in;
+} fdtd_ftrans;
+
+/* ------------------------------------- 1-D D/E form, lossy and Debye media
+ * fd1d/pycuda/fd1d_2_1.py .. fd1d_2_3.py: D update with an additive source,
+ * E from D through the medium (nax, nbx: lossy; with ncx, ndx: Debye
+ * relaxation sx), the running DFT of ex, then fd1d_hyfield (two-step
+ * boundary, H update).  Serial semantics of fd1d/clang/test_2_3.c:26-90,
+ * FP32, bit-exact.                                                          */
+/* Replaces the device struct `medium` of fd1d/pycuda/fd1d_2_3.py:68-71.    */
+typedef struct fdtd_medium1d {
+    float *nax, *nbx;
+    float *ncx, *ndx;           /* NULL: the lossy form of fd1d_2_2 (no sx) */
+} fdtd_medium1d;
+
+/* `dxfield(t, nx, dx, hy)` (fd1d/pycuda/fd1d_2_3.py:96-102): dx update for
+ * i in [1, nx), then dx[src_pos] += src_val (src_pos < 0: no source; the
+ * reference adds gaussian(t, 50, 10) at 1).                                 */
+int fd1d_dxfield(int t, int nx, float *dx, const float *hy, int src_pos, float src_val,
+                 void *stream);
+/* `exfield(nx, md, dx, ix, sx, ex)` (fd1d/pycuda/fd1d_2_3.py:105-113; with
+ * md->ncx NULL, fd1d_2_2.py's lossy form; sx unused).                      */
+int fd1d_exfield_dform(int nx, const fdtd_medium1d *md, const float *dx, float *ix, float *sx,
+                       float *ex, void *stream);
+/* `fourier(t, nf, nx, dt, freq, ex, ft)` (fd1d/pycuda/fd1d_2_3.py:79-93):
+ * planes r_pt/i_pt of nx, probe ex[10]; freq host, twiddles as the 2-D
+ * fd2d_fourier.                                                            */
+int fd1d_fourier(int t, int nf, int nx, float dt, const float *freq, const float *ex,
+                 const fdtd_ftrans *ft, void *stream);
+/* `dielectric(nx, dt, chi, tau, epsr, sigma)` of fd1d/clang/test_2_3.c:75-90
+ * (md->ncx NULL: test_2_2.c's lossy medium): free space below nx/2.        */
+int fd1d_dielectric_dform(int nx, float dt, float chi, float tau, float epsr, float sigma,
+                          const fdtd_medium1d *md, void *stream);
+
+/* A whole 1-D D/E-form run, per step t = t0+1 .. t0+nsteps: dxfield with
+ * dx[src_pos] += src_wave[t-1], exfield, fourier (nf > 0), fd1d_hyfield,
+ * probes ex[probe_k[p]] -> probe_out[p*probe_len + t-1].                    */
+typedef struct fd1d_dform_desc {
+    int nx;
+    float *dx, *ix, *sx, *ex, *hy, *bc;
+    fdtd_medium1d md;
+    int src_pos;                /* < 0: no source                           */
+    const float *src_wave;      /* host                                     */
+    int wave_len;
+    int nf;
+    const float *freq;          /* host                                     */
+    float dt;
+    fdtd_ftrans ft;
+    int nprobe;
+    int probe_k[FDTD_MAX_PROBES];
+    float *probe_out;
+    int probe_len;
+} fd1d_dform_desc;
+int fd1d_dform_run(const fd1d_dform_desc *d, int t0, int nsteps, void *stream);
+
 /* ------------------------------------------ plane wave (TF/SF) and DFT
  * The incident plane wave of fd2d/pycuda/fd2d_3_3.py:80-169 and
  * fd2d_3_4.py:121-226: a 1-D grid along y (ezi, hxi: ny cells) with the
  * two-step boundary (bc[4]) at both ends, and the total-field/scattered-
  * field corrections on the box of rows and columns npml-1 .. n-npml.
  * Serial semantics of fd2d/clang/test_3_4.c:38-147, FP32, bit-exact.       */
-#define FDTD_MAX_FREQ 16
-/* Replaces the device struct `ftrans` (fd2d/pycuda/fd2d_3_4.py:100-103):
- * r_pt/i_pt are nf planes of nx*ny (plane n at n*nx*ny), r_in/i_in nf.     */
-typedef struct fdtd_ftrans {
-    float *r_pt, *i_pt;
-    float *r_in, *i_in;
-} fdtd_ftrans;
 
 /* `ezinct(ny, ezi, hxi, bc)` (fd2d_3_4.py:140-148): ezi update, then the
  * boundary at both ends (one workgroup: every update precedes them).        */

@@ -163,3 +163,72 @@ def tfsf_run(state, inc, nsteps, npml, pml, naz, nbz, inc_wave, freq=(), dt=0.0,
         hfield(nx, ny, pml, state["ez"], state["ihx"], state["ihy"], state["hx"], state["hy"])
         incthx(nx, ny, npml, inc["ezi"], state["hx"])
         incthy(nx, ny, npml, inc["ezi"], state["hy"])
+
+
+# ------------------------------------------------ 1-D D/E form (lossy, Debye)
+# fd1d/clang/test_2_3.c:26-90 (FP32), fd1d/clang/fd1d_2_3.c (FP64),
+# fd1d/clang/test_2_2.c (lossy medium, no relaxation term).
+
+def dielectric_dform(nx, dt, chi, tau, epsr, sigma, debye, dtype):
+    """test_2_3.c:75-90 / test_2_2.c:73-86 in the program's precision: free
+    space below nx/2; ncx = exp(-dt/tau) in double, stored."""
+    import math
+    f = np.dtype(dtype).type
+    dt, chi, tau, epsr, sigma = f(dt), f(chi), f(tau), f(epsr), f(sigma)
+    eps0 = f(8.854e-12)
+    md = {"nax": np.ones(nx, dtype), "nbx": np.zeros(nx, dtype)}
+    lo = nx // 2
+    if debye:
+        md["ncx"] = np.zeros(nx, dtype)
+        md["ndx"] = np.zeros(nx, dtype)
+        md["nax"][lo:] = f(1) / (epsr + sigma * dt / eps0 + chi * dt / tau)
+        md["ncx"][lo:] = f(math.exp(float(-dt / tau)))
+        md["ndx"][lo:] = chi * dt / tau
+    else:
+        md["nax"][lo:] = f(1) / (epsr + sigma * dt / eps0)
+    md["nbx"][lo:] = sigma * dt / eps0
+    return md
+
+
+def dxfield(dx, hy, src_pos, src_val):
+    """test_2_3.c:46-52: update for i in [1, nx), then the additive source."""
+    h = dx.dtype.type(0.5)
+    dx[1:] += h * (hy[:-1] - hy[1:])
+    if src_pos >= 1:
+        dx[src_pos] += src_val
+
+
+def exfield_dform(md, dx, ix, sx, ex):
+    """test_2_3.c:55-62 (Debye) / test_2_2.c:54-60 (lossy)."""
+    if "ncx" in md:
+        ex[1:] = md["nax"][1:] * (dx[1:] - ix[1:] - md["ncx"][1:] * sx[1:])
+        ix[1:] += md["nbx"][1:] * ex[1:]
+        sx[1:] = md["ncx"][1:] * sx[1:] + md["ndx"][1:] * ex[1:]
+    else:
+        ex[1:] = md["nax"][1:] * (dx[1:] - ix[1:])
+        ix[1:] += md["nbx"][1:] * ex[1:]
+
+
+def hyfield1d(ex, hy, bc):
+    """test_2_3.c:65-72: the two-step boundary at both ends, then H."""
+    nx = len(ex)
+    h = ex.dtype.type(0.5)
+    ex[0] = bc[0]; bc[0] = bc[1]; bc[1] = ex[1]
+    ex[nx-1] = bc[3]; bc[3] = bc[2]; bc[2] = ex[nx-2]
+    hy[:-1] += h * (ex[:-1] - ex[1:])
+
+
+def dform_run(st, md, nsteps, src_wave, freq=(), dt=0.0, ft=None, t0=0, src_pos=1, probes=()):
+    """The main loop of test_2_3.c (dxfield, exfield, fourier, hyfield) for
+    nsteps from t0; st {dx, ix, sx, ex, hy, bc}; returns the probe traces."""
+    out = np.zeros((len(probes), nsteps), st["ex"].dtype)
+    for s in range(nsteps):
+        t = t0 + s + 1
+        dxfield(st["dx"], st["hy"], src_pos, src_wave[t - 1])
+        exfield_dform(md, st["dx"], st["ix"], st["sx"], st["ex"])
+        if len(freq):
+            fourier(t, freq, dt, st["ex"], st["ex"], ft, probe=10)
+        hyfield1d(st["ex"], st["hy"], st["bc"])
+        for p, k in enumerate(probes):
+            out[p, s] = st["ex"][k]
+    return out

@@ -85,6 +85,25 @@ class FdtdFtrans(C.Structure):
     _fields_ = [(k, _f) for k in ("r_pt", "i_pt", "r_in", "i_in")]
 
 
+class FdtdMedium1d(C.Structure):
+    """fdtd_medium1d: nax, nbx, ncx, ndx (fd1d/pycuda/fd1d_2_3.py:68-71)."""
+    _fields_ = [(k, _f) for k in ("nax", "nbx", "ncx", "ndx")]
+
+
+class Fd1dDformDesc(C.Structure):
+    """fd1d_dform_desc: a 1-D D/E-form run (include/fdtd.h)."""
+    _fields_ = [
+        ("nx", C.c_int),
+        ("dx", _f), ("ix", _f), ("sx", _f), ("ex", _f), ("hy", _f), ("bc", _f),
+        ("md", FdtdMedium1d),
+        ("src_pos", C.c_int), ("src_wave", _f), ("wave_len", C.c_int),
+        ("nf", C.c_int), ("freq", _f), ("dt", C.c_float),
+        ("ft", FdtdFtrans),
+        ("nprobe", C.c_int), ("probe_k", C.c_int * MAX_PROBES),
+        ("probe_out", _f), ("probe_len", C.c_int),
+    ]
+
+
 class Fd2dTfsfDesc(C.Structure):
     """fd2d_tfsf_desc: incident plane wave and DFT of a fd2d_tfsf_run."""
     _fields_ = [
@@ -153,6 +172,11 @@ SIGNATURES = {
     "fd2d_batch_graph": (_I, [C.POINTER(Fd2dBatchDesc), _I, _I, _I, _I, _P, C.POINTER(C.c_void_p)]),
     "fdtd_graph_launch": (_I, [_P, _P]),
     "fdtd_graph_destroy": (_I, [_P]),
+    "fd1d_dxfield": (_I, [_I, _I, _P, _P, _I, _F, _P]),
+    "fd1d_exfield_dform": (_I, [_I, C.POINTER(FdtdMedium1d), _P, _P, _P, _P, _P]),
+    "fd1d_fourier": (_I, [_I, _I, _I, _F, C.POINTER(C.c_float), _P, C.POINTER(FdtdFtrans), _P]),
+    "fd1d_dielectric_dform": (_I, [_I, _F, _F, _F, _F, _F, C.POINTER(FdtdMedium1d), _P]),
+    "fd1d_dform_run": (_I, [C.POINTER(Fd1dDformDesc), _I, _I, _P]),
     "fd2d_ezinct": (_I, [_I, _P, _P, _P, _P]),
     "fd2d_dfield_inc": (_I, [_I, _I, _I, C.POINTER(FdtdPml), _P, _P, _P, _P, _I, _F, _P]),
     "fd2d_inctdz": (_I, [_I, _I, _I, _P, _P, _P]),

@@ -605,6 +605,99 @@ int fd2d_tm_run(const fd2d_tm_desc *d, int t0, int nsteps, void *stream)
     return 0;
 }
 
+// ------------------------------------------------ 1-D D/E form (lossy, Debye)
+
+static int medium1d_check(const char *fn, const fdtd_medium1d *md)
+{
+    if (!md || !md->nax || !md->nbx) return fail("%s: md->nax and md->nbx are required", fn);
+    if ((md->ncx != nullptr) != (md->ndx != nullptr)) return fail("%s: ncx and ndx go together", fn);
+    return 0;
+}
+
+int fd1d_dxfield(int t, int nx, float *dx, const float *hy, int src_pos, float src_val,
+                 void *stream)
+{
+    (void)t;
+    clear_error();
+    if (nx < 2) return fail("fd1d_dxfield: nx=%d", nx);
+    if (!dx || !hy) return fail("fd1d_dxfield: dx and hy are required");
+    if (src_pos >= nx) return fail("fd1d_dxfield: src_pos=%d outside the grid", src_pos);
+    return launch_dxfield(nx, dx, hy, src_pos, src_val, src_pos >= 1, S(stream));
+}
+
+int fd1d_exfield_dform(int nx, const fdtd_medium1d *md, const float *dx, float *ix, float *sx,
+                       float *ex, void *stream)
+{
+    clear_error();
+    if (nx < 2) return fail("fd1d_exfield_dform: nx=%d", nx);
+    int rc = medium1d_check("fd1d_exfield_dform", md);
+    if (rc) return rc;
+    if (!dx || !ix || !ex || (md->ncx && !sx)) return fail("fd1d_exfield_dform: missing field");
+    return launch_exfield_dform(nx, *md, dx, ix, sx, ex, S(stream));
+}
+
+int fd1d_fourier(int t, int nf, int nx, float dt, const float *freq, const float *ex,
+                 const fdtd_ftrans *ft, void *stream)
+{
+    clear_error();
+    if (nx < 11 || !ex) return fail("fd1d_fourier: nx=%d (probe ex[10]), ex required", nx);
+    if (nf < 1 || nf > FDTD_MAX_FREQ || !freq) return fail("fd1d_fourier: nf=%d, freq (host)", nf);
+    if (!ft || !ft->r_pt || !ft->i_pt) return fail("fd1d_fourier: ft->r_pt and ft->i_pt are required");
+    return launch_fourier1d(t, nf, nx, dt, freq, ex, 10, *ft, S(stream));
+}
+
+int fd1d_dielectric_dform(int nx, float dt, float chi, float tau, float epsr, float sigma,
+                          const fdtd_medium1d *md, void *stream)
+{
+    clear_error();
+    if (nx < 2) return fail("fd1d_dielectric_dform: nx=%d", nx);
+    int rc = medium1d_check("fd1d_dielectric_dform", md);
+    if (rc) return rc;
+    if (md->ncx && !(tau > 0.0f)) return fail("fd1d_dielectric_dform: tau=%g", tau);
+    return launch_debye(nx, dt, chi, tau, epsr, sigma, *md, S(stream));
+}
+
+int fd1d_dform_run(const fd1d_dform_desc *d, int t0, int nsteps, void *stream)
+{
+    clear_error();
+    if (!d) return fail("fd1d_dform_run: NULL descriptor");
+    const int nx = d->nx;
+    if (nx < 11) return fail("fd1d_dform_run: nx=%d", nx);
+    int rc = medium1d_check("fd1d_dform_run", &d->md);
+    if (rc) return rc;
+    if (!d->dx || !d->ix || !d->ex || !d->hy || !d->bc || (d->md.ncx && !d->sx))
+        return fail("fd1d_dform_run: dx, ix, ex, hy, bc (and sx with a Debye medium) are required");
+    if (t0 < 0 || nsteps < 0) return fail("fd1d_dform_run: t0=%d nsteps=%d", t0, nsteps);
+    if (d->src_pos >= nx) return fail("fd1d_dform_run: src_pos=%d", d->src_pos);
+    if (d->src_pos >= 1 && (!d->src_wave || d->wave_len < t0 + nsteps))
+        return fail("fd1d_dform_run: src_wave (host) must cover step %d", t0 + nsteps);
+    if (d->nf) {
+        if (d->nf < 0 || d->nf > FDTD_MAX_FREQ || !d->freq) return fail("fd1d_dform_run: nf=%d", d->nf);
+        if (!d->ft.r_pt || !d->ft.i_pt) return fail("fd1d_dform_run: DFT planes required");
+    }
+    if (d->nprobe < 0 || d->nprobe > FDTD_MAX_PROBES) return fail("fd1d_dform_run: nprobe=%d", d->nprobe);
+    if (d->nprobe) {
+        if (!d->probe_out || d->probe_len < t0 + nsteps) return fail("fd1d_dform_run: probe_out");
+        for (int p = 0; p < d->nprobe; ++p)
+            if (d->probe_k[p] < 0 || d->probe_k[p] >= nx) return fail("fd1d_dform_run: probe %d", p);
+    }
+    hipStream_t s = S(stream);
+    int zeros[FDTD_MAX_PROBES] = {0};
+    for (int k = 0; k < nsteps; ++k) {
+        const int t = t0 + k + 1;
+        const bool src = d->src_pos >= 1;
+        rc = launch_dxfield(nx, d->dx, d->hy, d->src_pos, src ? d->src_wave[t - 1] : 0.0f, src, s);
+        if (!rc) rc = launch_exfield_dform(nx, d->md, d->dx, d->ix, d->sx, d->ex, s);
+        if (!rc && d->nf) rc = launch_fourier1d(t, d->nf, nx, d->dt, d->freq, d->ex, 10, d->ft, s);
+        if (!rc) rc = launch_fd1d_hyfield<float>(nx, d->ex, d->hy, d->bc, s);
+        if (!rc && d->nprobe)
+            rc = launch_fd2d_probe(nx, d->ex, d->nprobe, zeros, d->probe_k, d->probe_out,
+                                   d->probe_len, t - 1, s);
+        if (rc) return rc;
+    }
+    return 0;
+}
+
 // ------------------------------------------------ plane wave (TF/SF) and DFT
 
 static int tfsf_box_check(const char *fn, int nx, int ny, int npml)

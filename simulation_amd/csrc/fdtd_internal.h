@@ -86,6 +86,15 @@ int launch_fd2d_energy_diff(int wn, int wm, int nya, int ia0, int ja0, const flo
                             const float *hxa, const float *hya, int nyb, int ib0, int jb0,
                             const float *ezb, const float *hxb, const float *hyb, double *work,
                             hipStream_t s);
+// 1-D D/E form (dform1d.hip)
+int launch_dxfield(int nx, float *dx, const float *hy, int src_pos, float src_val, int src_on,
+                   hipStream_t s);
+int launch_exfield_dform(int nx, const fdtd_medium1d &md, const float *dx, float *ix, float *sx,
+                         float *ex, hipStream_t s);
+int launch_fourier1d(int t, int nf, int nx, float dt, const float *freq, const float *ex, int probe,
+                     const fdtd_ftrans &ft, hipStream_t s);
+int launch_debye(int nx, float dt, float chi, float tau, float epsr, float sigma,
+                 const fdtd_medium1d &md, hipStream_t s);
 // plane wave (tfsf.hip)
 void dft_twiddle(int t, float freq, float dt, float *c, float *s);
 int launch_ezinct(int ny, float *ezi, const float *hxi, float *bc, int set_inc, int inc_pos,

@@ -168,3 +168,124 @@ def simulate(cfg: Fd1dConfig, device=None) -> dict:
         out["snap_ex"] = torch.stack(snaps_ex).cpu().numpy()
         out["snap_hy"] = torch.stack(snaps_hy).cpu().numpy()
     return out
+
+
+# ------------------------------------------------ D/E form, lossy and Debye media
+# fd1d/pycuda/fd1d_2_1.py .. fd1d_2_3.py
+
+from collections import namedtuple  # noqa: E402
+import ctypes as C  # noqa: E402
+
+from ._dev import fptr  # noqa: E402
+
+medium1d = namedtuple("medium", ("nax", "nbx", "ncx", "ndx"))
+ftrans = namedtuple("ftrans", ("r_pt", "i_pt", "r_in", "i_in"))
+
+
+def _md1(md: medium1d) -> _lib.FdtdMedium1d:
+    return _lib.FdtdMedium1d(fptr(md.nax), fptr(md.nbx), fptr(md.ncx), fptr(md.ndx))
+
+
+def _ft1(ft: ftrans) -> _lib.FdtdFtrans:
+    return _lib.FdtdFtrans(fptr(ft.r_pt), fptr(ft.i_pt), fptr(ft.r_in), fptr(ft.i_in))
+
+
+def dielectric_dform(nx: int, dt: float, chi: float, tau: float, epsr: float, sigma: float,
+                     debye: bool = True, device=None) -> medium1d:
+    """nax/nbx(/ncx/ndx) of fd1d/clang/test_2_3.c:75-90 (debye=False:
+    test_2_2.c's lossy medium): free space below nx/2."""
+    dev = device or require_gpu()
+    z = lambda: torch.empty(nx, dtype=torch.float32, device=dev)  # noqa: E731
+    md = medium1d(z(), z(), z() if debye else None, z() if debye else None)
+    call("fd1d_dielectric_dform", int(nx), float(dt), float(chi), float(tau), float(epsr),
+         float(sigma), C.byref(_md1(md)), stream())
+    return md
+
+
+def dxfield(t: int, nx: int, dx, hy, src_pos: int = 1, src_val: float | None = None) -> None:
+    """D update and the additive source dx[src_pos] += src_val, default
+    gaussian(t, 50, 10) at 1 (fd1d/pycuda/fd1d_2_3.py:96-102)."""
+    v = gaussian(t, 50, 10.0) if src_val is None else src_val
+    call("fd1d_dxfield", int(t), int(nx), ptr(dx), ptr(hy), int(src_pos), float(v), stream())
+
+
+def exfield_dform(nx: int, md: medium1d, dx, ix, sx, ex) -> None:
+    """E from D through the medium (fd1d/pycuda/fd1d_2_3.py:105-113)."""
+    call("fd1d_exfield_dform", int(nx), C.byref(_md1(md)), ptr(dx), ptr(ix), ptr(sx), ptr(ex),
+         stream())
+
+
+def fourier(t: int, nf: int, nx: int, dt: float, freq, ex, ft: ftrans) -> None:
+    """Running DFT of ex and of the probe ex[10] (fd1d/pycuda/fd1d_2_3.py:79-93)."""
+    fr = (C.c_float * nf)(*[float(f) for f in freq])
+    call("fd1d_fourier", int(t), int(nf), int(nx), float(dt), fr, ptr(ex), C.byref(_ft1(ft)),
+         stream())
+
+
+class DFormSolver:
+    """The 1-D D/E-form programs (fd1d/pycuda/fd1d_2_2.py, fd1d_2_3.py): a
+    pulse added at src_pos striking a lossy (or Debye) half space, with the
+    running DFT at freq, through fd1d_dform_run.  src_wave: host values of
+    the additive source for steps 1.. (default gaussian(t, 50, 10))."""
+
+    def __init__(self, nx, md: medium1d, nsteps_max, freq=(), dt=None, src_pos=1,
+                 src_wave=None, probes=(), device=None):
+        dev = device or require_gpu()
+        self.nx, self.md, self.device = int(nx), md, dev
+        z = lambda: torch.zeros(nx, dtype=torch.float32, device=dev)  # noqa: E731
+        self.dx, self.ix, self.ex, self.hy = z(), z(), z(), z()
+        self.sx = z() if md.ncx is not None else None
+        self.bc = torch.zeros(4, dtype=torch.float32, device=dev)
+        if src_wave is None:
+            src_wave = [gaussian(t, 50, 10.0) for t in range(1, nsteps_max + 1)]
+        self.src_wave = np.ascontiguousarray(np.asarray(src_wave, np.float32))
+        self.src_pos = int(src_pos)
+        self.freq = np.ascontiguousarray(np.asarray(freq, np.float32))
+        self.dt = float(np.float32(np.float32(0.01) / np.float32(6e8)) if dt is None else dt)
+        nf = len(self.freq)
+        self.ft = (ftrans(torch.zeros((nf, nx), dtype=torch.float32, device=dev),
+                          torch.zeros((nf, nx), dtype=torch.float32, device=dev),
+                          torch.zeros(nf, dtype=torch.float32, device=dev),
+                          torch.zeros(nf, dtype=torch.float32, device=dev)) if nf else None)
+        self.probes = [int(k) for k in probes]
+        self.probe_len = int(nsteps_max)
+        self.probe_out = (torch.zeros((len(probes), self.probe_len), dtype=torch.float32,
+                                      device=dev) if probes else None)
+        self.t = 0
+
+    def desc(self) -> _lib.Fd1dDformDesc:
+        d = _lib.Fd1dDformDesc()
+        d.nx = self.nx
+        for k in ("dx", "ix", "sx", "ex", "hy", "bc"):
+            setattr(d, k, fptr(getattr(self, k)))
+        d.md = _md1(self.md)
+        d.src_pos = self.src_pos
+        d.src_wave = self.src_wave.ctypes.data_as(C.POINTER(C.c_float))
+        d.wave_len = len(self.src_wave)
+        d.nf = len(self.freq)
+        d.freq = self.freq.ctypes.data_as(C.POINTER(C.c_float)) if d.nf else None
+        d.dt = self.dt
+        if self.ft is not None:
+            d.ft = _ft1(self.ft)
+        d.nprobe = len(self.probes)
+        for p, k in enumerate(self.probes):
+            d.probe_k[p] = k
+        d.probe_out = fptr(self.probe_out)
+        d.probe_len = self.probe_len
+        return d
+
+    def run(self, nsteps: int) -> None:
+        d = self.desc()
+        call("fd1d_dform_run", C.byref(d), int(self.t), int(nsteps), stream())
+        self.t += int(nsteps)
+
+    def amplitude_phase(self):
+        """(amplt, phase), (nf, nx): the reference's post-processing
+        (fd1d/pycuda/fd1d_2_3.py:219-220), host NumPy on the FP32 transforms."""
+        r_pt, i_pt = self.ft.r_pt.cpu().numpy(), self.ft.i_pt.cpu().numpy()
+        r_in, i_in = self.ft.r_in.cpu().numpy()[:, None], self.ft.i_in.cpu().numpy()[:, None]
+        return (1 / np.hypot(r_in, i_in) * np.hypot(r_pt, i_pt),
+                np.arctan2(i_pt, r_pt) - np.arctan2(i_in, r_in))
+
+
+__all__ += ["medium1d", "dielectric_dform", "dxfield", "exfield_dform", "fourier", "DFormSolver"]

@@ -62,3 +62,18 @@ def tfsf_case(name: str, gaussian):
     return dict(nx=meta["nx"], ny=meta["ny"], ns=meta["ns"], npml=meta["npml"], dtype=dt,
                 pml=pml, naz=a["in_naz"], nbz=a["in_nbz"], freq=a["in_freq"],
                 dt=dt(meta["dt"]), inc_pos=inc["pos"], wave=wave, meta=meta, exp=a)
+
+
+DFORM_CASES = ("fd1d_debye_f32", "fd1d_debye_f64", "fd1d_lossy_f32", "fd1d_lossy_f64")
+
+
+def dform_case(name: str, gaussian):
+    """A 1-D D/E-form fixture (fd1d/clang/test_2_3.c or test_2_2.c main loop)."""
+    meta, a = golden(name)
+    dt = np.float32 if meta["dtype"] == "float32" else np.float64
+    src = meta["source"]
+    wave = np.array([gaussian(t, src["t0"], src["spread"], dt) for t in range(1, meta["ns"] + 1)],
+                    dt)
+    return dict(nx=meta["nx"], ns=meta["ns"], dtype=dt, debye=meta["debye"], freq=a["in_freq"],
+                dt=dt(meta["dt"]), src_pos=src["pos"], wave=wave, probes=tuple(meta["probes"]),
+                meta=meta, exp=a)

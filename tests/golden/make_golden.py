@@ -20,6 +20,9 @@ and non-square grids):
                                                   probe ex[150], snapshots every 100 steps
   fd1d_1_3_{f32,f64}   test_1_3.c / fd1d_1_3.c   nx=200, 300 steps, cb, additive Gaussian
   fd1d_1_5_{f32,f64}   test_1_5.c / fd1d_1_5.c   nx=200, 400 steps, ca/cb, additive sinusoid
+  fd1d_debye_{f32,f64} test_2_3.c / fd1d_2_3.c   nx=300, 600 steps, D/E form, Debye half space,
+                                                  additive Gaussian at 1, DFT at 3 frequencies
+  fd1d_lossy_{f32,f64} test_2_2.c / fd1d_2_2.c   the same with the lossy half space (no Debye)
   fd2d_3_1_{f32,f64}   test_3_1.c / fd2d_3_1.c   60x70, 60 steps, free space
   fd2d_3_2_{f32,f64}   test_3_2.c / fd2d_3_2.c   64x80, npml=8, 100 steps, PML, sinusoid
   fd2d_c2_{f32,f64}    test_3_4.c / fd2d_3_4.c   97x131, npml=8, 150 steps: PML dfield/hfield,
@@ -297,6 +300,60 @@ def case_fd2d_tfsf(tag, prog, prog64, dt, nx, ny, ns, npml, rgrid, epsr, sigma, 
          **{f"out_ft_{k}": v for k, v in ft.items()})
 
 
+def case_fd1d_dform(tag, prog, dt, nx, ns, debye, freqs, probes):
+    """The 1-D D/E-form program's main loop (dxfield with its additive
+    gaussian(t, 50, 10) at cell 1, exfield, fourier, hyfield) with its own
+    dielectric: lossy (test_2_2.c / fd1d_2_2.c) or Debye (test_2_3.c /
+    fd1d_2_3.c), running DFT at freqs."""
+    lib = load(prog)
+    R = real(dt)
+    ptr = C.POINTER(R)
+    keys = ("nax", "nbx", "ncx", "ndx") if debye else ("nax", "nbx")
+
+    class Medium(C.Structure):
+        _fields_ = [(k, ptr) for k in keys]
+
+    class Ftrans(C.Structure):
+        _fields_ = [(k, ptr) for k in ("r_pt", "i_pt", "r_in", "i_in")]
+    if dt == np.float32:
+        dtv = np.float32(np.float32(0.01) / np.float32(6e8))
+    else:
+        dtv = np.float64(0.01 / 6e8)
+    chi, tau, epsr, sigma = 2.0, 0.001e-6, 2.0, 0.01
+    lib.dielectric.restype = Medium
+    if debye:
+        md = lib.dielectric(C.c_int(nx), R(dtv), R(chi), R(tau), R(epsr), R(sigma))
+    else:
+        md = lib.dielectric(C.c_int(nx), R(dtv), R(epsr), R(sigma))
+    mda = {k: np.ctypeslib.as_array(getattr(md, k), shape=(nx,)).copy() for k in keys}
+    nf = len(freqs)
+    freq = np.asarray(freqs, dt)
+    ft = {"r_pt": np.zeros((nf, nx), dt), "i_pt": np.zeros((nf, nx), dt),
+          "r_in": np.zeros(nf, dt), "i_in": np.zeros(nf, dt)}
+    fts = Ftrans(*[P(ft[k]) for k in ("r_pt", "i_pt", "r_in", "i_in")])
+    f = {k: np.zeros(nx, dt) for k in ("dx", "ix", "sx", "ex", "hy")}
+    bc = np.zeros(4, dt)
+    probe_out = np.zeros((len(probes), ns), dt)
+    I = C.c_int
+    for t in range(1, ns + 1):
+        lib.dxfield(I(t), I(nx), P(f["dx"]), P(f["hy"]))
+        if debye:
+            lib.exfield(I(nx), C.byref(md), P(f["dx"]), P(f["ix"]), P(f["sx"]), P(f["ex"]))
+        else:
+            lib.exfield(I(nx), C.byref(md), P(f["dx"]), P(f["ix"]), P(f["ex"]))
+        lib.fourier(I(t), I(nf), I(nx), R(dtv), P(freq), P(f["ex"]), C.byref(fts))
+        lib.hyfield(I(nx), P(f["ex"]), P(f["hy"]), P(bc))
+        for p, k in enumerate(probes):
+            probe_out[p, t - 1] = f["ex"][k]
+    meta = dict(program=prog, nx=nx, ns=ns, debye=debye, dt=float(dtv), chi=chi, tau=tau,
+                epsr=epsr, sigma=sigma, dtype=str(np.dtype(dt)), freqs=list(freqs),
+                source=dict(kind="gaussian", pos=1, t0=50, spread=10.0, mode="add"),
+                dft_probe=10, probes=list(probes))
+    outs = {f"out_{k}": v for k, v in f.items() if debye or k != "sx"}
+    save(tag, meta, **{f"out_md_{k}": v for k, v in mda.items()}, in_freq=freq, **outs,
+         out_bc=bc, out_probe=probe_out, **{f"out_ft_{k}": v for k, v in ft.items()})
+
+
 def case_dielectric_c(tag, nx, ny, npml, rgrid, epsr, sigma):
     lib = load("fd2d_3_4")
     M64 = medium_struct(np.float64)
@@ -330,6 +387,10 @@ def main():
         case_fd1d(f"{p}_1_5", f"fd1d_1_5_{suf}", dt, 200, 400, "cacb", probes=(150,),
                   extra=dict(epsr=4.0, sigma=0.04))
         q = "test" if dt == np.float32 else "fd2d"
+        case_fd1d_dform(f"fd1d_debye_{suf}", f"{p}_2_3", dt, 300, 600, True,
+                        (50e6, 200e6, 500e6), probes=(100, 200))
+        case_fd1d_dform(f"fd1d_lossy_{suf}", f"{p}_2_2", dt, 300, 600, False,
+                        (50e6, 200e6, 500e6), probes=(100, 200))
         case_fd2d_3_1(f"fd2d_3_1_{suf}", f"{q}_3_1", dt, 60, 70, 60,
                       probes=[(30, 35), (10, 20), (45, 60)])
         case_fd2d_3_2(f"fd2d_3_2_{suf}", f"{q}_3_2", dt, 64, 80, 100, 8,

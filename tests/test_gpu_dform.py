@@ -1,0 +1,75 @@
+"""1-D D/E form with lossy and Debye media and the 1-D running DFT on the GPU
+against the reference's own programs (fixtures tests/golden/fd1d_debye_f32,
+fd1d_lossy_f32 from fd1d/clang/test_2_3.c, test_2_2.c).  Bar: bit-exact."""
+import numpy as np
+import pytest
+import torch
+
+from cases import dform_case
+
+pytestmark = pytest.mark.gpu
+
+CASES = ("fd1d_debye_f32", "fd1d_lossy_f32")
+
+
+def _gauss(t, t0, spread, dt):
+    from simulation_amd import fd1d
+    return fd1d.gaussian(t, t0, spread, dt)
+
+
+def _setup(c, dev):
+    from simulation_amd import fd1d
+    m = c["meta"]
+    md = fd1d.dielectric_dform(c["nx"], float(c["dt"]), m["chi"], m["tau"], m["epsr"], m["sigma"],
+                               debye=c["debye"], device=dev)
+    s = fd1d.DFormSolver(c["nx"], md, c["ns"], freq=c["freq"], dt=float(c["dt"]),
+                         src_pos=c["src_pos"], src_wave=c["wave"], probes=c["probes"], device=dev)
+    return s, md
+
+
+def _check(c, s):
+    e = c["exp"]
+    torch.cuda.synchronize()
+    for k in ("dx", "ix", "ex", "hy", "bc") + (("sx",) if c["debye"] else ()):
+        assert np.array_equal(getattr(s, k).cpu().numpy(), e["out_" + k]), k
+    for k in ("r_pt", "i_pt", "r_in", "i_in"):
+        assert np.array_equal(getattr(s.ft, k).cpu().numpy(), e["out_ft_" + k]), k
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_dform_coefficients_bit_exact(gpu, name):
+    c = dform_case(name, _gauss)
+    _, md = _setup(c, gpu)
+    for k in ("nax", "nbx", "ncx", "ndx"):
+        if "out_md_" + k in c["exp"]:
+            assert np.array_equal(getattr(md, k).cpu().numpy(), c["exp"]["out_md_" + k]), k
+
+
+@pytest.mark.parametrize("name", CASES)
+@pytest.mark.parametrize("chunks", [(600,), (1, 299, 300)])
+def test_dform_run_bit_exact(gpu, name, chunks):
+    c = dform_case(name, _gauss)
+    print(f"[dform] {name} chunks={chunks}", flush=True)
+    s, _ = _setup(c, gpu)
+    for n in chunks:
+        s.run(n)
+    _check(c, s)
+    torch.cuda.synchronize()
+    assert np.array_equal(s.probe_out.cpu().numpy(), c["exp"]["out_probe"])
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_dform_step_kernels_bit_exact(gpu, name):
+    from simulation_amd import fd1d
+    c = dform_case(name, _gauss)
+    print(f"[dform] per-step {name}", flush=True)
+    s, md = _setup(c, gpu)
+    nx, nf = c["nx"], len(c["freq"])
+    for t in range(1, c["ns"] + 1):
+        fd1d.dxfield(t, nx, s.dx, s.hy, c["src_pos"], float(c["wave"][t - 1]))
+        fd1d.exfield_dform(nx, md, s.dx, s.ix, s.sx, s.ex)
+        fd1d.fourier(t, nf, nx, float(c["dt"]), c["freq"], s.ex, s.ft)
+        fd1d.hyfield(nx, s.ex, s.hy, s.bc)
+    _check(c, s)
+    a, p = s.amplitude_phase()
+    assert a.shape == (nf, nx) and np.all(np.isfinite(a))

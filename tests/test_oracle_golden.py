@@ -3,7 +3,8 @@ programs (tests/golden/make_golden.py): bit-exact, FP32 and FP64."""
 import numpy as np
 import pytest
 
-from cases import FD1D_CASES, FD2D_CASES, TFSF_CASES, fd1d_case, fd2d_case, tfsf_case
+from cases import (DFORM_CASES, FD1D_CASES, FD2D_CASES, TFSF_CASES, dform_case, fd1d_case,
+                   fd2d_case, tfsf_case)
 
 
 def _wave(O):
@@ -109,3 +110,30 @@ def test_tfsf_oracle_matches_reference(oracle, name):
         assert np.array_equal(v, e["out_ft_" + k]), k
     # the plane wave crossed the box and the cylinder scattered it
     assert np.abs(e["out_ez"]).max() > 0.1 and np.abs(e["out_ft_r_pt"]).max() > 1.0
+
+
+@pytest.mark.parametrize("name", DFORM_CASES)
+def test_dform_oracle_matches_reference(oracle, name):
+    """1-D D/E form with a lossy or Debye half space and the running DFT: the
+    NumPy restatement (coefficients included) against the reference
+    program's own functions, bit-exact."""
+    from oracle import numpy_oracle as NO
+    c = dform_case(name, oracle.gaussian)
+    dt, nx, m = c["dtype"], c["nx"], c["meta"]
+    md = NO.dielectric_dform(nx, c["dt"], m["chi"], m["tau"], m["epsr"], m["sigma"], c["debye"], dt)
+    e = c["exp"]
+    for k, v in md.items():
+        assert np.array_equal(v, e["out_md_" + k]), k
+    st = {k: np.zeros(nx, dt) for k in ("dx", "ix", "sx", "ex", "hy")}
+    st["bc"] = np.zeros(4, dt)
+    nf = len(c["freq"])
+    ft = {"r_pt": np.zeros((nf, nx), dt), "i_pt": np.zeros((nf, nx), dt),
+          "r_in": np.zeros(nf, dt), "i_in": np.zeros(nf, dt)}
+    pr = NO.dform_run(st, md, c["ns"], c["wave"], c["freq"], c["dt"], ft, src_pos=c["src_pos"],
+                      probes=c["probes"])
+    for k in ("dx", "ix", "ex", "hy", "bc") + (("sx",) if c["debye"] else ()):
+        assert np.array_equal(st[k], e["out_" + k]), k
+    for k, v in ft.items():
+        assert np.array_equal(v, e["out_ft_" + k]), k
+    assert np.array_equal(pr, e["out_probe"])
+    assert np.abs(e["out_ex"]).max() > 0.1
