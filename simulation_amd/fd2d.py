@@ -455,8 +455,11 @@ def build_solver(cfg: TMConfig, device=None) -> TMSolver:
                     depth=cfg.depth, rows=cfg.rows, cols=cfg.cols, device=dev)
 
 
-def simulate(cfg: TMConfig, device=None, snapshot_fields=("ez", "hy")) -> dict:
-    """Run cfg; returns final state, probe traces and the snapshots."""
+def simulate(cfg: TMConfig, device=None, snapshot_fields=("ez", "hy"), out_dir=None) -> dict:
+    """Run cfg; returns final state, probe traces and the snapshots.  With
+    out_dir, each snapshot is also written as <field>_t<step>.npy and the
+    probe traces as probes.csv (simulation_amd.store)."""
+    from . import store
     s = build_solver(cfg, device)
     snaps = {k: [] for k in snapshot_fields}
     snap_t = []
@@ -466,7 +469,13 @@ def simulate(cfg: TMConfig, device=None, snapshot_fields=("ez", "hy")) -> dict:
         if cfg.snap_every and s.t % cfg.snap_every == 0:
             for k in snapshot_fields:
                 snaps[k].append(getattr(s, k).cpu().numpy())
+                if out_dir:
+                    store.write_snapshot(out_dir, k, s.t, snaps[k][-1])
             snap_t.append(s.t)
     out = dict(state=s.state(), probe=s.probe_traces(), snap_t=snap_t,
                snapshots={k: np.stack(v) for k, v in snaps.items() if v})
+    if out_dir and out["probe"] is not None:
+        import os
+        store.write_probes_csv(os.path.join(out_dir, "probes.csv"), out["probe"],
+                               [f"ez[{i};{j}]" for i, j in s.probes])
     return out

@@ -1,0 +1,112 @@
+"""Checkpoint / resume on the GPU: a solver restored from a checkpoint at step
+t and advanced by n steps holds the same bits as the original advanced from
+t by n (the schemes are deterministic), for the 2-D TM solver (PML, lossy
+medium, sources, probes), the plane-wave solver (incident grid, DFT), the
+learned-boundary solver (history) and the 1-D D/E-form solver."""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+
+def _snap(fields):
+    torch.cuda.synchronize()
+    return {k: (None if v is None else v.detach().cpu().numpy().copy()) for k, v in fields.items()}
+
+
+def _same(a, b):
+    assert a.keys() == b.keys()
+    for k in a:
+        if a[k] is None:
+            assert b[k] is None, k
+        else:
+            assert np.array_equal(a[k], b[k]), k
+
+
+def _roundtrip(make, first, then, tmp_path):
+    from simulation_amd import store
+    s = make()
+    s.run(first)
+    path = str(tmp_path / "ck.npz")
+    store.save_checkpoint(path, s)
+    s.run(then)
+    want = _snap(store._fields(s)[1])
+    r = make()
+    t = store.load_checkpoint(path, r)
+    assert t == first
+    r.run(then)
+    _same(_snap(store._fields(r)[1]), want)
+    return r
+
+
+def test_tm_checkpoint_resume(gpu, tmp_path):
+    from simulation_amd import fd1d, fd2d
+    n, ns = 96, 140
+    print("[store] tm", flush=True)
+
+    def make():
+        pml = fd2d.new_pmlayer(n, n, gpu)
+        fd2d.pmlparam(n, n, 8, pml)
+        md = fd2d.dielectric(n, n, 8, 0, fd2d.time_step(), 1.0, 0.0, gpu,
+                             cylinders=[(50.0, 40.0, 12.0, 4.0, 0.02)])
+        w = fd1d.waveform("gaussian", ns, 0, np.float32, center=20, spread=6.0)
+        return fd2d.TMSolver(n, n, pml, md, [(48, 48, w)], [(40, 40), (60, 20)], nsteps_max=ns,
+                             device=gpu)
+    r = _roundtrip(make, 61, 79, tmp_path)
+    assert np.abs(r.probe_traces()).max() > 1e-3
+
+
+def test_plane_wave_checkpoint_resume(gpu, tmp_path):
+    from simulation_amd import fd2d
+    n, ns = 80, 150
+    print("[store] plane wave", flush=True)
+
+    def make():
+        pml = fd2d.new_pmlayer(n, n, gpu)
+        fd2d.pmlparam(n, n, 8, pml)
+        md = fd2d.dielectric(n, n, 8, 10, fd2d.time_step(), 30.0, 0.3, gpu)
+        return fd2d.PlaneWaveSolver(n, n, 8, pml, md, ns, freq=(50e6, 300e6, 700e6), device=gpu)
+    r = _roundtrip(make, 70, 80, tmp_path)
+    assert r.ft.r_pt.abs().max().item() > 0.1
+
+
+def test_learned_checkpoint_resume(gpu, tmp_path):
+    from simulation_amd import fd2d, learned
+    n = 96
+    print("[store] learned", flush=True)
+    net = learned.load(device=gpu)
+
+    def make():
+        md = fd2d.dielectric(n, n, 0, 0, fd2d.time_step(), 1.0, 0.0, gpu)
+        s = learned.LearnedTMSolver(n, n, net, md, [], [], nsteps_max=200, device=gpu)
+        s.tm.dz.copy_(learned.pulse(n, n, 40.0, 52.0, 3.0, gpu))
+        return s
+    r = _roundtrip(make, 60, 90, tmp_path)
+    assert r.hist.abs().max().item() > 1e-3
+
+
+def test_dform_checkpoint_resume(gpu, tmp_path):
+    from simulation_amd import fd1d
+    nx, ns = 400, 700
+    print("[store] dform", flush=True)
+    dt = float(np.float32(np.float32(0.01) / np.float32(6e8)))
+
+    def make():
+        md = fd1d.dielectric_dform(nx, dt, 2.0, 0.001e-6, 2.0, 0.01, device=gpu)
+        return fd1d.DFormSolver(nx, md, ns, freq=(50e6, 200e6, 500e6), dt=dt, probes=(100, 300),
+                                device=gpu)
+    r = _roundtrip(make, 333, 300, tmp_path)
+    assert r.ex.abs().max().item() > 0.05
+
+
+def test_checkpoint_rejects_another_solver(gpu, tmp_path):
+    from simulation_amd import fd1d, store
+    dt = float(np.float32(np.float32(0.01) / np.float32(6e8)))
+    md = fd1d.dielectric_dform(300, dt, 2.0, 0.001e-6, 2.0, 0.01, device=gpu)
+    a = fd1d.DFormSolver(300, md, 10, device=gpu)
+    store.save_checkpoint(str(tmp_path / "a.npz"), a)
+    md2 = fd1d.dielectric_dform(301, dt, 2.0, 0.001e-6, 2.0, 0.01, device=gpu)
+    b = fd1d.DFormSolver(301, md2, 10, device=gpu)
+    with pytest.raises(ValueError, match="shape"):
+        store.load_checkpoint(str(tmp_path / "a.npz"), b)
