@@ -342,8 +342,13 @@ def bench_tfsf(a):
     1024^2, npml=80, cylinder of radius 1.5 m (eps_r 30, sigma 0.3), plane
     wave through the TF/SF box, running DFT at 50, 300 and 700 MHz.  One
     step = ezinct, D update, inctdz, E update, fourier, hxinct, H update,
-    incthx, incthy (fd2d_tfsf_run).  Algorithmic bytes per cell-update: the
-    lossy PML state (60 B) plus R+W of the 2 x 3 DFT planes (48 B)."""
+    incthx, incthy (fd2d_tfsf_run, fused: per 64 steps one incident-grid
+    launch, then temporally blocked launches that apply the TF/SF
+    corrections and accumulate the DFT).  Algorithmic bytes per
+    cell-update: the lossy PML state (60 B) plus R+W of the 2 x 3 DFT planes
+    (48 B) -- what one step of the unfused form moves, so `achieved` is an
+    effective rate (the blocked launches move it once per K steps).  The
+    per-step kernels (work = NULL) are timed beside it."""
     import numpy as np
     import torch
     from simulation_amd import fd2d
@@ -357,7 +362,8 @@ def bench_tfsf(a):
     pml = fd2d.new_pmlayer(n, n, dev)
     fd2d.pmlparam(n, n, npml, pml)
     md = fd2d.dielectric(n, n, npml, rgrid, dt, 30.0, 0.30, dev)
-    s = fd2d.PlaneWaveSolver(n, n, npml, pml, md, W + K, freq=freq, dt=dt, device=dev)
+    s = fd2d.PlaneWaveSolver(n, n, npml, pml, md, W + K, freq=freq, dt=dt, device=dev,
+                             depth=a.depth, rows=a.rows, cols=a.cols)
     s.run(W)
     torch.cuda.synchronize()
     st = torch.cuda.current_stream()
@@ -370,6 +376,17 @@ def bench_tfsf(a):
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     step_ms = e0.elapsed_time(e1) / K
+    # the per-step kernels on the same workload, a tenth of the steps
+    ps = fd2d.PlaneWaveSolver(n, n, npml, pml, md, W + K, freq=freq, dt=dt, device=dev,
+                              fused=False)
+    kp = max(K // 10, 10)
+    ps.run(min(W, 10))
+    torch.cuda.synchronize()
+    tp = time.perf_counter()
+    ps.run(kp)
+    torch.cuda.synchronize()
+    per_step = n * n * kp / (time.perf_counter() - tp) / 1e6
+    del ps
     b_alg = 60 + 16 * len(freq)
     achieved = b_alg * n * n / (step_ms * 1e-3) / 1e9
     cpu = None if a.no_cpu else cpu_baseline_tfsf(n, a.cpu_seconds, npml, rgrid, freq)
@@ -381,10 +398,12 @@ def bench_tfsf(a):
         "config": {"workload": f"2-D TM plane wave {n}x{n} FP32, npml={npml} PML, TF/SF box, "
                                f"cylinder r={rgrid} eps_r=30 sigma=0.3, DFT at 3 frequencies "
                                f"(fd2d/clang/test_3_4.c)", "grid": [n, n], "npml": npml,
-                   "freq": list(freq)},
+                   "freq": list(freq), "depth": s.tm.depth or 8, "rows": s.tm.rows, "cols": s.tm.cols or 1,
+                   "per_step_kernels_value": round(per_step, 1)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4), "traffic": None,
-                     "kernel": "per-step kernels of fd2d_tfsf_run (one step)",
+                     "kernel": "fused fd2d_tfsf_run, effective per step (tm_tb_kernel "
+                               "with TF/SF + DFT, both parts, and the incident chunk)",
                      "avg_launch_ms": round(step_ms, 4), "alg_bytes_per_launch": b_alg * n * n},
         "cpu_baseline": cpu,
     }

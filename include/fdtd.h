@@ -524,7 +524,15 @@ typedef struct fd2d_tfsf_desc {
     float dt;
     fdtd_ftrans ft;
     int dft_probe;              /* incident probe cell (6 in the reference)  */
+    float *work;                /* NULL: the per-step kernels; else fused     */
+    size_t work_elems;          /* >= fd2d_tfsf_work_elems(ny, npml)          */
 } fd2d_tfsf_desc;
+/* With work, the run is fused: per chunk of up to 64 steps the incident grid
+ * advances in one workgroup and records what the box needs per step into
+ * work; the temporally blocked launches (d's partner set and launch shape,
+ * as fd2d_tm_run) apply the TF/SF corrections and accumulate the DFT (lossy
+ * PML form, nf <= 3), with the same bits as the per-step path.              */
+size_t fd2d_tfsf_work_elems(int ny, int npml);
 int fd2d_tfsf_run(const fd2d_tm_desc *d, const fd2d_tfsf_desc *b, int t0, int nsteps,
                   void *stream);
 

@@ -112,6 +112,7 @@ class Fd2dTfsfDesc(C.Structure):
         ("inc_pos", C.c_int), ("inc_wave", _f), ("inc_len", C.c_int),
         ("nf", C.c_int), ("freq", _f), ("dt", C.c_float),
         ("ft", FdtdFtrans), ("dft_probe", C.c_int),
+        ("work", _f), ("work_elems", C.c_size_t),
     ]
 
 
@@ -185,6 +186,7 @@ SIGNATURES = {
     "fd2d_incthy": (_I, [_I, _I, _I, _P, _P, _P]),
     "fd2d_fourier": (_I, [_I, _I, _I, _I, _F, C.POINTER(C.c_float), _P, _P, C.POINTER(FdtdFtrans), _P]),
     "fd2d_tfsf_run": (_I, [C.POINTER(Fd2dTmDesc), C.POINTER(Fd2dTfsfDesc), _I, _I, _P]),
+    "fd2d_tfsf_work_elems": (C.c_size_t, [_I, _I]),
     "lbc_hist_elems": (C.c_size_t, [_I, _I]),
     "lbc_work_bytes": (C.c_size_t, [_I, _I]),
     "fd2d_lbc_margin": (_I, []),

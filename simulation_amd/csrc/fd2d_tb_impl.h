@@ -63,6 +63,10 @@ __device__ __forceinline__ float from_right(float v)
 
 template <int C> struct HRow { float hx[C], hy[C], ix[C], iy[C]; };
 template <int C> struct DRow { float dz[C], iz[C], na[C], nb[C]; };
+// Running-DFT accumulators of a row (NF frequencies; none when NF = 0): they
+// follow the D row through the levels, so each DFT plane is read and written
+// once per launch.
+template <int C, int NF> struct FRow { float r[NF ? NF : 1][C], i[NF ? NF : 1][C]; };
 
 // Registers one time level carries from one sweep iteration to the next.
 template <int C>
@@ -185,14 +189,16 @@ __device__ __forceinline__ void col_coef(const TbParams &P, int j0, ColCoef<C> &
 // p-s and the H update of row p-s-1.  BROW: rows may lie outside [1, nx-2] or
 // outside the default band; BCOL: the same for the strip's columns; SP: a
 // source or probe lies in the rows and columns the wave computes.
-template <int K, int C, bool LOSSY, bool PML, bool VEC, bool BROW, bool BCOL, bool SP>
+template <int K, int C, bool LOSSY, bool PML, bool VEC, bool BROW, bool BCOL, bool SP, int NF>
 __device__ __forceinline__ void iterate(const TbParams &P, const Geom &g, const ColCoef<C> &cc,
                                         int p, const HRow<C> &Hload, const DRow<C> &Dload,
-                                        Carry<C> (&St)[K])
+                                        Carry<C> (&St)[K], const FRow<C, NF> &Fload,
+                                        FRow<C, NF> (&SF)[K])
 {
     const int nx = P.nx, ny = P.ny;
     HRow<C> Hin = Hload;
     DRow<C> Dpend;                        // this iteration's D output of the previous level
+    FRow<C, NF> Fpend;
 #pragma unroll
     for (int s = 0; s < K; ++s) {
         const int r = p - s;              // E row of this level
@@ -201,11 +207,17 @@ __device__ __forceinline__ void iterate(const TbParams &P, const Geom &g, const 
         // D input: the previous level's output of the last iteration, which
         // is then replaced by its output of this iteration
         DRow<C> Din;
+        FRow<C, NF> Fin;
         if (s == 0) {
             Din = Dload;
+            if constexpr (NF > 0) Fin = Fload;
         } else {
             Din = St[s - 1].d;
             St[s - 1].d = Dpend;
+            if constexpr (NF > 0) {
+                Fin = SF[s - 1];
+                SF[s - 1] = Fpend;
+            }
         }
         const HRow<C> Hp = St[s].h;
         float gx2r = 1.f, gx3r = 1.f, fx1q = 0.f, fx2q = 1.f, fx3q = 1.f;
@@ -237,6 +249,17 @@ __device__ __forceinline__ void iterate(const TbParams &P, const Geom &g, const 
                     if (g.j0 + c == P.src_j[k]) Dn.dz[c] = v;
             }
         }
+        // plane wave: the TF/SF correction of dz on the box's columns
+        // (fd2d/pycuda/fd2d_3_4.py:164-170), after the D update
+        if (SP && P.tf_npml && r >= P.tf_npml - 1 && r <= nx - P.tf_npml) {
+            const float *iv = P.inc + (size_t)(t - P.inc_t0 - 1) * P.inc_stride;
+#pragma unroll
+            for (int c = 0; c < C; ++c) {
+                const int j = g.j0 + c;
+                if (j == P.tf_npml - 1) Dn.dz[c] = Dn.dz[c] + 0.5f * iv[0];
+                if (j == ny - P.tf_npml) Dn.dz[c] = Dn.dz[c] - 0.5f * iv[1];
+            }
+        }
 #pragma unroll
         for (int c = 0; c < C; ++c) {
             const float dz = Dn.dz[c];
@@ -251,6 +274,16 @@ __device__ __forceinline__ void iterate(const TbParams &P, const Geom &g, const 
             Dn.na[c] = Din.na[c];
             Dn.nb[c] = Din.nb[c];
             En[c] = ez;
+        }
+        FRow<C, NF> Fn;
+        if constexpr (NF > 0) {
+#pragma unroll
+            for (int f = 0; f < NF; ++f)
+#pragma unroll
+                for (int c = 0; c < C; ++c) {
+                    Fn.r[f][c] = Fin.r[f][c] + P.tw_c[s][f] * En[c];
+                    Fn.i[f][c] = Fin.i[f][c] - P.tw_s[s][f] * En[c];
+                }
         }
         const float enr = from_right(En[0]);
         if (SP && P.nprobe && r >= g.i0 && r < g.i1) {
@@ -294,6 +327,30 @@ __device__ __forceinline__ void iterate(const TbParams &P, const Geom &g, const 
             }
             Hout.hx[c] = hx; Hout.hy[c] = hy; Hout.ix[c] = ax; Hout.iy[c] = ay;
         }
+        // plane wave: TF/SF corrections of hx on the box's side columns and
+        // of hy on its top and bottom rows (fd2d/pycuda/fd2d_3_4.py:211-226)
+        if (SP && P.tf_npml) {
+            const int a = P.tf_npml - 1, b = ny - P.tf_npml, qb = nx - P.tf_npml;
+            const float *iv = P.inc + (size_t)(t - P.inc_t0 - 1) * P.inc_stride + 2 - a;
+            if (q >= a && q <= qb) {
+#pragma unroll
+                for (int c = 0; c < C; ++c) {
+                    const int j = g.j0 + c;
+                    if (j == P.tf_npml - 2) Hout.hx[c] = Hout.hx[c] + 0.5f * iv[a];
+                    if (j == b) Hout.hx[c] = Hout.hx[c] - 0.5f * iv[b];
+                }
+            }
+            if (q == P.tf_npml - 2 || q == qb) {
+#pragma unroll
+                for (int c = 0; c < C; ++c) {
+                    const int j = g.j0 + c;
+                    if (j >= a && j <= b) {
+                        if (q == P.tf_npml - 2) Hout.hy[c] = Hout.hy[c] - 0.5f * iv[j];
+                        if (q == qb) Hout.hy[c] = Hout.hy[c] + 0.5f * iv[j];
+                    }
+                }
+            }
+        }
         // carry this level's rows to the next iteration
         St[s].h = Hin;
 #pragma unroll
@@ -302,6 +359,7 @@ __device__ __forceinline__ void iterate(const TbParams &P, const Geom &g, const 
         if (s + 1 < K) {
             Hin = Hout;          // input of the next level
             Dpend = Dn;
+            if constexpr (NF > 0) Fpend = Fn;
         } else {
             // last level: D/E of row r and H of row q are final for this launch
             // (cells within skip_w of an edge are left to another kernel)
@@ -312,6 +370,15 @@ __device__ __forceinline__ void iterate(const TbParams &P, const Geom &g, const 
                 st<C, VEC>(P.dz_o, row, g.j0, c0, c1, Dn.dz);
                 if (LOSSY) st<C, VEC>(P.iz_o, row, g.j0, c0, c1, Dn.iz);
                 if (P.ez_o) st<C, VEC>(P.ez_o, row, g.j0, c0, c1, En);
+                if constexpr (NF > 0) {
+                    const size_t plane = (size_t)P.nx * (size_t)ny;
+#pragma unroll
+                    for (int f = 0; f < NF; ++f)
+                        if (f < P.nf) {
+                            st<C, VEC>(P.ft_r + f * plane, row, g.j0, c0, c1, Fn.r[f]);
+                            st<C, VEC>(P.ft_i + f * plane, row, g.j0, c0, c1, Fn.i[f]);
+                        }
+                }
             }
             if (q >= g.i0 && q < g.i1 && q >= sw && q < P.nx - sw) {
                 const unsigned row = (unsigned)q * (unsigned)ny;
@@ -326,12 +393,41 @@ __device__ __forceinline__ void iterate(const TbParams &P, const Geom &g, const 
     }
 }
 
-template <int K, int C, bool LOSSY, bool PML, bool VEC, bool BROW, bool BCOL, bool SP>
+// DFT accumulators of row p (planes f < P.nf; rows outside the grid read an
+// in-bounds row, never stored)
+template <int C, int NF, bool VEC, bool CLAMP>
+__device__ __forceinline__ void load_ft(const TbParams &P, int p, int j0, FRow<C, NF> &F)
+{
+    const int pc = min(max(p, 0), P.nx - 1);
+    const unsigned row = (unsigned)pc * (unsigned)P.ny;
+    const size_t plane = (size_t)P.nx * (size_t)P.ny;
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+        if (f < P.nf) {
+            ld<C, VEC, CLAMP>(P.ft_r + f * plane, row, j0, P.ny, F.r[f]);
+            ld<C, VEC, CLAMP>(P.ft_i + f * plane, row, j0, P.ny, F.i[f]);
+        } else {
+#pragma unroll
+            for (int c = 0; c < C; ++c) { F.r[f][c] = 0.f; F.i[f][c] = 0.f; }
+        }
+    }
+}
+
+template <int K, int C, bool LOSSY, bool PML, bool VEC, bool BROW, bool BCOL, bool SP, int NF = 0>
 __device__ __forceinline__ void sweep(const TbParams &P, const Geom &g)
 {
     ColCoef<C> cc;
     col_coef<C, PML, BCOL>(P, g.j0, cc);
     Carry<C> St[K];
+    FRow<C, NF> SF[K];
+    if constexpr (NF > 0) {
+#pragma unroll
+        for (int s = 0; s < K; ++s)
+#pragma unroll
+            for (int f = 0; f < NF; ++f)
+#pragma unroll
+                for (int c = 0; c < C; ++c) { SF[s].r[f][c] = 0.f; SF[s].i[f][c] = 0.f; }
+    }
 #pragma unroll
     for (int s = 0; s < K; ++s) {
 #pragma unroll
@@ -347,12 +443,16 @@ __device__ __forceinline__ void sweep(const TbParams &P, const Geom &g)
     // row p+1 is in flight while row p is computed
     HRow<C> H0, H1;
     DRow<C> D0, D1;
+    FRow<C, NF> F0, F1;
     load_row<C, LOSSY, PML, VEC, BCOL>(P, pbeg, g.j0, H0, D0);
+    if constexpr (NF > 0) load_ft<C, NF, VEC, BCOL>(P, pbeg, g.j0, F0);
     for (int p = pbeg; p <= pend; ++p) {
         load_row<C, LOSSY, PML, VEC, BCOL>(P, p + 1, g.j0, H1, D1);
-        iterate<K, C, LOSSY, PML, VEC, BROW, BCOL, SP>(P, g, cc, p, H0, D0, St);
+        if constexpr (NF > 0) load_ft<C, NF, VEC, BCOL>(P, p + 1, g.j0, F1);
+        iterate<K, C, LOSSY, PML, VEC, BROW, BCOL, SP, NF>(P, g, cc, p, H0, D0, St, F0, SF);
         H0 = H1;
         D0 = D1;
+        if constexpr (NF > 0) F0 = F1;
     }
 }
 
@@ -361,7 +461,7 @@ __device__ __forceinline__ void sweep(const TbParams &P, const Geom &g)
 // wave returns at once if it belongs to the other part, so the two kernels
 // write disjoint cells and may run concurrently; each gets the register
 // allocation of its own body.
-template <int K, int C, bool LOSSY, bool PML, bool VEC, int PART>
+template <int K, int C, bool LOSSY, bool PML, bool VEC, int PART, int NF = 0>
 __global__ void __launch_bounds__(64) tm_tb_kernel(const TbParams P)
 {
     constexpr int S = 64 * C;
@@ -384,12 +484,22 @@ __global__ void __launch_bounds__(64) tm_tb_kernel(const TbParams P)
     for (int k = 0; k < P.nprobe; ++k)
         sp |= P.probe_i[k] >= g.i0 && P.probe_i[k] < g.i1 && P.probe_j[k] >= js &&
               P.probe_j[k] < js + S;
+    if (P.tf_npml) {
+        // a TF/SF-corrected cell in the rows and columns this wave computes:
+        // rows npml-2 .. nx-npml on columns npml-2, npml-1, ny-npml, or rows
+        // npml-2 and nx-npml on columns npml-1 .. ny-npml
+        const int m = P.tf_npml, rlo = g.i0 - 2 * K, rhi = g.i1 + K;
+        auto row_in = [&](int r) { return r >= rlo && r <= rhi; };
+        auto col_in = [&](int j) { return j >= js && j < js + S; };
+        sp |= rhi >= m - 2 && rlo <= P.nx - m && (col_in(m - 2) || col_in(m - 1) || col_in(P.ny - m));
+        sp |= (row_in(m - 2) || row_in(P.nx - m)) && js + S > m - 1 && js <= P.ny - m;
+    }
     const bool interior = !brow && !bcol && !sp;
     if (PART == 0) {
-        if (interior) sweep<K, C, LOSSY, PML, VEC, false, false, false>(P, g);
+        if (interior) sweep<K, C, LOSSY, PML, VEC, false, false, false, NF>(P, g);
     } else if (!interior) {
-        if (!brow && !sp) sweep<K, C, LOSSY, PML, VEC, false, true, false>(P, g);
-        else sweep<K, C, LOSSY, PML, VEC, true, true, true>(P, g);
+        if (!brow && !sp) sweep<K, C, LOSSY, PML, VEC, false, true, false, NF>(P, g);
+        else sweep<K, C, LOSSY, PML, VEC, true, true, true, NF>(P, g);
     }
 }
 
@@ -399,6 +509,13 @@ int launch_kv(const TbParams &p, bool lossy, bool pml, hipStream_t s)
     constexpr int S = 64 * C;
     const int w = S - 2 * p.halo;
     const dim3 grid((p.ny + w - 1) / w, (p.nx + p.rows - 1) / p.rows);
+    if (p.nf) {
+        if (!(lossy && pml) || p.nf > TB_MAX_FREQ)
+            return fail("fused running DFT: lossy PML form and at most %d frequencies", TB_MAX_FREQ);
+        hipLaunchKernelGGL((tm_tb_kernel<K, C, true, true, VEC, PART, TB_MAX_FREQ>), grid, dim3(64),
+                           0, s, p);
+        return hip_check(hipGetLastError(), "fd2d_tm_run launch");
+    }
     if (lossy && pml) hipLaunchKernelGGL((tm_tb_kernel<K, C, true, true, VEC, PART>), grid, dim3(64), 0, s, p);
     else if (lossy) hipLaunchKernelGGL((tm_tb_kernel<K, C, true, false, VEC, PART>), grid, dim3(64), 0, s, p);
     else if (pml) hipLaunchKernelGGL((tm_tb_kernel<K, C, false, true, VEC, PART>), grid, dim3(64), 0, s, p);
@@ -429,6 +546,9 @@ tm_tb_batch_kernel(const fd2d_grid *__restrict__ grids, const int *__restrict__ 
         const int gi = units[4 * u], sx = units[4 * u + 1], by = units[4 * u + 2];
         const fd2d_grid &G = grids[gi];
         TbParams P;
+        P.tf_npml = 0;                        // no plane wave, no DFT in a batch
+        P.nf = 0;
+        P.inc = nullptr;
         P.nx = G.nx;
         P.ny = G.ny;
         const bool swap = parity != 0;        // odd launches read the partner set

@@ -783,6 +783,103 @@ int fd2d_fourier(int t, int nf, int nx, int ny, float dt, const float *freq, con
     return launch_fourier(t, nf, nx, ny, dt, freq, ezi, 6, ez, *ft, S(stream));
 }
 
+size_t fd2d_tfsf_work_elems(int ny, int npml)
+{
+    if (npml < 2 || ny - 2 * npml + 2 < 1) return 0;
+    return (size_t)INC_CHUNK * (size_t)(ny - 2 * npml + 4);
+}
+
+// The fused plane-wave run: per chunk of up to INC_CHUNK steps, the incident
+// grid advances in one workgroup (inc_chunk_kernel, writing the table the
+// 2-D launches read), then the temporally blocked launches apply the TF/SF
+// corrections and accumulate the DFT (as fd2d_tm_run otherwise).
+static int tfsf_fused(const fd2d_tm_desc *d, const fd2d_tfsf_desc *b, int t0, int nsteps,
+                      hipStream_t s0)
+{
+    const bool lossy = d->md.nbz != nullptr;
+    const bool pml = pml_all(&d->pml);
+    const int stride = d->ny - 2 * b->npml + 4;
+    Aux *ax = nullptr;
+    int rc = aux_for_current_device(&ax);
+    if (rc) return rc;
+    int done = 0;
+    while (done < nsteps) {
+        const int n = nsteps - done < INC_CHUNK ? nsteps - done : INC_CHUNK;
+        const int tc = t0 + done;                  // steps tc+1 .. tc+n
+        Plan pl;
+        rc = plan_tm(d, n, &pl);
+        if (rc) return rc;
+        if (pl.tail_unfused || pl.launches == 0) {
+            // too short for an even number of launches: the per-step kernels
+            fd2d_tfsf_desc one = *b;
+            one.work = nullptr;
+            rc = fd2d_tfsf_run(d, &one, tc, n, s0);
+            if (rc) return rc;
+            done += n;
+            continue;
+        }
+        IncChunk ic{};
+        ic.n = n; ic.npml = b->npml; ic.stride = stride; ic.inc_pos = b->inc_pos;
+        ic.probe = b->dft_probe; ic.nf = b->nf;
+        for (int k = 0; k < n; ++k) {
+            ic.wave[k] = b->inc_wave[tc + k];
+            for (int f = 0; f < b->nf; ++f) dft_twiddle(tc + k + 1, b->freq[f], b->dt, &ic.c[k][f], &ic.s[k][f]);
+        }
+        rc = launch_inc_chunk(d->ny, b->ezi, b->hxi, b->bc, b->nf ? b->ft.r_in : nullptr,
+                              b->nf ? b->ft.i_in : nullptr, b->work, ic, s0);
+        if (rc) return rc;
+        TbParams p{};
+        p.nx = d->nx; p.ny = d->ny;
+        p.naz = d->md.naz; p.nbz = d->md.nbz;
+        if (pml) p.pml = to_ptrs(d->pml);
+        p.halo = pl.halo; p.rows = pl.rows;
+        if (pml) {
+            const int *br = d->band_rows, *bc = d->band_cols;
+            p.rlo = br[1] > br[0] ? (br[0] > 1 ? br[0] : 1) : 0;
+            p.rhi = br[1] > br[0] ? (br[1] < d->nx - 1 ? br[1] : d->nx - 1) : 0;
+            p.clo = bc[1] > bc[0] ? (bc[0] > 1 ? bc[0] : 1) : 0;
+            p.chi = bc[1] > bc[0] ? (bc[1] < d->ny - 1 ? bc[1] : d->ny - 1) : 0;
+        } else {
+            p.rlo = 1; p.rhi = d->nx - 1; p.clo = 1; p.chi = d->ny - 1;
+        }
+        p.nsrc = d->nsrc;
+        for (int k = 0; k < d->nsrc; ++k) { p.src_i[k] = d->src_i[k]; p.src_j[k] = d->src_j[k]; }
+        p.src_wave = d->src_wave; p.wave_len = d->wave_len;
+        p.nprobe = d->nprobe;
+        for (int k = 0; k < d->nprobe; ++k) { p.probe_i[k] = d->probe_i[k]; p.probe_j[k] = d->probe_j[k]; }
+        p.probe_out = d->probe_out; p.probe_len = d->probe_len;
+        p.tf_npml = b->npml; p.inc = b->work; p.inc_t0 = tc; p.inc_stride = stride;
+        p.nf = b->nf; p.ft_r = b->ft.r_pt; p.ft_i = b->ft.i_pt;
+        float *A[6] = {d->dz, d->iz, d->hx, d->hy, d->ihx, d->ihy};
+        float *B[6] = {d->dz2, d->iz2, d->hx2, d->hy2, d->ihx2, d->ihy2};
+        const int L = pl.launches;
+        int dl = 0;
+        for (int l = 0; l < L; ++l) {
+            const int k = n / L + (l < n % L ? 1 : 0);
+            float **in = (l % 2 == 0) ? A : B;
+            float **out = (l % 2 == 0) ? B : A;
+            p.dz_i = in[0]; p.iz_i = in[1]; p.hx_i = in[2]; p.hy_i = in[3]; p.ihx_i = in[4]; p.ihy_i = in[5];
+            p.dz_o = out[0]; p.iz_o = out[1]; p.hx_o = out[2]; p.hy_o = out[3]; p.ihx_o = out[4]; p.ihy_o = out[5];
+            p.ez_o = (l == L - 1) ? d->ez : nullptr;
+            p.t_first = tc + dl + 1;
+            for (int q = 0; q < k; ++q)
+                for (int f = 0; f < b->nf; ++f) dft_twiddle(p.t_first + q, b->freq[f], b->dt, &p.tw_c[q][f], &p.tw_s[q][f]);
+            rc = hip_check(hipEventRecord(ax->fork, s0), "fork event");
+            if (!rc) rc = hip_check(hipStreamWaitEvent(ax->s, ax->fork, 0), "fork wait");
+            if (!rc) rc = launch_fd2d_tb(p, k, pl.cols, lossy, pml, 1, ax->s);
+            if (!rc && l > 0) rc = hip_check(hipStreamWaitEvent(s0, ax->join, 0), "join wait");
+            if (!rc) rc = launch_fd2d_tb(p, k, pl.cols, lossy, pml, 0, s0);
+            if (!rc) rc = hip_check(hipEventRecord(ax->join, ax->s), "join event");
+            if (rc) return rc;
+            dl += k;
+        }
+        rc = hip_check(hipStreamWaitEvent(s0, ax->join, 0), "join wait");
+        if (rc) return rc;
+        done += n;
+    }
+    return 0;
+}
+
 int fd2d_tfsf_run(const fd2d_tm_desc *d, const fd2d_tfsf_desc *b, int t0, int nsteps, void *stream)
 {
     clear_error();
@@ -800,6 +897,17 @@ int fd2d_tfsf_run(const fd2d_tm_desc *d, const fd2d_tfsf_desc *b, int t0, int ns
         if (rc) return rc;
     }
     const bool pml = pml_all(&d->pml);
+    if (b->work) {
+        if (b->work_elems < fd2d_tfsf_work_elems(d->ny, b->npml))
+            return fail("fd2d_tfsf_run: work has %zu floats, %zu needed", b->work_elems,
+                        fd2d_tfsf_work_elems(d->ny, b->npml));
+        if (b->nf > TB_MAX_FREQ || (b->nf && !(pml && d->md.nbz)))
+            return fail("fd2d_tfsf_run: the fused path takes the lossy PML form and at most %d "
+                        "frequencies (pass work = NULL for the per-step path)", TB_MAX_FREQ);
+        if (!d->dz2 || !d->hx2 || !d->hy2 || (d->md.nbz && !d->iz2) || (pml && (!d->ihx2 || !d->ihy2)))
+            return fail("fd2d_tfsf_run: the fused path needs the partner state set");
+        return tfsf_fused(d, b, t0, nsteps, S(stream));
+    }
     PmlPtrs pp = to_ptrs(d->pml);
     hipStream_t s = S(stream);
     for (int k = 0; k < nsteps; ++k) {

@@ -47,6 +47,9 @@ struct CylList {
 };
 
 // Arguments of one temporally blocked launch (fd2d_tb.hip).
+constexpr int TB_MAX_DEPTH = 8;     // temporally blocked launches: steps per launch
+constexpr int TB_MAX_FREQ = 3;      // fused running DFT: frequencies
+
 struct TbParams {
     int nx, ny;
     const float *dz_i, *iz_i, *hx_i, *hy_i, *ihx_i, *ihy_i;
@@ -68,6 +71,17 @@ struct TbParams {
     float *probe_out;
     int probe_len;
     int skip_w;                    // leave cells within skip_w of an edge unwritten (0: none)
+    // plane wave: TF/SF box rows/columns tf_npml-1 .. n-tf_npml (0: none);
+    // step t reads inc[(t - inc_t0 - 1) * inc_stride + k]: k = 0, 1 the
+    // incident hx beside the box (hxi[npml-2], hxi[ny-npml]), then the
+    // incident ez over the box columns npml-1 .. ny-npml
+    int tf_npml, inc_t0, inc_stride;
+    const float *inc;
+    // running DFT of ez into nf <= TB_MAX_FREQ planes (0: none); tw_c/tw_s
+    // [s][f]: twiddles of level s of the launch
+    int nf;
+    float *ft_r, *ft_i;
+    float tw_c[TB_MAX_DEPTH][TB_MAX_FREQ], tw_s[TB_MAX_DEPTH][TB_MAX_FREQ];
 };
 
 // Arguments of one learned-boundary frame launch (lbc_frame.hip).
@@ -96,6 +110,14 @@ int launch_fourier1d(int t, int nf, int nx, float dt, const float *freq, const f
 int launch_debye(int nx, float dt, float chi, float tau, float epsr, float sigma,
                  const fdtd_medium1d &md, hipStream_t s);
 // plane wave (tfsf.hip)
+constexpr int INC_CHUNK = 64;       // steps of incident grid per fused chunk
+struct IncChunk {
+    int n, npml, stride, inc_pos, probe, nf;
+    float wave[INC_CHUNK];
+    float c[INC_CHUNK][TB_MAX_FREQ], s[INC_CHUNK][TB_MAX_FREQ];
+};
+int launch_inc_chunk(int ny, float *ezi, float *hxi, float *bc, float *r_in, float *i_in,
+                     float *table, const IncChunk &a, hipStream_t s);
 void dft_twiddle(int t, float freq, float dt, float *c, float *s);
 int launch_ezinct(int ny, float *ezi, const float *hxi, float *bc, int set_inc, int inc_pos,
                   float inc_val, hipStream_t s);

@@ -113,7 +113,110 @@ __global__ void __launch_bounds__(256) fourier_kernel(long n, Twiddle tw, const 
 
 inline int blocks(long n, int b) { return (int)((n + b - 1) / b); }
 
+// The incident grid for a chunk of steps (one workgroup: every phase waits
+// for the previous), recording per step what the fused 2-D launches read:
+// table[s] = {hxi[npml-2], hxi[ny-npml] (before this step's hxinct),
+// ezi[npml-1 .. ny-npml] (after ezinct and the source)}; also the incident
+// probe's DFT (r_in, i_in; the reference's fourier after efield).
+__global__ void __launch_bounds__(1024) inc_chunk_kernel(int ny, float *ezi, float *hxi, float *bc,
+                                                         float *r_in, float *i_in, float *table,
+                                                         IncChunk a)
+{
+    const int tid = threadIdx.x, m = a.npml, nbox = ny - 2 * m + 2;
+    for (int s = 0; s < a.n; ++s) {
+        for (int j = tid + 1; j < ny; j += blockDim.x) ezi[j] = ezi[j] + 0.5f * (hxi[j - 1] - hxi[j]);
+        __syncthreads();
+        if (tid == 0) {
+            ezi[0] = bc[0];
+            bc[0] = bc[1];
+            bc[1] = ezi[1];
+            ezi[ny - 1] = bc[3];
+            bc[3] = bc[2];
+            bc[2] = ezi[ny - 2];
+            ezi[a.inc_pos] = a.wave[s];
+        }
+        __syncthreads();
+        float *row = table + (size_t)s * a.stride;
+        if (tid == 0) {
+            row[0] = hxi[m - 2];
+            row[1] = hxi[ny - m];
+            if (r_in) {
+                const float e = ezi[a.probe];
+                for (int f = 0; f < a.nf; ++f) {
+                    r_in[f] = r_in[f] + a.c[s][f] * e;
+                    i_in[f] = i_in[f] - a.s[s][f] * e;
+                }
+            }
+        }
+        for (int k = tid; k < nbox; k += blockDim.x) row[2 + k] = ezi[m - 1 + k];
+        __syncthreads();
+        for (int j = tid; j < ny - 1; j += blockDim.x) hxi[j] = hxi[j] + 0.5f * (ezi[j] - ezi[j + 1]);
+        __syncthreads();
+    }
+}
+
+// The same with the incident grid held in LDS for the whole chunk (ezi,
+// hxi: 2 ny floats; bc in registers of thread 0): per step only barriers, no
+// global round trips; the table rows go out as plain stores.
+__global__ void __launch_bounds__(1024) inc_chunk_lds_kernel(int ny, float *ezi, float *hxi,
+                                                             float *bc, float *r_in, float *i_in,
+                                                             float *table, IncChunk a)
+{
+    extern __shared__ float sm[];
+    float *e = sm, *h = sm + ny;
+    const int tid = threadIdx.x, m = a.npml, nbox = ny - 2 * m + 2;
+    for (int j = tid; j < ny; j += blockDim.x) { e[j] = ezi[j]; h[j] = hxi[j]; }
+    float b0 = 0.f, b1 = 0.f, b2 = 0.f, b3 = 0.f;
+    if (tid == 0) { b0 = bc[0]; b1 = bc[1]; b2 = bc[2]; b3 = bc[3]; }
+    __syncthreads();
+    for (int s = 0; s < a.n; ++s) {
+        for (int j = tid + 1; j < ny; j += blockDim.x) e[j] = e[j] + 0.5f * (h[j - 1] - h[j]);
+        __syncthreads();
+        if (tid == 0) {
+            e[0] = b0;
+            b0 = b1;
+            b1 = e[1];
+            e[ny - 1] = b3;
+            b3 = b2;
+            b2 = e[ny - 2];
+            e[a.inc_pos] = a.wave[s];
+        }
+        __syncthreads();
+        float *row = table + (size_t)s * a.stride;
+        if (tid == 0) {
+            row[0] = h[m - 2];
+            row[1] = h[ny - m];
+            if (r_in) {
+                const float v = e[a.probe];
+                for (int f = 0; f < a.nf; ++f) {
+                    r_in[f] = r_in[f] + a.c[s][f] * v;
+                    i_in[f] = i_in[f] - a.s[s][f] * v;
+                }
+            }
+        }
+        for (int k = tid; k < nbox; k += blockDim.x) row[2 + k] = e[m - 1 + k];
+        __syncthreads();
+        for (int j = tid; j < ny - 1; j += blockDim.x) h[j] = h[j] + 0.5f * (e[j] - e[j + 1]);
+        __syncthreads();
+    }
+    for (int j = tid; j < ny; j += blockDim.x) { ezi[j] = e[j]; hxi[j] = h[j]; }
+    if (tid == 0) { bc[0] = b0; bc[1] = b1; bc[2] = b2; bc[3] = b3; }
+}
+
 }  // namespace
+
+int launch_inc_chunk(int ny, float *ezi, float *hxi, float *bc, float *r_in, float *i_in,
+                     float *table, const IncChunk &a, hipStream_t s)
+{
+    const size_t lds = 2 * (size_t)ny * sizeof(float);
+    if (lds <= 64 * 1024)
+        hipLaunchKernelGGL(inc_chunk_lds_kernel, dim3(1), dim3(1024), lds, s, ny, ezi, hxi, bc, r_in,
+                           i_in, table, a);
+    else
+        hipLaunchKernelGGL(inc_chunk_kernel, dim3(1), dim3(1024), 0, s, ny, ezi, hxi, bc, r_in, i_in,
+                           table, a);
+    return hip_check(hipGetLastError(), "incident chunk");
+}
 
 // The reference's twiddle: cosf/sinf of 2*M_PI*freq*dt*t evaluated in double
 // and narrowed to float (fd2d/clang/test_3_4.c:42-43).

@@ -201,9 +201,10 @@ class PlaneWaveSolver:
     DFT frequencies (none: no DFT)."""
 
     def __init__(self, nx, ny, npml, pml, md, nsteps_max, freq=(), dt=None, inc_pos=3,
-                 inc_wave=None, probes=(), device=None):
+                 inc_wave=None, probes=(), device=None, fused=True, depth=0, rows=0, cols=0):
         from .fd1d import gaussian
-        self.tm = TMSolver(nx, ny, pml, md, (), probes, nsteps_max, device=device)
+        self.tm = TMSolver(nx, ny, pml, md, (), probes, nsteps_max, depth, rows, cols,
+                           device=device)
         dev = self.tm.device
         self.npml = int(npml)
         self.ezi = torch.zeros(ny, dtype=torch.float32, device=dev)
@@ -216,6 +217,12 @@ class PlaneWaveSolver:
         self.freq = np.ascontiguousarray(np.asarray(freq, np.float32))
         self.dt = float(time_step() if dt is None else dt)
         self.ft = new_ftrans(len(self.freq), nx, ny, dev) if len(self.freq) else None
+        # fused: the temporally blocked launches with the incident table in
+        # work (they take the DFT in the lossy PML form, up to 3 frequencies)
+        nf = len(self.freq)
+        fused = fused and (nf == 0 or (nf <= 3 and pml is not None and md.nbz is not None))
+        n = _lib.lib().fd2d_tfsf_work_elems(int(ny), int(npml)) if fused else 0
+        self.work = torch.empty(n, dtype=torch.float32, device=dev) if n else None
 
     @property
     def t(self) -> int:
@@ -234,6 +241,8 @@ class PlaneWaveSolver:
         if self.ft is not None:
             b.ft = _ft_struct(self.ft)
         b.dft_probe = 6
+        b.work = fptr(self.work)
+        b.work_elems = 0 if self.work is None else self.work.numel()
         return b
 
     def run(self, nsteps: int) -> None:

@@ -55,6 +55,71 @@ def test_plane_wave_run_bit_exact(gpu, chunks):
     _check(c, s)
 
 
+@pytest.mark.parametrize("nf,depth,cols", [(3, 0, 0), (0, 0, 0), (2, 3, 2), (1, 5, 1)])
+def test_fused_matches_per_step(gpu, nf, depth, cols):
+    """The fused path (incident chunks + temporally blocked launches carrying
+    the TF/SF corrections and the DFT) against the per-step kernels, on a
+    grid with many strips and row blocks, a lossy cylinder, probes, and runs
+    split into chunks of 1 (per-step inside), 70 (two incident chunks) and
+    99 steps."""
+    from simulation_amd import fd2d
+    nx, ny, npml, ns = 300, 262, 12, 170
+    freq = (50e6, 300e6, 700e6)[:nf]
+    print(f"[tfsf] fused vs per-step nf={nf} depth={depth} cols={cols}", flush=True)
+
+    def make(fused):
+        pml = fd2d.new_pmlayer(nx, ny, gpu)
+        fd2d.pmlparam(nx, ny, npml, pml)
+        md = fd2d.dielectric(nx, ny, npml, 0, fd2d.time_step(), 1.0, 0.0, gpu,
+                             cylinders=[(140.0, 120.0, 30.0, 30.0, 0.3)])
+        return fd2d.PlaneWaveSolver(nx, ny, npml, pml, md, ns, freq=freq, probes=[(150, 131), (20, 30)],
+                                    device=gpu, fused=fused, depth=depth, cols=cols, rows=40)
+    a, b = make(True), make(False)
+    assert a.work is not None and b.work is None
+    for n in (1, 70, 99):
+        a.run(n)
+        b.run(n)
+    sa, sb = a.state(), b.state()
+    for k in ("dz", "iz", "ez", "hx", "hy", "ihx", "ihy"):
+        assert np.array_equal(sa[k], sb[k]), k
+    for k in ("ezi", "hxi", "bc"):
+        assert torch.equal(getattr(a, k), getattr(b, k)), k
+    assert torch.equal(a.tm.probe_out, b.tm.probe_out)
+    if nf:
+        for k in ("r_pt", "i_pt", "r_in", "i_in"):
+            assert torch.equal(getattr(a.ft, k), getattr(b.ft, k)), k
+        assert a.ft.r_pt.abs().max().item() > 0.1
+    assert np.abs(sa["ez"]).max() > 0.1
+
+
+def test_fused_wide_grid(gpu):
+    """ny > 8192: the incident chunk runs from global memory (two ny-float
+    arrays no longer fit the workgroup's LDS); still the per-step bits."""
+    from simulation_amd import fd2d
+    nx, ny, npml, ns = 40, 8250, 8, 90
+    print(f"[tfsf] fused wide grid {nx}x{ny}", flush=True)
+
+    def make(fused):
+        pml = fd2d.new_pmlayer(nx, ny, gpu)
+        fd2d.pmlparam(nx, ny, npml, pml)
+        md = fd2d.dielectric(nx, ny, npml, 0, fd2d.time_step(), 1.0, 0.0, gpu,
+                             cylinders=[(20.0, 60.0, 6.0, 4.0, 0.1)])
+        return fd2d.PlaneWaveSolver(nx, ny, npml, pml, md, ns, freq=(300e6,), device=gpu,
+                                    fused=fused)
+    a, b = make(True), make(False)
+    for n in (70, 20):
+        a.run(n)
+        b.run(n)
+    sa, sb = a.state(), b.state()
+    for k in ("dz", "iz", "ez", "hx", "hy", "ihx", "ihy"):
+        assert np.array_equal(sa[k], sb[k]), k
+    for k in ("ezi", "hxi", "bc"):
+        assert torch.equal(getattr(a, k), getattr(b, k)), k
+    for k in ("r_pt", "i_pt", "r_in", "i_in"):
+        assert torch.equal(getattr(a.ft, k), getattr(b.ft, k)), k
+    assert np.abs(sa["ez"]).max() > 0.1
+
+
 def test_plane_wave_step_kernels_bit_exact(gpu):
     """The per-step drop-ins, called in the reference's order."""
     from simulation_amd import fd2d
