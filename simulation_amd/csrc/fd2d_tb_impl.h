@@ -43,6 +43,8 @@
 // + 4 B ez on the last launch of a run), for K cell-updates.
 #pragma once
 
+#include <algorithm>
+
 #include "fdtd_internal.h"
 
 namespace fdtd {
@@ -465,10 +467,53 @@ template <int K, int C, bool LOSSY, bool PML, bool VEC, int PART, int NF = 0>
 __global__ void __launch_bounds__(64) tm_tb_kernel(const TbParams P)
 {
     constexpr int S = 64 * C;
-    const int js = (int)blockIdx.x * (S - 2 * P.halo) - P.halo;   // first column of the strip
+    int bx, by;
+    if (PART == 1 && P.n_rb >= 0) {
+        // compact boundary launch: listed row blocks, then listed strips,
+        // then single tiles; a tile listed twice is left to its first entry
+        const int w = S - 2 * P.halo;
+        const int nsx = (P.ny + w - 1) / w, nrb = (P.nx + P.rows - 1) / P.rows;
+        int i = (int)blockIdx.x;
+        auto in_rb = [&](int b) {
+            for (int k = 0; k < P.n_rb; ++k)
+                if (P.rb_list[k] == b) return true;
+            return false;
+        };
+        auto in_sx = [&](int x) {
+            for (int k = 0; k < P.n_sx; ++k)
+                if (P.sx_list[k] == x) return true;
+            return false;
+        };
+        if (i < P.n_rb * nsx) {
+            by = P.rb_list[i / nsx];
+            bx = i % nsx;
+        } else if ((i -= P.n_rb * nsx) < P.n_sx * nrb) {
+            bx = P.sx_list[i / nrb];
+            by = i % nrb;
+            if (in_rb(by)) return;
+        } else {
+            i -= P.n_sx * nrb;
+            bx = P.t_list[i] >> 16;
+            by = P.t_list[i] & 0xffff;
+            if (in_rb(by) || in_sx(bx)) return;
+        }
+    } else {
+        // XCD-aware tile order: the dispatcher hands workgroup L (x fastest)
+        // to XCD L % 8.  The workgroups of one XCD take a contiguous run of
+        // tiles, strips fastest, so neighbouring strips -- which read each
+        // other's halo columns at the same rows at about the same time -- and
+        // the row blocks below them share that XCD's L2.
+        const unsigned nt = gridDim.x * gridDim.y;
+        const unsigned L = blockIdx.x + blockIdx.y * gridDim.x;
+        const unsigned xcd = L % 8u, q = nt / 8u, rem = nt % 8u;
+        const unsigned tile = xcd * q + min(xcd, rem) + L / 8u;
+        bx = (int)(tile % gridDim.x);
+        by = (int)(tile / gridDim.x);
+    }
+    const int js = bx * (S - 2 * P.halo) - P.halo;   // first column of the strip
     Geom g;
     g.j0 = js + (int)threadIdx.x * C;
-    g.i0 = (int)blockIdx.y * P.rows;
+    g.i0 = by * P.rows;
     g.i1 = min(g.i0 + P.rows, P.nx);
     g.vj0 = js + P.halo;
     g.vj1 = min(js + S - P.halo, P.ny);
@@ -503,12 +548,88 @@ __global__ void __launch_bounds__(64) tm_tb_kernel(const TbParams P)
     }
 }
 
-template <int K, int C, bool VEC, int PART>
-int launch_kv(const TbParams &p, bool lossy, bool pml, hipStream_t s)
+// The tiles of the boundary launch as lists (TbParams::rb_list, sx_list,
+// t_list), from the same predicates as tm_tb_kernel: row blocks with brow or
+// a TF/SF row in reach, strips with bcol or a TF/SF column, and the tiles of
+// sources and probes elsewhere.  Returns the number of workgroups of the
+// compact launch, or -1 if a list would overflow (q.n_rb = -1: whole grid).
+template <int K, int C>
+int tb_tile_lists(TbParams &q)
 {
     constexpr int S = 64 * C;
-    const int w = S - 2 * p.halo;
-    const dim3 grid((p.ny + w - 1) / w, (p.nx + p.rows - 1) / p.rows);
+    const int w = S - 2 * q.halo;
+    const int nsx = (q.ny + w - 1) / w, nrb = (q.nx + q.rows - 1) / q.rows;
+    q.n_rb = q.n_sx = q.n_t = 0;
+    auto fail_list = [&]() { q.n_rb = -1; q.n_sx = q.n_t = 0; return -1; };
+    const int m = q.tf_npml;
+    for (int b = 0; b < nrb; ++b) {
+        const int i0 = b * q.rows, i1 = std::min(i0 + q.rows, q.nx);
+        bool in = !(i0 - 2 * K >= q.rlo && i1 - 1 + K < q.rhi);
+        if (m) {
+            const int lo = i0 - 2 * K, hi = i1 + K;
+            in = in || (m - 2 >= lo && m - 2 <= hi) || (q.nx - m >= lo && q.nx - m <= hi);
+        }
+        if (in) {
+            if (q.n_rb == TB_MAX_LIST_RB) return fail_list();
+            q.rb_list[q.n_rb++] = b;
+        }
+    }
+    for (int x = 0; x < nsx; ++x) {
+        const int js = x * w - q.halo;
+        auto col_in = [&](int j) { return j >= js && j < js + S; };
+        bool in = !(js >= q.clo && js + S <= q.chi);
+        if (m) in = in || col_in(m - 2) || col_in(m - 1) || col_in(q.ny - m);
+        if (in) {
+            if (q.n_sx == TB_MAX_LIST_SX) return fail_list();
+            q.sx_list[q.n_sx++] = x;
+        }
+    }
+    auto listed = [&](int x, int b) {
+        for (int k = 0; k < q.n_rb; ++k)
+            if (q.rb_list[k] == b) return true;
+        for (int k = 0; k < q.n_sx; ++k)
+            if (q.sx_list[k] == x) return true;
+        for (int k = 0; k < q.n_t; ++k)
+            if (q.t_list[k] == (x << 16 | b)) return true;
+        return false;
+    };
+    // cells (i, j) whose tiles are single entries: a source reaches rows
+    // [i0 - 2K, i1 + K] of a tile, a probe only its own rows
+    auto add_cell = [&](int ci, int cj, int below, int above) {
+        for (int b = 0; b < nrb; ++b) {
+            const int i0 = b * q.rows, i1 = std::min(i0 + q.rows, q.nx);
+            if (!(ci >= i0 - below && ci < i1 + above)) continue;
+            for (int x = 0; x < nsx; ++x) {
+                const int js = x * w - q.halo;
+                if (!(cj >= js && cj < js + S) || listed(x, b)) continue;
+                if (q.n_t == TB_MAX_LIST_T) return false;
+                q.t_list[q.n_t++] = x << 16 | b;
+            }
+        }
+        return true;
+    };
+    for (int k = 0; k < q.nsrc; ++k)
+        if (!add_cell(q.src_i[k], q.src_j[k], 2 * K, K + 1)) return fail_list();
+    for (int k = 0; k < q.nprobe; ++k)
+        if (!add_cell(q.probe_i[k], q.probe_j[k], 0, 0)) return fail_list();
+    if (nsx >= 0x10000 || nrb >= 0x10000) return fail_list();
+    return q.n_rb * nsx + q.n_sx * nrb + q.n_t;
+}
+
+template <int K, int C, bool VEC, int PART>
+int launch_kv(const TbParams &p_in, bool lossy, bool pml, hipStream_t s)
+{
+    constexpr int S = 64 * C;
+    const int w = S - 2 * p_in.halo;
+    dim3 grid((p_in.ny + w - 1) / w, (p_in.nx + p_in.rows - 1) / p_in.rows);
+    TbParams p = p_in;
+    p.n_rb = -1;
+    p.n_sx = p.n_t = 0;
+    if (PART == 1) {
+        const int n = tb_tile_lists<K, C>(p);
+        if (n == 0) return 0;                 // every tile is interior
+        if (n > 0) grid = dim3(n, 1);
+    }
     if (p.nf) {
         if (!(lossy && pml) || p.nf > TB_MAX_FREQ)
             return fail("fused running DFT: lossy PML form and at most %d frequencies", TB_MAX_FREQ);
@@ -549,6 +670,8 @@ tm_tb_batch_kernel(const fd2d_grid *__restrict__ grids, const int *__restrict__ 
         P.tf_npml = 0;                        // no plane wave, no DFT in a batch
         P.nf = 0;
         P.inc = nullptr;
+        P.n_rb = -1;                          // (tile lists: single-grid launches only)
+        P.n_sx = P.n_t = 0;
         P.nx = G.nx;
         P.ny = G.ny;
         const bool swap = parity != 0;        // odd launches read the partner set

@@ -108,9 +108,9 @@ int plan_tm(const fd2d_tm_desc *d, int nsteps, Plan *pl)
     if (64 * C - 2 * D < 8) return fail("fd2d_tm: depth %d too large for %d columns/lane", D, C);
     int H = d->rows;
     if (H == 0) {
-        // default from autotuner runs on MI355X (8192^2: depth 8, one column
-        // per lane, ~220 row blocks per strip -- many short waves, so the
-        // slower edge waves are spread over more of the launch)
+        // default from row sweeps on MI355X (depth 8, one column per lane,
+        // ~220 row blocks per strip: 4096^2 and 16384^2 at their best,
+        // 8192^2 within 1 %)
         long h = d->nx / 220;
         if (h < 16) h = 16;
         if (h > 512) h = 512;

@@ -49,6 +49,9 @@ struct CylList {
 // Arguments of one temporally blocked launch (fd2d_tb.hip).
 constexpr int TB_MAX_DEPTH = 8;     // temporally blocked launches: steps per launch
 constexpr int TB_MAX_FREQ = 3;      // fused running DFT: frequencies
+constexpr int TB_MAX_LIST_RB = 64;  // boundary launch: whole row blocks listed
+constexpr int TB_MAX_LIST_SX = 64;  // boundary launch: whole strips listed
+constexpr int TB_MAX_LIST_T = 128;  // boundary launch: single tiles listed
 
 struct TbParams {
     int nx, ny;
@@ -82,6 +85,12 @@ struct TbParams {
     int nf;
     float *ft_r, *ft_i;
     float tw_c[TB_MAX_DEPTH][TB_MAX_FREQ], tw_s[TB_MAX_DEPTH][TB_MAX_FREQ];
+    // tiles of the boundary launch (part 1): the row blocks rb_list (every
+    // strip), the strips sx_list (every row block) and single tiles t_list
+    // (strip << 16 | row block); together a superset of the tiles that are
+    // not interior.  n_rb < 0: the launch covers the whole grid instead.
+    int n_rb, n_sx, n_t;
+    int rb_list[TB_MAX_LIST_RB], sx_list[TB_MAX_LIST_SX], t_list[TB_MAX_LIST_T];
 };
 
 // Arguments of one learned-boundary frame launch (lbc_frame.hip).
