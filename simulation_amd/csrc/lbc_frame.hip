@@ -9,7 +9,7 @@
 // and right edges H at depth d takes ez at depth d-1, so H reaches K+2):
 // after K steps the blocked launch is exact at depth >= K+3.
 //
-// Each workgroup owns one 40-cell chunk of one edge's frame (depth K+3) and
+// Each workgroup owns one 32-cell chunk of one edge's frame (depth K+3) and
 // evolves, in LDS, the chunk's dependency region.  Per step the valid part
 // of a region shrinks by one cell on a cut side, except along an edge in the
 // direction of increasing index, where it shrinks by two: H at tau takes ez
@@ -47,11 +47,12 @@ namespace fdtd {
 namespace lbc {
 namespace {
 
-constexpr int CH = 40;                        // chunk length along the edge
+constexpr int CH = 32;                        // chunk length along the edge (at 2048^2:
+                                              // 256 workgroups, one per CU)
 constexpr int KMAX = 8;
 constexpr int NT = 512;                       // threads per workgroup (8 waves)
 // region cells: corner chunks (3K+6) x (CH + 2K + 1), others at most
-// (2K+6) x (CH + 3K + 2) (30 x 57 and 22 x 66 at K = 8)
+// (2K+6) x (CH + 3K + 2) (30 x 49 and 22 x 58 at K = 8)
 constexpr int RMAX_C = (3 * KMAX + 6) * (CH + 2 * KMAX + 1);
 constexpr int RMAX_E = (2 * KMAX + 6) * (CH + 3 * KMAX + 2);
 constexpr int RMAX = ((RMAX_C > RMAX_E ? RMAX_C : RMAX_E) + 3) / 4 * 4;   // keeps the ring 16-byte aligned
@@ -178,7 +179,7 @@ __device__ inline void block_geo(int b, int nx, int ny, int K, Geo &G, int &kind
 }
 
 // One workgroup per frame workgroup: its position list (thread 0; at most
-// CH + 3K + 2 + 2 (3K + 3) = 120 positions) and sample-source table.
+// CH + 3K + 2 + 2 (3K + 3) = 112 positions) and sample-source table.
 __global__ void __launch_bounds__(NT) frame_setup_kernel(int nx, int ny, int K, FrameTab *tabs)
 {
     __shared__ FrameTab S;
@@ -191,7 +192,7 @@ __global__ void __launch_bounds__(NT) frame_setup_kernel(int nx, int ny, int K, 
     block_geo(blockIdx.x, nx, ny, K, G, kind, q);
     const int RH = G.RH, RW = G.RW;
     // ---------------------------------------------------------- positions
-    // (thread 0 builds the list; at most CH + 3K + 2 + 2 (3K + 3) = 120)
+    // (thread 0 builds the list; at most CH + 3K + 2 + 2 (3K + 3) = 112)
     if (tid == 0) {
         int n = 0;
         auto add = [&](int e, int t0, int t1, int w0, int w1) {
