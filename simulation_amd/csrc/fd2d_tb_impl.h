@@ -458,22 +458,41 @@ __device__ __forceinline__ void sweep(const TbParams &P, const Geom &g)
     }
 }
 
-// PART 0: the interior waves; PART 1: every other wave (grid edges, PML
-// bands, sources and probes).  Both launches cover the whole grid and each
-// wave returns at once if it belongs to the other part, so the two kernels
-// write disjoint cells and may run concurrently; each gets the register
-// allocation of its own body.
+// PART TB_WHOLE: one launch over every tile -- the listed boundary tiles
+// first (grid edges, PML bands, sources and probes: the general bodies,
+// slower, so they start early), then one workgroup per tile of the grid,
+// which computes its tile if it is interior (any tile when there are no
+// lists).  TB_INTERIOR / TB_BOUNDARY: the interior tiles (whole grid) or the
+// listed others, as two launches that write disjoint cells and may run
+// concurrently.  A wave returns at once if its tile is not its part's.
 template <int K, int C, bool LOSSY, bool PML, bool VEC, int PART, int NF = 0>
 __global__ void __launch_bounds__(64) tm_tb_kernel(const TbParams P)
 {
     constexpr int S = 64 * C;
+    const int w = S - 2 * P.halo;
+    const int nsx = (P.ny + w - 1) / w, nrb = (P.nx + P.rows - 1) / P.rows;
+    // which tiles this workgroup may compute, and where its index comes from
+    bool want_int = PART != TB_BOUNDARY, want_bnd = PART != TB_INTERIOR;
+    int bi = -1;                     // index into the boundary lists
+    unsigned L = blockIdx.x, nt = gridDim.x;
+    if (PART == TB_BOUNDARY) {
+        if (P.n_rb >= 0) bi = (int)blockIdx.x;
+    } else if (PART == TB_WHOLE) {
+        if ((int)blockIdx.x < P.nb_pad) {
+            if ((int)blockIdx.x >= P.nb) return;
+            bi = (int)blockIdx.x;
+            want_int = false;
+        } else {
+            L = blockIdx.x - (unsigned)P.nb_pad;       // nb_pad % 8 == 0: the XCD of L
+            nt = gridDim.x - (unsigned)P.nb_pad;
+            want_bnd = P.n_rb < 0;                     // no lists: every tile here
+        }
+    }
     int bx, by;
-    if (PART == 1 && P.n_rb >= 0) {
-        // compact boundary launch: listed row blocks, then listed strips,
-        // then single tiles; a tile listed twice is left to its first entry
-        const int w = S - 2 * P.halo;
-        const int nsx = (P.ny + w - 1) / w, nrb = (P.nx + P.rows - 1) / P.rows;
-        int i = (int)blockIdx.x;
+    if (bi >= 0) {
+        // listed row blocks, then listed strips, then single tiles; a tile
+        // listed twice is left to its first entry
+        int i = bi;
         auto in_rb = [&](int b) {
             for (int k = 0; k < P.n_rb; ++k)
                 if (P.rb_list[k] == b) return true;
@@ -498,17 +517,15 @@ __global__ void __launch_bounds__(64) tm_tb_kernel(const TbParams P)
             if (in_rb(by) || in_sx(bx)) return;
         }
     } else {
-        // XCD-aware tile order: the dispatcher hands workgroup L (x fastest)
-        // to XCD L % 8.  The workgroups of one XCD take a contiguous run of
-        // tiles, strips fastest, so neighbouring strips -- which read each
-        // other's halo columns at the same rows at about the same time -- and
-        // the row blocks below them share that XCD's L2.
-        const unsigned nt = gridDim.x * gridDim.y;
-        const unsigned L = blockIdx.x + blockIdx.y * gridDim.x;
+        // XCD-aware tile order: the dispatcher hands workgroup L to XCD
+        // L % 8.  The workgroups of one XCD take a contiguous run of tiles,
+        // strips fastest, so neighbouring strips -- which read each other's
+        // halo columns at the same rows at about the same time -- and the
+        // row blocks below them share that XCD's L2.
         const unsigned xcd = L % 8u, q = nt / 8u, rem = nt % 8u;
         const unsigned tile = xcd * q + min(xcd, rem) + L / 8u;
-        bx = (int)(tile % gridDim.x);
-        by = (int)(tile / gridDim.x);
+        bx = (int)(tile % (unsigned)nsx);
+        by = (int)(tile / (unsigned)nsx);
     }
     const int js = bx * (S - 2 * P.halo) - P.halo;   // first column of the strip
     Geom g;
@@ -540,9 +557,9 @@ __global__ void __launch_bounds__(64) tm_tb_kernel(const TbParams P)
         sp |= (row_in(m - 2) || row_in(P.nx - m)) && js + S > m - 1 && js <= P.ny - m;
     }
     const bool interior = !brow && !bcol && !sp;
-    if (PART == 0) {
-        if (interior) sweep<K, C, LOSSY, PML, VEC, false, false, false, NF>(P, g);
-    } else if (!interior) {
+    if (interior) {
+        if (want_int) sweep<K, C, LOSSY, PML, VEC, false, false, false, NF>(P, g);
+    } else if (want_bnd) {
         if (!brow && !sp) sweep<K, C, LOSSY, PML, VEC, false, true, false, NF>(P, g);
         else sweep<K, C, LOSSY, PML, VEC, true, true, true, NF>(P, g);
     }
@@ -621,36 +638,64 @@ int launch_kv(const TbParams &p_in, bool lossy, bool pml, hipStream_t s)
 {
     constexpr int S = 64 * C;
     const int w = S - 2 * p_in.halo;
-    dim3 grid((p_in.ny + w - 1) / w, (p_in.nx + p_in.rows - 1) / p_in.rows);
+    const long ntile = (long)((p_in.ny + w - 1) / w) * (long)((p_in.nx + p_in.rows - 1) / p_in.rows);
     TbParams p = p_in;
     p.n_rb = -1;
     p.n_sx = p.n_t = 0;
-    if (PART == 1) {
+    p.nb = p.nb_pad = 0;
+    long nwg = ntile;
+    if (PART != TB_INTERIOR) {
         const int n = tb_tile_lists<K, C>(p);
-        if (n == 0) return 0;                 // every tile is interior
-        if (n > 0) grid = dim3(n, 1);
+        if (PART == TB_BOUNDARY) {
+            if (n == 0) return 0;             // every tile is interior
+            if (n > 0) nwg = n;
+        } else if (n > 0) {
+            p.nb = n;
+            p.nb_pad = (n + 7) / 8 * 8;       // tile workgroups keep their XCD order
+            nwg = p.nb_pad + ntile;
+        }
+    }
+    if (nwg > 0x7fffffffL) return fail("fd2d_tm_run: %ld workgroups", nwg);
+    const dim3 grid((unsigned)nwg);
+    if constexpr (PART != TB_WHOLE) {
+        // the split launches serve the learned boundary's form only
+        if (pml || p.nf) return fail("split TM launch: the form without PML and DFT only");
+        if (lossy) hipLaunchKernelGGL((tm_tb_kernel<K, C, true, false, VEC, PART>), grid, dim3(64), 0, s, p);
+        else hipLaunchKernelGGL((tm_tb_kernel<K, C, false, false, VEC, PART>), grid, dim3(64), 0, s, p);
+        return hip_check(hipGetLastError(), "fd2d_tm_run launch");
     }
     if (p.nf) {
         if (!(lossy && pml) || p.nf > TB_MAX_FREQ)
             return fail("fused running DFT: lossy PML form and at most %d frequencies", TB_MAX_FREQ);
-        hipLaunchKernelGGL((tm_tb_kernel<K, C, true, true, VEC, PART, TB_MAX_FREQ>), grid, dim3(64),
+        hipLaunchKernelGGL((tm_tb_kernel<K, C, true, true, VEC, TB_WHOLE, TB_MAX_FREQ>), grid, dim3(64),
                            0, s, p);
         return hip_check(hipGetLastError(), "fd2d_tm_run launch");
     }
-    if (lossy && pml) hipLaunchKernelGGL((tm_tb_kernel<K, C, true, true, VEC, PART>), grid, dim3(64), 0, s, p);
-    else if (lossy) hipLaunchKernelGGL((tm_tb_kernel<K, C, true, false, VEC, PART>), grid, dim3(64), 0, s, p);
-    else if (pml) hipLaunchKernelGGL((tm_tb_kernel<K, C, false, true, VEC, PART>), grid, dim3(64), 0, s, p);
-    else hipLaunchKernelGGL((tm_tb_kernel<K, C, false, false, VEC, PART>), grid, dim3(64), 0, s, p);
+    if (lossy && pml) hipLaunchKernelGGL((tm_tb_kernel<K, C, true, true, VEC, TB_WHOLE>), grid, dim3(64), 0, s, p);
+    else if (lossy) hipLaunchKernelGGL((tm_tb_kernel<K, C, true, false, VEC, TB_WHOLE>), grid, dim3(64), 0, s, p);
+    else if (pml) hipLaunchKernelGGL((tm_tb_kernel<K, C, false, true, VEC, TB_WHOLE>), grid, dim3(64), 0, s, p);
+    else hipLaunchKernelGGL((tm_tb_kernel<K, C, false, false, VEC, TB_WHOLE>), grid, dim3(64), 0, s, p);
     return hip_check(hipGetLastError(), "fd2d_tm_run launch");
 }
 
 template <int K, int C, int PART>
-int launch_part(const TbParams &p, bool lossy, bool pml, hipStream_t s)
+int launch_cols(const TbParams &p, bool lossy, bool pml, hipStream_t s)
 {
     // vector loads need every lane's columns wholly inside or outside the grid
     // (the host rounds the halo up to a multiple of the columns per lane)
     if (C == 2 && p.ny % 2 == 0) return launch_kv<K, C, true, PART>(p, lossy, pml, s);
     return launch_kv<K, C, false, PART>(p, lossy, pml, s);
+}
+
+template <int K, int C>
+int launch_part(const TbParams &p, bool lossy, bool pml, int part, hipStream_t s)
+{
+    switch (part) {
+    case TB_WHOLE: return launch_cols<K, C, TB_WHOLE>(p, lossy, pml, s);
+    case TB_INTERIOR: return launch_cols<K, C, TB_INTERIOR>(p, lossy, pml, s);
+    case TB_BOUNDARY: return launch_cols<K, C, TB_BOUNDARY>(p, lossy, pml, s);
+    default: return fail("TM launch part %d", part);
+    }
 }
 
 // ----------------------------------------------------------- batched grids
@@ -672,6 +717,7 @@ tm_tb_batch_kernel(const fd2d_grid *__restrict__ grids, const int *__restrict__ 
         P.inc = nullptr;
         P.n_rb = -1;                          // (tile lists: single-grid launches only)
         P.n_sx = P.n_t = 0;
+        P.nb = P.nb_pad = 0;
         P.nx = G.nx;
         P.ny = G.ny;
         const bool swap = parity != 0;        // odd launches read the partner set
@@ -777,10 +823,8 @@ int fd2d_batch_resident_depth(int cols, bool vec, int *waves)
 template <int K>
 int launch_fd2d_tb_depth(const TbParams &p, int cols, bool lossy, bool pml, int part, hipStream_t s)
 {
-    if (cols == 2)
-        return part == 0 ? tb::launch_part<K, 2, 0>(p, lossy, pml, s) : tb::launch_part<K, 2, 1>(p, lossy, pml, s);
-    if (cols == 1)
-        return part == 0 ? tb::launch_part<K, 1, 0>(p, lossy, pml, s) : tb::launch_part<K, 1, 1>(p, lossy, pml, s);
+    if (cols == 2) return tb::launch_part<K, 2>(p, lossy, pml, part, s);
+    if (cols == 1) return tb::launch_part<K, 1>(p, lossy, pml, part, s);
     return fail("columns per lane %d not built", cols);
 }
 

@@ -131,11 +131,11 @@ int plan_tm(const fd2d_tm_desc *d, int nsteps, Plan *pl)
     return 0;
 }
 
-// The auxiliary stream on which the boundary part of each temporally blocked
-// launch runs, concurrently with the interior part on the caller's stream,
-// and the two events that order them.  One set per host thread and device,
-// created on first use and kept for the life of the process (no device
-// memory is involved).
+// The learned boundary's two extra streams (the boundary part of the blocked
+// TM launch and the frame kernel run concurrently with the interior part on
+// the caller's stream) and the events that order them.  One set per host
+// thread and device, created on first use and kept for the life of the
+// process (no device memory is involved).
 struct Aux {
     hipStream_t s = nullptr, s2 = nullptr;
     hipEvent_t fork = nullptr, join = nullptr, join2 = nullptr, join0 = nullptr;
@@ -552,11 +552,6 @@ int fd2d_tm_run(const fd2d_tm_desc *d, int t0, int nsteps, void *stream)
     float *A[6] = {d->dz, d->iz, d->hx, d->hy, d->ihx, d->ihy};
     float *B[6] = {d->dz2, d->iz2, d->hx2, d->hy2, d->ihx2, d->ihy2};
     const int L = pl.launches;
-    Aux *ax = nullptr;
-    if (L) {
-        rc = aux_for_current_device(&ax);
-        if (rc) return rc;
-    }
     int done = 0;
     for (int l = 0; l < L; ++l) {
         const int k = nfused / L + (l < nfused % L ? 1 : 0);
@@ -566,21 +561,9 @@ int fd2d_tm_run(const fd2d_tm_desc *d, int t0, int nsteps, void *stream)
         p.dz_o = out[0]; p.iz_o = out[1]; p.hx_o = out[2]; p.hy_o = out[3]; p.ihx_o = out[4]; p.ihy_o = out[5];
         p.ez_o = (l == L - 1 && !pl.tail_unfused) ? d->ez : nullptr;
         p.t_first = t0 + done + 1;
-        // boundary part on the auxiliary stream, after all work queued so far
-        // on the caller's stream (which includes the previous interior part)
-        rc = hip_check(hipEventRecord(ax->fork, S(stream)), "fork event");
-        if (!rc) rc = hip_check(hipStreamWaitEvent(ax->s, ax->fork, 0), "fork wait");
-        if (!rc) rc = launch_fd2d_tb(p, k, pl.cols, lossy, pml, 1, ax->s);
-        // interior part on the caller's stream, after the previous boundary part
-        if (!rc && l > 0) rc = hip_check(hipStreamWaitEvent(S(stream), ax->join, 0), "join wait");
-        if (!rc) rc = launch_fd2d_tb(p, k, pl.cols, lossy, pml, 0, S(stream));
-        if (!rc) rc = hip_check(hipEventRecord(ax->join, ax->s), "join event");
+        rc = launch_fd2d_tb(p, k, pl.cols, lossy, pml, TB_WHOLE, S(stream));
         if (rc) return rc;
         done += k;
-    }
-    if (L) {
-        rc = hip_check(hipStreamWaitEvent(S(stream), ax->join, 0), "join wait");
-        if (rc) return rc;
     }
     if (pl.tail_unfused) {
         const int t = t0 + done + 1;
@@ -799,9 +782,7 @@ static int tfsf_fused(const fd2d_tm_desc *d, const fd2d_tfsf_desc *b, int t0, in
     const bool lossy = d->md.nbz != nullptr;
     const bool pml = pml_all(&d->pml);
     const int stride = d->ny - 2 * b->npml + 4;
-    Aux *ax = nullptr;
-    int rc = aux_for_current_device(&ax);
-    if (rc) return rc;
+    int rc = 0;
     int done = 0;
     while (done < nsteps) {
         const int n = nsteps - done < INC_CHUNK ? nsteps - done : INC_CHUNK;
@@ -864,17 +845,10 @@ static int tfsf_fused(const fd2d_tm_desc *d, const fd2d_tfsf_desc *b, int t0, in
             p.t_first = tc + dl + 1;
             for (int q = 0; q < k; ++q)
                 for (int f = 0; f < b->nf; ++f) dft_twiddle(p.t_first + q, b->freq[f], b->dt, &p.tw_c[q][f], &p.tw_s[q][f]);
-            rc = hip_check(hipEventRecord(ax->fork, s0), "fork event");
-            if (!rc) rc = hip_check(hipStreamWaitEvent(ax->s, ax->fork, 0), "fork wait");
-            if (!rc) rc = launch_fd2d_tb(p, k, pl.cols, lossy, pml, 1, ax->s);
-            if (!rc && l > 0) rc = hip_check(hipStreamWaitEvent(s0, ax->join, 0), "join wait");
-            if (!rc) rc = launch_fd2d_tb(p, k, pl.cols, lossy, pml, 0, s0);
-            if (!rc) rc = hip_check(hipEventRecord(ax->join, ax->s), "join event");
+            rc = launch_fd2d_tb(p, k, pl.cols, lossy, pml, TB_WHOLE, s0);
             if (rc) return rc;
             dl += k;
         }
-        rc = hip_check(hipStreamWaitEvent(s0, ax->join, 0), "join wait");
-        if (rc) return rc;
         done += n;
     }
     return 0;
@@ -1009,7 +983,8 @@ int fd2d_lbc_run(const fd2d_tm_desc *d, const fd2d_lbc_desc *b, int t0, int nste
     void *tabq = wk + lbc_frame_table_offset(d->nx, d->ny, kq);
     void *tabr = kr ? wk + lbc_frame_table_offset(d->nx, d->ny, kq + 1) : nullptr;
     // three streams: interior part P0 on the caller's, boundary part P1 on
-    // ax->s, frame F on ax->s2.  Launch l:
+    // ax->s, frame F on ax->s2 (one merged launch would start ahead of the
+    // frame and keep it from the CUs).  Launch l:
     //   P0(l) after P0(l-1), P1(l-1)   (reads and overwrites what they used)
     //   P1(l) after P0(l-1), P1(l-1), F(l-1)
     //   F(l)  after P1(l-1), F(l-1)    (P0 stays DZ cells clear of the frame)
@@ -1053,9 +1028,9 @@ int fd2d_lbc_run(const fd2d_tm_desc *d, const fd2d_lbc_desc *b, int t0, int nste
         }
         rc = launch_lbc_frame(f, ax->s2);
         if (!rc) rc = hip_check(hipEventRecord(ax->join2, ax->s2), "F event");
-        if (!rc) rc = launch_fd2d_tb(p, k, pl.cols, lossy, false, 1, ax->s);
+        if (!rc) rc = launch_fd2d_tb(p, k, pl.cols, lossy, false, TB_BOUNDARY, ax->s);
         if (!rc) rc = hip_check(hipEventRecord(ax->join, ax->s), "P1 event");
-        if (!rc) rc = launch_fd2d_tb(p, k, pl.cols, lossy, false, 0, s0);
+        if (!rc) rc = launch_fd2d_tb(p, k, pl.cols, lossy, false, TB_INTERIOR, s0);
         if (!rc) rc = hip_check(hipEventRecord(ax->join0, s0), "P0 event");
         if (rc) return rc;
         done += k;

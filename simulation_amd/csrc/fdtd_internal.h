@@ -85,11 +85,13 @@ struct TbParams {
     int nf;
     float *ft_r, *ft_i;
     float tw_c[TB_MAX_DEPTH][TB_MAX_FREQ], tw_s[TB_MAX_DEPTH][TB_MAX_FREQ];
-    // tiles of the boundary launch (part 1): the row blocks rb_list (every
-    // strip), the strips sx_list (every row block) and single tiles t_list
-    // (strip << 16 | row block); together a superset of the tiles that are
-    // not interior.  n_rb < 0: the launch covers the whole grid instead.
-    int n_rb, n_sx, n_t;
+    // boundary tiles: the row blocks rb_list (every strip), the strips
+    // sx_list (every row block) and single tiles t_list (strip << 16 | row
+    // block); together a superset of the tiles that are not interior.  n_rb
+    // < 0: no lists (every workgroup takes its tile from the whole grid).
+    // A merged launch runs nb boundary workgroups (padded to nb_pad, a
+    // multiple of 8), then one per tile of the grid.
+    int n_rb, n_sx, n_t, nb, nb_pad;
     int rb_list[TB_MAX_LIST_RB], sx_list[TB_MAX_LIST_SX], t_list[TB_MAX_LIST_T];
 };
 
@@ -178,8 +180,11 @@ int launch_fd2d_dielectric(int nx, int ny, int npml, double dt, const CylList &c
 int launch_fd2d_energy(int nx, int ny, int i0, int i1, int j0, int j1, const float *ez,
                        const float *hx, const float *hy, double *work, hipStream_t s);
 
-// Temporally blocked TM launch: depth in [1, tb_max_depth()], cols in {1,2};
-// part 0 = interior waves, part 1 = all other waves (see fd2d_tb_impl.h).
+// Temporally blocked TM launch (fd2d_tb_impl.h): depth in [1,
+// tb_max_depth()], cols in {1,2}; part TB_WHOLE: every tile in one launch
+// (the boundary tiles first), TB_INTERIOR / TB_BOUNDARY: the interior tiles
+// or the others only (two launches that may run concurrently).
+enum { TB_INTERIOR = 0, TB_BOUNDARY = 1, TB_WHOLE = 2 };
 int tb_max_depth();
 bool tb_supported(int depth, int cols);
 int launch_fd2d_tb(const TbParams &p, int depth, int cols, bool lossy, bool pml, int part,
