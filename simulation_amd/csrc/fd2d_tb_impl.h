@@ -143,6 +143,30 @@ __device__ __forceinline__ void st(float *__restrict__ base, unsigned row, int j
     }
 }
 
+// Buffer-resource stores (BS sweeps): the range check drops the lanes and
+// rows that are not written (offset OOB) instead of branches, so each sweep
+// iteration issues the same number of stores and the wait for the next
+// row's loads need not wait for the stores just issued.
+constexpr unsigned OOB = 0xFFFFFFFCu;      // >= any grid's byte size (< 2^30 cells)
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t out_rsrc(const float *base, unsigned bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(base), (short)0, (int)bytes,
+                                             0x00020000);
+}
+
+template <int C>
+__device__ __forceinline__ void st_rb(__amdgpu_buffer_rsrc_t rs, unsigned row, int j0, int c0, int c1,
+                                      bool okrow, const float (&v)[C])
+{
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+        const int j = j0 + c;
+        const unsigned off = (okrow && j >= c0 && j < c1) ? (row + (unsigned)j) << 2 : OOB;
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v[c]), rs, off, 0, 0);
+    }
+}
+
 template <int C, bool LOSSY, bool PML, bool VEC, bool CLAMP>
 __device__ __forceinline__ void load_row(const TbParams &P, int p, int j0, HRow<C> &H, DRow<C> &D)
 {
@@ -191,7 +215,8 @@ __device__ __forceinline__ void col_coef(const TbParams &P, int j0, ColCoef<C> &
 // p-s and the H update of row p-s-1.  BROW: rows may lie outside [1, nx-2] or
 // outside the default band; BCOL: the same for the strip's columns; SP: a
 // source or probe lies in the rows and columns the wave computes.
-template <int K, int C, bool LOSSY, bool PML, bool VEC, bool BROW, bool BCOL, bool SP, int NF>
+template <int K, int C, bool LOSSY, bool PML, bool VEC, bool BROW, bool BCOL, bool SP, int NF,
+          bool BS>
 __device__ __forceinline__ void iterate(const TbParams &P, const Geom &g, const ColCoef<C> &cc,
                                         int p, const HRow<C> &Hload, const DRow<C> &Dload,
                                         Carry<C> (&St)[K], const FRow<C, NF> &Fload,
@@ -367,6 +392,24 @@ __device__ __forceinline__ void iterate(const TbParams &P, const Geom &g, const 
             // (cells within skip_w of an edge are left to another kernel)
             const int sw = P.skip_w;
             const int c0 = max(g.vj0, sw), c1 = min(g.vj1, ny - sw);
+            if constexpr (BS) {
+                static_assert(NF == 0, "buffer stores: no DFT planes");
+                const unsigned nb = (unsigned)P.nx * (unsigned)ny * 4u;   // < 2^32
+                const bool okr = r >= g.i0 && r < g.i1 && r >= sw && r < P.nx - sw;
+                const unsigned rowr = (unsigned)r * (unsigned)ny;
+                st_rb<C>(out_rsrc(P.dz_o, nb), rowr, g.j0, c0, c1, okr, Dn.dz);
+                if (LOSSY) st_rb<C>(out_rsrc(P.iz_o, nb), rowr, g.j0, c0, c1, okr, Dn.iz);
+                st_rb<C>(out_rsrc(P.ez_o, P.ez_o ? nb : 0u), rowr, g.j0, c0, c1, okr, En);
+                const bool okq = q >= g.i0 && q < g.i1 && q >= sw && q < P.nx - sw;
+                const unsigned rowq = (unsigned)q * (unsigned)ny;
+                st_rb<C>(out_rsrc(P.hx_o, nb), rowq, g.j0, c0, c1, okq, Hout.hx);
+                st_rb<C>(out_rsrc(P.hy_o, nb), rowq, g.j0, c0, c1, okq, Hout.hy);
+                if (PML) {
+                    st_rb<C>(out_rsrc(P.ihx_o, nb), rowq, g.j0, c0, c1, okq, Hout.ix);
+                    st_rb<C>(out_rsrc(P.ihy_o, nb), rowq, g.j0, c0, c1, okq, Hout.iy);
+                }
+                continue;
+            }
             if (r >= g.i0 && r < g.i1 && r >= sw && r < P.nx - sw) {
                 const unsigned row = (unsigned)r * (unsigned)ny;
                 st<C, VEC>(P.dz_o, row, g.j0, c0, c1, Dn.dz);
@@ -415,7 +458,8 @@ __device__ __forceinline__ void load_ft(const TbParams &P, int p, int j0, FRow<C
     }
 }
 
-template <int K, int C, bool LOSSY, bool PML, bool VEC, bool BROW, bool BCOL, bool SP, int NF = 0>
+template <int K, int C, bool LOSSY, bool PML, bool VEC, bool BROW, bool BCOL, bool SP, int NF = 0,
+          bool BS = false>
 __device__ __forceinline__ void sweep(const TbParams &P, const Geom &g)
 {
     ColCoef<C> cc;
@@ -448,10 +492,14 @@ __device__ __forceinline__ void sweep(const TbParams &P, const Geom &g)
     FRow<C, NF> F0, F1;
     load_row<C, LOSSY, PML, VEC, BCOL>(P, pbeg, g.j0, H0, D0);
     if constexpr (NF > 0) load_ft<C, NF, VEC, BCOL>(P, pbeg, g.j0, F0);
+    // BS: the first row is needed at once; waiting for it outside the loop
+    // keeps the loop head free of a wait that would also catch the stores of
+    // the previous iteration
+    if constexpr (BS) __builtin_amdgcn_s_waitcnt(0);
     for (int p = pbeg; p <= pend; ++p) {
         load_row<C, LOSSY, PML, VEC, BCOL>(P, p + 1, g.j0, H1, D1);
         if constexpr (NF > 0) load_ft<C, NF, VEC, BCOL>(P, p + 1, g.j0, F1);
-        iterate<K, C, LOSSY, PML, VEC, BROW, BCOL, SP, NF>(P, g, cc, p, H0, D0, St, F0, SF);
+        iterate<K, C, LOSSY, PML, VEC, BROW, BCOL, SP, NF, BS>(P, g, cc, p, H0, D0, St, F0, SF);
         H0 = H1;
         D0 = D1;
         if constexpr (NF > 0) F0 = F1;
@@ -767,8 +815,8 @@ tm_tb_batch_kernel(const fd2d_grid *__restrict__ grids, const int *__restrict__ 
         if (P.nprobe)
             sp |= P.probe_i[0] >= g.i0 && P.probe_i[0] < g.i1 && P.probe_j[0] >= js &&
                   P.probe_j[0] < js + S;
-        if (!brow && !bcol && !sp) sweep<K, C, true, true, VEC, false, false, false>(P, g);
-        else sweep<K, C, true, true, VEC, true, true, true>(P, g);
+        if (!brow && !bcol && !sp) sweep<K, C, true, true, VEC, false, false, false, 0, true>(P, g);
+        else sweep<K, C, true, true, VEC, true, true, true, 0, true>(P, g);
     }
 }
 
