@@ -413,9 +413,11 @@ def bench_tfsf(a):
 def bench_dform(a):
     """The reference's 1-D Debye program (fd1d/clang/test_2_3.c main): 38000
     cells, Debye half space, additive Gaussian at 1, DFT at 50/200/500 MHz;
-    one step = dxfield, exfield, fourier, hyfield (fd1d_dform_run).
-    Algorithmic bytes per cell-update: R+W dx ix sx hy (32), R nax..ndx
-    (16), W ex (4), R+W 6 DFT planes (48)."""
+    one step = dxfield, exfield, fourier, hyfield (fd1d_dform_run, fused:
+    segments of 512 cells advance up to 32 steps per launch).  Algorithmic
+    bytes per cell-update: R+W dx ix sx hy (32), R nax..ndx (16), W ex (4),
+    R+W 6 DFT planes (48) -- what one step of the per-step form moves, so
+    `achieved` is an effective rate."""
     import numpy as np
     import torch
     from simulation_amd import fd1d
@@ -452,7 +454,7 @@ def bench_dform(a):
                                f"(fd1d/clang/test_2_3.c)", "grid": [nx], "freq": list(freq)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 5), "traffic": None,
-                     "kernel": "per-step kernels of fd1d_dform_run (one step)",
+                     "kernel": "dform_tb_kernel (fused fd1d_dform_run), effective per step",
                      "avg_launch_ms": round(step_ms, 4), "alg_bytes_per_launch": b_alg * nx},
         "cpu_baseline": cpu,
     }

@@ -474,7 +474,14 @@ typedef struct fd1d_dform_desc {
     int probe_k[FDTD_MAX_PROBES];
     float *probe_out;
     int probe_len;
+    float *work;                /* NULL: the per-step kernels; else fused     */
+    size_t work_elems;          /* >= fd1d_dform_work_elems(nx)               */
 } fd1d_dform_desc;
+/* With work, the run is fused: workgroups of 512 cells advance up to 32
+ * steps per launch in LDS and registers (nf <= 3), reading one state set and
+ * writing the other (work holds the partner set of dx ix sx ex hy bc), with
+ * the same bits as the per-step kernels.                                    */
+size_t fd1d_dform_work_elems(int nx);
 int fd1d_dform_run(const fd1d_dform_desc *d, int t0, int nsteps, void *stream);
 
 /* ------------------------------------------ plane wave (TF/SF) and DFT

@@ -101,6 +101,7 @@ class Fd1dDformDesc(C.Structure):
         ("ft", FdtdFtrans),
         ("nprobe", C.c_int), ("probe_k", C.c_int * MAX_PROBES),
         ("probe_out", _f), ("probe_len", C.c_int),
+        ("work", _f), ("work_elems", C.c_size_t),
     ]
 
 
@@ -178,6 +179,7 @@ SIGNATURES = {
     "fd1d_fourier": (_I, [_I, _I, _I, _F, C.POINTER(C.c_float), _P, C.POINTER(FdtdFtrans), _P]),
     "fd1d_dielectric_dform": (_I, [_I, _F, _F, _F, _F, _F, C.POINTER(FdtdMedium1d), _P]),
     "fd1d_dform_run": (_I, [C.POINTER(Fd1dDformDesc), _I, _I, _P]),
+    "fd1d_dform_work_elems": (C.c_size_t, [_I]),
     "fd2d_ezinct": (_I, [_I, _P, _P, _P, _P]),
     "fd2d_dfield_inc": (_I, [_I, _I, _I, C.POINTER(FdtdPml), _P, _P, _P, _P, _I, _F, _P]),
     "fd2d_inctdz": (_I, [_I, _I, _I, _P, _P, _P]),

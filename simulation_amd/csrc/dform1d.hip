@@ -100,6 +100,163 @@ __global__ void __launch_bounds__(256) debye_kernel(int nx, float dt, float chi,
     }
 }
 
+// Fused run: each workgroup owns DF_W cells and evolves, for K <= DF_MAXK
+// steps, the region of its cells widened by K on both sides (one step moves
+// information one cell each way: dx[i] takes hy[i-1], hy[i] takes ex[i+1]),
+// so its own cells are exact after K steps.  ex and hy sit in LDS (the only
+// fields read across cells), the rest in registers; per step two barriers.
+// Reads one state set and writes the other (dx ix sx ex hy bc: a block near
+// the right edge may simulate the boundary it does not own); the DFT planes
+// and the incident-probe DFT are updated in place by their single owner.
+// Operands as the per-step kernels above, so the bits are theirs.
+template <int NF>
+__global__ void __launch_bounds__(DF_NT) dform_tb_kernel(const DformTbParams P)
+{
+    constexpr int CPT = (DF_W + 2 * DF_MAXK + DF_NT - 1) / DF_NT;
+    __shared__ float ex_s[DF_W + 2 * DF_MAXK], hy_s[DF_W + 2 * DF_MAXK];
+    const int nx = P.nx, K = P.K, tid = threadIdx.x;
+    const int o0 = blockIdx.x * DF_W, o1 = min(o0 + DF_W, nx);
+    const int r0 = max(o0 - K, 0), r1 = min(o1 + K, nx), R = r1 - r0;
+    const bool debye = P.ncx != nullptr;
+    const bool left_abc = r0 == 0, right_abc = r1 == nx;   // the region holds that grid edge
+    float dx[CPT], ix[CPT], sx[CPT], na[CPT], nb[CPT], nc[CPT], nd[CPT];
+    float fr[CPT][NF ? NF : 1], fi[CPT][NF ? NF : 1];
+    float b0 = 0.f, b1 = 0.f, b2 = 0.f, b3 = 0.f;          // bc of the edge cells' threads
+    float rin[NF ? NF : 1], iin[NF ? NF : 1];
+#pragma unroll
+    for (int c = 0; c < CPT; ++c) {
+        const int j = tid + c * DF_NT, i = r0 + j;
+        dx[c] = ix[c] = sx[c] = na[c] = nb[c] = nc[c] = nd[c] = 0.f;
+#pragma unroll
+        for (int f = 0; f < (NF ? NF : 1); ++f) { fr[c][f] = 0.f; fi[c][f] = 0.f; }
+        if (j < R) {
+            dx[c] = P.dx_i[i]; ix[c] = P.ix_i[i];
+            na[c] = P.nax[i]; nb[c] = P.nbx[i];
+            if (debye) { sx[c] = P.sx_i[i]; nc[c] = P.ncx[i]; nd[c] = P.ndx[i]; }
+            ex_s[j] = P.ex_i[i];
+            hy_s[j] = P.hy_i[i];
+            if (i >= o0 && i < o1) {
+#pragma unroll
+                for (int f = 0; f < NF; ++f)
+                    if (f < P.nf) {
+                        fr[c][f] = P.r_pt[(long)f * nx + i];
+                        fi[c][f] = P.i_pt[(long)f * nx + i];
+                    }
+            }
+            if (i == 0 && left_abc) { b0 = P.bc_i[0]; b1 = P.bc_i[1]; }
+            if (i == nx - 2 && right_abc) { b2 = P.bc_i[2]; b3 = P.bc_i[3]; }
+        }
+    }
+    const bool own_inc = NF > 0 && P.r_in && o0 <= DF_INC_PROBE && DF_INC_PROBE < o1;
+#pragma unroll
+    for (int f = 0; f < (NF ? NF : 1); ++f) { rin[f] = 0.f; iin[f] = 0.f; }
+    if (own_inc && DF_INC_PROBE - r0 >= 0 && (DF_INC_PROBE - r0) % DF_NT == tid) {
+#pragma unroll
+        for (int f = 0; f < NF; ++f)
+            if (f < P.nf) { rin[f] = P.r_in[f]; iin[f] = P.i_in[f]; }
+    }
+    __syncthreads();
+    for (int s = 0; s < K; ++s) {
+        // D update, source, E update, DFT (ex of every cell after the E update)
+#pragma unroll
+        for (int c = 0; c < CPT; ++c) {
+            const int j = tid + c * DF_NT, i = r0 + j;
+            if (j >= R) continue;
+            if (i >= 1) {
+                const float hl = hy_s[j > 0 ? j - 1 : 0];          // j = 0: a cut (stale)
+                float d = dx[c] + 0.5f * (hl - hy_s[j]);
+                if (i == P.src_pos) d = d + P.src_val[s];
+                dx[c] = d;
+                float e;
+                if (debye) {
+                    const float sv = sx[c];
+                    e = na[c] * (d - ix[c] - nc[c] * sv);
+                    ix[c] = ix[c] + nb[c] * e;
+                    sx[c] = nc[c] * sv + nd[c] * e;
+                } else {
+                    e = na[c] * (d - ix[c]);
+                    ix[c] = ix[c] + nb[c] * e;
+                }
+                ex_s[j] = e;
+            }
+            const float ev = ex_s[j];
+#pragma unroll
+            for (int f = 0; f < NF; ++f)
+                if (f < P.nf) {
+                    fr[c][f] = fr[c][f] + P.tw_c[s][f] * ev;
+                    fi[c][f] = fi[c][f] - P.tw_s[s][f] * ev;
+                }
+            if (own_inc && i == DF_INC_PROBE) {
+#pragma unroll
+                for (int f = 0; f < NF; ++f)
+                    if (f < P.nf) {
+                        rin[f] = rin[f] + P.tw_c[s][f] * ev;
+                        iin[f] = iin[f] - P.tw_s[s][f] * ev;
+                    }
+            }
+        }
+        __syncthreads();
+        // two-step boundary at the grid edges, H update, probes
+#pragma unroll
+        for (int c = 0; c < CPT; ++c) {
+            const int j = tid + c * DF_NT, i = r0 + j;
+            if (j >= R || i >= nx - 1) continue;
+            float e0 = ex_s[j];
+            float e1 = ex_s[j + 1 < R ? j + 1 : j];                // the right cut (stale)
+            if (i == 0 && left_abc) {
+                const float ob0 = b0;
+                ex_s[0] = ob0;
+                b0 = b1;
+                b1 = e1;
+                e0 = ob0;
+            }
+            if (i == nx - 2 && right_abc) {
+                const float ob3 = b3;
+                ex_s[j + 1] = ob3;
+                b3 = b2;
+                b2 = e0;
+                e1 = ob3;
+            }
+            hy_s[j] = hy_s[j] + 0.5f * (e0 - e1);
+            if (i >= o0 && i < o1) {
+                for (int p = 0; p < P.nprobe; ++p)
+                    if (P.probe_k[p] == i)
+                        P.probe_out[(long)p * P.probe_len + (P.t_first - 1 + s)] = e0;
+            }
+            if (i == nx - 2 && right_abc && nx - 1 >= o0 && nx - 1 < o1) {
+                for (int p = 0; p < P.nprobe; ++p)
+                    if (P.probe_k[p] == nx - 1)
+                        P.probe_out[(long)p * P.probe_len + (P.t_first - 1 + s)] = e1;
+            }
+        }
+        __syncthreads();
+    }
+    // the owned cells go out
+#pragma unroll
+    for (int c = 0; c < CPT; ++c) {
+        const int j = tid + c * DF_NT, i = r0 + j;
+        if (j >= R || i < o0 || i >= o1) continue;
+        P.dx_o[i] = dx[c];
+        P.ix_o[i] = ix[c];
+        if (debye) P.sx_o[i] = sx[c];
+        P.ex_o[i] = ex_s[j];
+        P.hy_o[i] = hy_s[j];
+#pragma unroll
+        for (int f = 0; f < NF; ++f)
+            if (f < P.nf) {
+                P.r_pt[(long)f * nx + i] = fr[c][f];
+                P.i_pt[(long)f * nx + i] = fi[c][f];
+            }
+        if (i == 0) { P.bc_o[0] = b0; P.bc_o[1] = b1; }
+        if (i == nx - 2) { P.bc_o[2] = b2; P.bc_o[3] = b3; }
+        if (own_inc && i == DF_INC_PROBE) {
+#pragma unroll
+            for (int f = 0; f < NF; ++f)
+                if (f < P.nf) { P.r_in[f] = rin[f]; P.i_in[f] = iin[f]; }
+        }
+    }
+}
+
 inline int nblk(long n) { return (int)((n + 255) / 256); }
 
 }  // namespace
@@ -142,6 +299,14 @@ int launch_debye(int nx, float dt, float chi, float tau, float epsr, float sigma
     hipLaunchKernelGGL(debye_kernel, dim3(nblk(nx)), dim3(256), 0, s, nx, dt, chi, tau, epsr, sigma,
                        ncx_val, md.nax, md.nbx, md.ncx, md.ndx);
     return hip_check(hipGetLastError(), "dielectric (Debye)");
+}
+
+int launch_dform_tb(const DformTbParams &p, hipStream_t s)
+{
+    const dim3 grid((p.nx + DF_W - 1) / DF_W);
+    if (p.nf) hipLaunchKernelGGL(dform_tb_kernel<DF_NF>, grid, dim3(DF_NT), 0, s, p);
+    else hipLaunchKernelGGL(dform_tb_kernel<0>, grid, dim3(DF_NT), 0, s, p);
+    return hip_check(hipGetLastError(), "fd1d_dform_run launch");
 }
 
 }  // namespace fdtd

@@ -640,6 +640,51 @@ int fd1d_dielectric_dform(int nx, float dt, float chi, float tau, float epsr, fl
     return launch_debye(nx, dt, chi, tau, epsr, sigma, *md, S(stream));
 }
 
+size_t fd1d_dform_work_elems(int nx)
+{
+    return nx < 1 ? 0 : 5 * (size_t)nx + 4;
+}
+
+// The fused D/E-form run: an even number L of launches of up to DF_MAXK
+// steps, ping-ponging between the caller's state set and the partner set in
+// work (so the result ends in the caller's arrays).
+static int dform_fused(const fd1d_dform_desc *d, int t0, int nsteps, hipStream_t s)
+{
+    const int nx = d->nx;
+    int L = (nsteps + DF_MAXK - 1) / DF_MAXK;
+    L += L % 2;
+    float *w = d->work;
+    float *A[6] = {d->dx, d->ix, d->sx, d->ex, d->hy, d->bc};
+    float *B[6] = {w, w + nx, w + 2 * (size_t)nx, w + 3 * (size_t)nx, w + 4 * (size_t)nx,
+                   w + 5 * (size_t)nx};
+    int done = 0;
+    for (int l = 0; l < L; ++l) {
+        const int k = nsteps / L + (l < nsteps % L ? 1 : 0);
+        float **in = (l % 2 == 0) ? A : B;
+        float **out = (l % 2 == 0) ? B : A;
+        DformTbParams p{};
+        p.nx = nx; p.t_first = t0 + done + 1; p.K = k;
+        p.dx_i = in[0]; p.ix_i = in[1]; p.sx_i = in[2]; p.ex_i = in[3]; p.hy_i = in[4]; p.bc_i = in[5];
+        p.dx_o = out[0]; p.ix_o = out[1]; p.sx_o = out[2]; p.ex_o = out[3]; p.hy_o = out[4]; p.bc_o = out[5];
+        p.nax = d->md.nax; p.nbx = d->md.nbx; p.ncx = d->md.ncx; p.ndx = d->md.ndx;
+        p.src_pos = d->src_pos >= 1 ? d->src_pos : -1;
+        p.nf = d->nf;
+        p.r_pt = d->ft.r_pt; p.i_pt = d->ft.i_pt; p.r_in = d->ft.r_in; p.i_in = d->ft.i_in;
+        for (int q = 0; q < k; ++q) {
+            const int t = p.t_first + q;
+            p.src_val[q] = p.src_pos >= 1 ? d->src_wave[t - 1] : 0.0f;
+            for (int f = 0; f < d->nf; ++f) dft_twiddle(t, d->freq[f], d->dt, &p.tw_c[q][f], &p.tw_s[q][f]);
+        }
+        p.nprobe = d->nprobe;
+        for (int q = 0; q < d->nprobe; ++q) p.probe_k[q] = d->probe_k[q];
+        p.probe_out = d->probe_out; p.probe_len = d->probe_len;
+        int rc = launch_dform_tb(p, s);
+        if (rc) return rc;
+        done += k;
+    }
+    return 0;
+}
+
 int fd1d_dform_run(const fd1d_dform_desc *d, int t0, int nsteps, void *stream)
 {
     clear_error();
@@ -665,6 +710,15 @@ int fd1d_dform_run(const fd1d_dform_desc *d, int t0, int nsteps, void *stream)
             if (d->probe_k[p] < 0 || d->probe_k[p] >= nx) return fail("fd1d_dform_run: probe %d", p);
     }
     hipStream_t s = S(stream);
+    if (d->work && nsteps >= 2) {
+        if (d->work_elems < fd1d_dform_work_elems(nx))
+            return fail("fd1d_dform_run: work has %zu floats, %zu needed", d->work_elems,
+                        fd1d_dform_work_elems(nx));
+        if (d->nf > DF_NF)
+            return fail("fd1d_dform_run: the fused path takes at most %d frequencies (pass work = "
+                        "NULL for the per-step path)", DF_NF);
+        return dform_fused(d, t0, nsteps, s);
+    }
     int zeros[FDTD_MAX_PROBES] = {0};
     for (int k = 0; k < nsteps; ++k) {
         const int t = t0 + k + 1;

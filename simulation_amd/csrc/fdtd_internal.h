@@ -120,6 +120,28 @@ int launch_fourier1d(int t, int nf, int nx, float dt, const float *freq, const f
                      const fdtd_ftrans &ft, hipStream_t s);
 int launch_debye(int nx, float dt, float chi, float tau, float epsr, float sigma,
                  const fdtd_medium1d &md, hipStream_t s);
+// 1-D D/E form, fused run (dform1d.hip): DF_W cells per workgroup, up to
+// DF_MAXK steps per launch, DFT at up to DF_NF frequencies.
+constexpr int DF_NT = 256, DF_W = 512, DF_MAXK = 32, DF_NF = 3;
+constexpr int DF_INC_PROBE = 10;    // incident-probe cell of the 1-D DFT (r_in, i_in)
+struct DformTbParams {
+    int nx, t_first, K;             // steps t_first .. t_first + K - 1
+    const float *dx_i, *ix_i, *sx_i, *ex_i, *hy_i;
+    float *dx_o, *ix_o, *sx_o, *ex_o, *hy_o;
+    const float *bc_i;
+    float *bc_o;
+    const float *nax, *nbx, *ncx, *ndx;   // ncx NULL: the lossy medium
+    int src_pos;                    // < 1: no source
+    float src_val[DF_MAXK];
+    int nf;
+    float *r_pt, *i_pt, *r_in, *i_in;
+    float tw_c[DF_MAXK][DF_NF], tw_s[DF_MAXK][DF_NF];
+    int nprobe;
+    int probe_k[FDTD_MAX_PROBES];
+    float *probe_out;
+    int probe_len;
+};
+int launch_dform_tb(const DformTbParams &p, hipStream_t s);
 // plane wave (tfsf.hip)
 constexpr int INC_CHUNK = 64;       // steps of incident grid per fused chunk
 struct IncChunk {

@@ -226,10 +226,12 @@ class DFormSolver:
     """The 1-D D/E-form programs (fd1d/pycuda/fd1d_2_2.py, fd1d_2_3.py): a
     pulse added at src_pos striking a lossy (or Debye) half space, with the
     running DFT at freq, through fd1d_dform_run.  src_wave: host values of
-    the additive source for steps 1.. (default gaussian(t, 50, 10))."""
+    the additive source for steps 1.. (default gaussian(t, 50, 10)).
+    fused: the segment-blocked launches (up to 32 steps each, nf <= 3) with
+    the partner state set in a work buffer; else the per-step kernels."""
 
     def __init__(self, nx, md: medium1d, nsteps_max, freq=(), dt=None, src_pos=1,
-                 src_wave=None, probes=(), device=None):
+                 src_wave=None, probes=(), device=None, fused=True):
         dev = device or require_gpu()
         self.nx, self.md, self.device = int(nx), md, dev
         z = lambda: torch.zeros(nx, dtype=torch.float32, device=dev)  # noqa: E731
@@ -252,6 +254,8 @@ class DFormSolver:
         self.probe_out = (torch.zeros((len(probes), self.probe_len), dtype=torch.float32,
                                       device=dev) if probes else None)
         self.t = 0
+        n = _lib.lib().fd1d_dform_work_elems(int(nx)) if fused and nf <= 3 else 0
+        self.work = torch.empty(n, dtype=torch.float32, device=dev) if n else None
 
     def desc(self) -> _lib.Fd1dDformDesc:
         d = _lib.Fd1dDformDesc()
@@ -272,6 +276,8 @@ class DFormSolver:
             d.probe_k[p] = k
         d.probe_out = fptr(self.probe_out)
         d.probe_len = self.probe_len
+        d.work = fptr(self.work)
+        d.work_elems = 0 if self.work is None else self.work.numel()
         return d
 
     def run(self, nsteps: int) -> None:

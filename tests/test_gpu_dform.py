@@ -73,3 +73,61 @@ def test_dform_step_kernels_bit_exact(gpu, name):
     _check(c, s)
     a, p = s.amplitude_phase()
     assert a.shape == (nf, nx) and np.all(np.isfinite(a))
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_dform_run_per_step_path_bit_exact(gpu, name):
+    """The per-step kernels of fd1d_dform_run (no work buffer) against the
+    reference fixtures."""
+    from simulation_amd import fd1d
+    c = dform_case(name, _gauss)
+    print(f"[dform] per-step run {name}", flush=True)
+    m = c["meta"]
+    md = fd1d.dielectric_dform(c["nx"], float(c["dt"]), m["chi"], m["tau"], m["epsr"], m["sigma"],
+                               debye=c["debye"], device=gpu)
+    s = fd1d.DFormSolver(c["nx"], md, c["ns"], freq=c["freq"], dt=float(c["dt"]),
+                         src_pos=c["src_pos"], src_wave=c["wave"], probes=c["probes"], device=gpu,
+                         fused=False)
+    assert s.work is None
+    s.run(c["ns"])
+    _check(c, s)
+    torch.cuda.synchronize()
+    assert np.array_equal(s.probe_out.cpu().numpy(), c["exp"]["out_probe"])
+
+
+@pytest.mark.parametrize("nx,debye,nf,chunks", [
+    (1029, True, 3, (1, 70, 99)),      # three segments, the last owning 5 cells
+    (1600, False, 3, (64, 33)),
+    (1100, True, 0, (2, 97)),
+    (2048, True, 2, (128,)),
+])
+def test_fused_matches_per_step(gpu, nx, debye, nf, chunks):
+    """The fused run (segments of 512 cells, up to 32 steps per launch, the
+    boundary simulated by every segment whose region reaches a grid edge)
+    against the per-step kernels: every field, bc, the DFT planes and the
+    incident-probe DFT, probes on both edges, the incident probe cell and
+    segment boundaries."""
+    from simulation_amd import fd1d
+    dt = float(np.float32(np.float32(0.01) / np.float32(6e8)))
+    freq = (50e6, 200e6, 500e6)[:nf]
+    ns = sum(chunks)
+    probes = (0, nx - 1, 10, 511, 512, nx // 2, nx - 3)
+    print(f"[dform] fused vs per-step nx={nx} debye={debye} nf={nf} chunks={chunks}", flush=True)
+
+    def make(fused):
+        md = fd1d.dielectric_dform(nx, dt, 2.0, 0.001e-6, 2.0, 0.01, debye=debye, device=gpu)
+        return fd1d.DFormSolver(nx, md, ns, freq=freq, dt=dt, probes=probes, device=gpu,
+                                fused=fused)
+    a, b = make(True), make(False)
+    assert a.work is not None and b.work is None
+    for n in chunks:
+        a.run(n)
+        b.run(n)
+    torch.cuda.synchronize()
+    for k in ("dx", "ix", "ex", "hy", "bc") + (("sx",) if debye else ()):
+        assert torch.equal(getattr(a, k), getattr(b, k)), k
+    if nf:
+        for k in ("r_pt", "i_pt", "r_in", "i_in"):
+            assert torch.equal(getattr(a.ft, k), getattr(b.ft, k)), k
+    assert torch.equal(a.probe_out, b.probe_out)
+    assert a.ex.abs().max().item() > 0.1
