@@ -398,7 +398,9 @@ def bench_tfsf(a):
         "config": {"workload": f"2-D TM plane wave {n}x{n} FP32, npml={npml} PML, TF/SF box, "
                                f"cylinder r={rgrid} eps_r=30 sigma=0.3, DFT at 3 frequencies "
                                f"(fd2d/clang/test_3_4.c)", "grid": [n, n], "npml": npml,
-                   "freq": list(freq), "depth": s.tm.depth or 8, "rows": s.tm.rows, "cols": s.tm.cols or 1,
+                   "freq": list(freq), "depth": s.tm.depth or (4 if len(freq) else 8),
+                   "rows": s.tm.rows or (max(4, min(512, n // 220)) if len(freq) else max(16, min(512, n // 220))),
+                   "cols": s.tm.cols or 1,
                    "per_step_kernels_value": round(per_step, 1)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4), "traffic": None,

@@ -838,6 +838,17 @@ static int tfsf_fused(const fd2d_tm_desc *d, const fd2d_tfsf_desc *b, int t0, in
     const int stride = d->ny - 2 * b->npml + 4;
     int rc = 0;
     int done = 0;
+    // with the DFT a level carries 6 more registers per column: default depth
+    // 4 and short row blocks (more waves; MI355X sweeps at 1024^2, npml 80, 3
+    // frequencies: 38k Mcell-updates/s at depth 8 rows 16, 60k at depth 4
+    // rows 4)
+    fd2d_tm_desc dd = *d;
+    if (b->nf && !dd.depth) dd.depth = 4;
+    if (b->nf && !dd.rows) {
+        const int h = d->nx / 220;
+        dd.rows = h < 4 ? 4 : (h > 512 ? 512 : h);
+    }
+    d = &dd;
     while (done < nsteps) {
         const int n = nsteps - done < INC_CHUNK ? nsteps - done : INC_CHUNK;
         const int tc = t0 + done;                  // steps tc+1 .. tc+n
