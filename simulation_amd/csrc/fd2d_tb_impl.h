@@ -756,7 +756,13 @@ tm_tb_batch_kernel(const fd2d_grid *__restrict__ grids, const int *__restrict__ 
                    int parity, int t_first, int write_ez, int halo, int rows)
 {
     constexpr int S = 64 * C;
-    for (int u = blockIdx.x; u < nunits; u += gridDim.x) {
+    // XCD-aware slots: wave w (on XCD w % 8) takes the units slot, slot +
+    // gridDim.x, ...; the slots of one XCD are contiguous, so in each round
+    // the neighbouring units of a grid (strips fastest in the list) run on
+    // the same XCD and share its L2
+    const unsigned nw = gridDim.x, xcd = blockIdx.x % 8u, q = nw / 8u, rem = nw % 8u;
+    const unsigned slot = xcd * q + min(xcd, rem) + blockIdx.x / 8u;
+    for (int u = (int)slot; u < nunits; u += (int)nw) {
         const int gi = units[4 * u], sx = units[4 * u + 1], by = units[4 * u + 2];
         const fd2d_grid &G = grids[gi];
         TbParams P;
