@@ -283,7 +283,7 @@ def cell(e: int, d: int, t: int, nx: int, ny: int):
     return [(d, t), (nx - 1 - d, t), (t, d), (t, ny - 1 - d)][e]
 
 
-def sample(ez, hx, hy, nx: int, ny: int) -> np.ndarray:
+def sample(ez, hx, hy, nx: int, ny: int, dtype=np.float32) -> np.ndarray:
     """Canonical samples (npos, 30) of the current fields: 27 features of
     grid depths D = 3, 4, 5 (feature depth d = D - 3; the cells just inside
     the three predicted ones, so the network never reads its own output):
@@ -298,10 +298,10 @@ def sample(ez, hx, hy, nx: int, ny: int) -> np.ndarray:
     either); tangential indices are clamped to the edge.  Then ez of grid
     depths 2, 1, 0 at offset 0 (targets and baseline, not network inputs)."""
     pos = positions(nx, ny)
-    out = np.zeros((len(pos), NSAMP), np.float32)
+    out = np.zeros((len(pos), NSAMP), dtype)
 
     def H(a, i, j):
-        return np.float32(0.0) if (i == nx - 1 or j == ny - 1) else a[i, j]
+        return dtype(0.0) if (i == nx - 1 or j == ny - 1) else a[i, j]
 
     def ezc(e, D, t):
         i, j = cell(e, D, t, nx, ny)
@@ -348,21 +348,45 @@ def predict(params, hist):
     return pred
 
 
+def predict_f64(params, hist):
+    """The same prediction evaluated in FP64 throughout (no bf16 or FP32
+    rounding points: the exact function of the FP32 master weights), for the
+    error of the FP32 path: base + s * (L xf + (f(xf) - f(-xf))/2)."""
+    X = np.concatenate([np.asarray(h[:, :NFEAT], np.float64) for h in hist], axis=1)
+    s = np.max(np.abs(X), axis=1)
+    big = s > float(S_MIN)
+    inv = np.where(big, 1.0 / np.where(big, s, 1.0), 0.0)
+    xf = X * inv[:, None]
+    P = unpack(params)
+    ident = lambda a: np.asarray(a, np.float64)  # noqa: E731
+    yp, _ = _branch(P, xf, ident, False)
+    ym, _ = _branch(P, -xf, ident, False)
+    y = xf @ P["L"].astype(np.float64).T + 0.5 * (yp - ym)
+    S1 = np.asarray(hist[1], np.float64)
+    b = np.stack([S1[:, S_EZ1], S1[:, S_EZ2], S1[:, sidx(0, 0, 0)]], axis=1)
+    pred = b + s[:, None] * y
+    pred[~big] = b[~big]
+    return pred
+
+
 def lbc_run(state: dict, naz, nbz, params, nsteps: int, hist=None, sources=(), src_wave=None,
             t0: int = 0):
     """Advance a learned-boundary grid (no PML) by nsteps in place.
 
-    state {dz, iz, ez, hx, hy} FP32; hist: list of 4 samples (newest first)
+    state {dz, iz, ez, hx, hy} FP32 (or FP64: the whole step, samples and
+    prediction in FP64, predict_f64); hist: list of 4 samples (newest first)
     or None for a start from rest; returns the updated history."""
     dz, iz, ez, hx, hy = (state[k] for k in ("dz", "iz", "ez", "hx", "hy"))
     nx, ny = ez.shape
+    f64 = ez.dtype == np.float64
+    dt = np.float64 if f64 else np.float32
     pos = positions(nx, ny)
     if hist is None:
-        z = np.zeros((len(pos), NSAMP), np.float32)
+        z = np.zeros((len(pos), NSAMP), dt)
         hist = [z, z.copy(), z.copy(), z.copy()]
     hist = list(hist)
     for s in range(nsteps):
-        pred = predict(params, hist)
+        pred = predict_f64(params, hist) if f64 else predict(params, hist)
         NO.dfield(nx, ny, None, dz, hx, hy)
         for k, (i, j) in enumerate(sources):
             dz[i, j] = src_wave[k][t0 + s]
@@ -371,5 +395,5 @@ def lbc_run(state: dict, naz, nbz, params, nsteps: int, hist=None, sources=(), s
             for d in range(3):
                 ez[cell(e, d, t, nx, ny)] = pred[p, d]
         NO.hfield(nx, ny, None, ez, None, None, hx, hy)
-        hist = [sample(ez, hx, hy, nx, ny)] + hist[:3]
+        hist = [sample(ez, hx, hy, nx, ny, dt)] + hist[:3]
     return hist

@@ -138,3 +138,32 @@ def test_fd2d_c3_grids_fp32_vs_fp64(gpu, oracle):
         r["probe"] = _rel(b.probe_trace(k), pr[0])
         _report(f"C3 batch grid {pick[k]} ({sp.nx}^2)", steps=ns, rel_l2=r)
         assert r["probe"] < 1e-4
+
+
+def test_fd2d_c4_learned_fp32_vs_fp64(gpu, oracle):
+    """The learned boundary (shipped network) on a 128 x 128 grid with a lossy
+    cylinder and a hard Gaussian at the centre, 300 steps (the pulse reaches
+    and leaves through the learned frame): the GPU's FP32 run against the FP64
+    evaluation of the same scheme (oracle/lbc_oracle.py: lbc_run with FP64
+    state, samples and prediction, FP64 coefficients), whole grid."""
+    from oracle import lbc_oracle as L
+    from simulation_amd import fd1d, fd2d, learned
+    n, ns, c = 128, 300, 64
+    print(f"[fp64] C4 learned {n}x{n} {ns} steps", flush=True)
+    net = learned.load(learned.WEIGHTS, gpu)
+    cyl = [(n / 2 + 10.5, n / 2 - 12.0, 14.0, 4.0, 0.02)]
+    dt = fd2d.time_step()
+    md = fd2d.dielectric(n, n, 0, 0, dt, 1.0, 0.0, gpu, cylinders=cyl)
+    w32 = fd1d.waveform("gaussian", ns, 0, np.float32, center=20, spread=6.0)
+    s = learned.LearnedTMSolver(n, n, net, md, [(c, c, w32)], [], nsteps_max=ns, device=gpu)
+    s.run(ns)
+    st = s.state()
+    n64, b64 = oracle.fd2d_dielectric(n, n, 0, dt, cyl)
+    w64 = oracle.waveform("gaussian", 0, ns, np.float64, center=20, spread=6.0)
+    o = {k: np.zeros((n, n), np.float64) for k in ("dz", "iz", "ez", "hx", "hy")}
+    L.lbc_run(o, n64, b64, net.numpy(), ns, None, [(c, c)], [w64])
+    rel = {f: _rel(st[f], o[f]) for f in FIELDS}
+    _report("C4 learned boundary 128^2, shipped network", steps=ns, window="whole grid",
+            rel_l2=rel)
+    assert np.abs(o["ez"]).max() > 1e-3
+    assert all(v < 1e-3 for v in rel.values())

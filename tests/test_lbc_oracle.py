@@ -198,3 +198,27 @@ def test_linear_path_is_scale_free_and_odd():
     ym, _ = L.forward(p, -X)
     assert np.array_equal(y, y4)
     assert np.array_equal(y, -ym)
+
+
+def test_predict_f64_pins():
+    """The FP64 prediction (the FP64 error reference of the learned form)
+    against the FP32 one: a zero network gives the baseline exactly in both;
+    a network whose hidden branch is silenced (W3 = 0) differs only by the
+    FP32 rounding of the linear path; a generic network by the bf16 rounding
+    points of the hidden layers."""
+    rng = np.random.default_rng(3)
+    hist = [rng.standard_normal((64, L.NSAMP)).astype(np.float32) for _ in range(4)]
+    z = np.zeros(L.NPARAM, np.float32)
+    b = L.base(hist[1])
+    assert np.array_equal(L.predict(z, hist), b)
+    assert np.array_equal(L.predict_f64(z, hist), b.astype(np.float64))
+    p = L.init_params(5)
+    d = L.unpack(p)
+    d["W3"][:] = 0.0
+    d["L"][:] = rng.standard_normal(d["L"].shape).astype(np.float32) * 0.05
+    q = L.pack(d)
+    a32, a64 = L.predict(q, hist), L.predict_f64(q, hist)
+    assert np.allclose(a32, a64, rtol=1e-5, atol=1e-6)
+    assert not np.array_equal(a64, b.astype(np.float64))
+    g32, g64 = L.predict(p, hist), L.predict_f64(p, hist)
+    assert np.abs(g32 - g64).max() < 0.05 * np.abs(g64).max()
