@@ -30,11 +30,12 @@
 //     run the general body.  Multiplying by a constant 1 is exact, so the
 //     compiler may drop it without changing any result; products with a
 //     constant 0 are kept.
-//   * The interior and the other waves are two kernels (two launches over
-//     the same grid, each wave returning at once if it belongs to the other
-//     launch), so the common interior body is not limited by the register
-//     needs of the general one.  The host runs the second on an auxiliary
-//     stream, concurrently with the first.
+//   * One launch covers a K-step group: the host lists the tiles that are
+//     not interior (row blocks, strips, source and probe tiles), their
+//     workgroups come first, then one workgroup per tile of the grid in an
+//     XCD-aware order, computing its tile if it is interior.  The learned
+//     boundary instead runs the interior and the listed tiles as two
+//     launches on two streams, beside its frame kernel.
 //   * Launches read one state set and write the other (ping-pong), so waves
 //     never read a cell another wave has already advanced.
 //
@@ -408,30 +409,30 @@ __device__ __forceinline__ void iterate(const TbParams &P, const Geom &g, const 
                     st_rb<C>(out_rsrc(P.ihx_o, nb), rowq, g.j0, c0, c1, okq, Hout.ix);
                     st_rb<C>(out_rsrc(P.ihy_o, nb), rowq, g.j0, c0, c1, okq, Hout.iy);
                 }
-                continue;
-            }
-            if (r >= g.i0 && r < g.i1 && r >= sw && r < P.nx - sw) {
-                const unsigned row = (unsigned)r * (unsigned)ny;
-                st<C, VEC>(P.dz_o, row, g.j0, c0, c1, Dn.dz);
-                if (LOSSY) st<C, VEC>(P.iz_o, row, g.j0, c0, c1, Dn.iz);
-                if (P.ez_o) st<C, VEC>(P.ez_o, row, g.j0, c0, c1, En);
-                if constexpr (NF > 0) {
-                    const size_t plane = (size_t)P.nx * (size_t)ny;
+            } else {
+                if (r >= g.i0 && r < g.i1 && r >= sw && r < P.nx - sw) {
+                    const unsigned row = (unsigned)r * (unsigned)ny;
+                    st<C, VEC>(P.dz_o, row, g.j0, c0, c1, Dn.dz);
+                    if (LOSSY) st<C, VEC>(P.iz_o, row, g.j0, c0, c1, Dn.iz);
+                    if (P.ez_o) st<C, VEC>(P.ez_o, row, g.j0, c0, c1, En);
+                    if constexpr (NF > 0) {
+                        const size_t plane = (size_t)P.nx * (size_t)ny;
 #pragma unroll
-                    for (int f = 0; f < NF; ++f)
-                        if (f < P.nf) {
-                            st<C, VEC>(P.ft_r + f * plane, row, g.j0, c0, c1, Fn.r[f]);
-                            st<C, VEC>(P.ft_i + f * plane, row, g.j0, c0, c1, Fn.i[f]);
-                        }
+                        for (int f = 0; f < NF; ++f)
+                            if (f < P.nf) {
+                                st<C, VEC>(P.ft_r + f * plane, row, g.j0, c0, c1, Fn.r[f]);
+                                st<C, VEC>(P.ft_i + f * plane, row, g.j0, c0, c1, Fn.i[f]);
+                            }
+                    }
                 }
-            }
-            if (q >= g.i0 && q < g.i1 && q >= sw && q < P.nx - sw) {
-                const unsigned row = (unsigned)q * (unsigned)ny;
-                st<C, VEC>(P.hx_o, row, g.j0, c0, c1, Hout.hx);
-                st<C, VEC>(P.hy_o, row, g.j0, c0, c1, Hout.hy);
-                if (PML) {
-                    st<C, VEC>(P.ihx_o, row, g.j0, c0, c1, Hout.ix);
-                    st<C, VEC>(P.ihy_o, row, g.j0, c0, c1, Hout.iy);
+                if (q >= g.i0 && q < g.i1 && q >= sw && q < P.nx - sw) {
+                    const unsigned row = (unsigned)q * (unsigned)ny;
+                    st<C, VEC>(P.hx_o, row, g.j0, c0, c1, Hout.hx);
+                    st<C, VEC>(P.hy_o, row, g.j0, c0, c1, Hout.hy);
+                    if (PML) {
+                        st<C, VEC>(P.ihx_o, row, g.j0, c0, c1, Hout.ix);
+                        st<C, VEC>(P.ihy_o, row, g.j0, c0, c1, Hout.iy);
+                    }
                 }
             }
         }
@@ -626,6 +627,7 @@ int tb_tile_lists(TbParams &q)
     const int nsx = (q.ny + w - 1) / w, nrb = (q.nx + q.rows - 1) / q.rows;
     q.n_rb = q.n_sx = q.n_t = 0;
     auto fail_list = [&]() { q.n_rb = -1; q.n_sx = q.n_t = 0; return -1; };
+    if (nsx >= 0x10000 || nrb >= 0x10000) return fail_list();    // t_list packs 16 + 16 bits
     const int m = q.tf_npml;
     for (int b = 0; b < nrb; ++b) {
         const int i0 = b * q.rows, i1 = std::min(i0 + q.rows, q.nx);
@@ -677,7 +679,6 @@ int tb_tile_lists(TbParams &q)
         if (!add_cell(q.src_i[k], q.src_j[k], 2 * K, K + 1)) return fail_list();
     for (int k = 0; k < q.nprobe; ++k)
         if (!add_cell(q.probe_i[k], q.probe_j[k], 0, 0)) return fail_list();
-    if (nsx >= 0x10000 || nrb >= 0x10000) return fail_list();
     return q.n_rb * nsx + q.n_sx * nrb + q.n_t;
 }
 
