@@ -38,6 +38,12 @@
 //     launches on two streams, beside its frame kernel.
 //   * Launches read one state set and write the other (ping-pong), so waves
 //     never read a cell another wave has already advanced.
+//   * Interior waves keep two sets of level carries and alternate them
+//     between sweep iterations (no register-to-register copies), and all
+//     waves without DFT planes store through buffer resources whose range
+//     check drops the cells a wave does not own, so every iteration issues
+//     the same stores and the wait for the next row's loads does not wait
+//     for them.
 //
 // HBM traffic per cell per launch (lossy PML form): reads dz iz hx hy ihx ihy
 // naz nbz (32 B, x halo/warm-up overlap), writes dz iz hx hy ihx ihy (24 B,
@@ -213,15 +219,19 @@ __device__ __forceinline__ void col_coef(const TbParams &P, int j0, ColCoef<C> &
 }
 
 // One sweep iteration at position p: level s computes the E update of row
-// p-s and the H update of row p-s-1.  BROW: rows may lie outside [1, nx-2] or
-// outside the default band; BCOL: the same for the strip's columns; SP: a
-// source or probe lies in the rows and columns the wave computes.
+// p-s and the H update of row p-s-1.  The carries of the previous iteration
+// are read from Si (SFi) and this iteration's go to So (SFo); the two may be
+// the same set (each level reads its entries before it replaces them).
+// BROW: rows may lie outside [1, nx-2] or outside the default band; BCOL: the
+// same for the strip's columns; SP: a source or probe lies in the rows and
+// columns the wave computes.
 template <int K, int C, bool LOSSY, bool PML, bool VEC, bool BROW, bool BCOL, bool SP, int NF,
           bool BS>
 __device__ __forceinline__ void iterate(const TbParams &P, const Geom &g, const ColCoef<C> &cc,
                                         int p, const HRow<C> &Hload, const DRow<C> &Dload,
-                                        Carry<C> (&St)[K], const FRow<C, NF> &Fload,
-                                        FRow<C, NF> (&SF)[K])
+                                        const Carry<C> (&Si)[K], Carry<C> (&So)[K],
+                                        const FRow<C, NF> &Fload, const FRow<C, NF> (&SFi)[K],
+                                        FRow<C, NF> (&SFo)[K])
 {
     const int nx = P.nx, ny = P.ny;
     HRow<C> Hin = Hload;
@@ -240,14 +250,14 @@ __device__ __forceinline__ void iterate(const TbParams &P, const Geom &g, const 
             Din = Dload;
             if constexpr (NF > 0) Fin = Fload;
         } else {
-            Din = St[s - 1].d;
-            St[s - 1].d = Dpend;
+            Din = Si[s - 1].d;
+            So[s - 1].d = Dpend;
             if constexpr (NF > 0) {
-                Fin = SF[s - 1];
-                SF[s - 1] = Fpend;
+                Fin = SFi[s - 1];
+                SFo[s - 1] = Fpend;
             }
         }
-        const HRow<C> Hp = St[s].h;
+        const HRow<C> Hp = Si[s].h;
         float gx2r = 1.f, gx3r = 1.f, fx1q = 0.f, fx2q = 1.f, fx3q = 1.f;
         if (PML && BROW) {
             const int rc = min(max(r, 0), nx - 1), qc = min(max(q, 0), nx - 1);
@@ -329,8 +339,8 @@ __device__ __forceinline__ void iterate(const TbParams &P, const Geom &g, const 
         HRow<C> Hout;
 #pragma unroll
         for (int c = 0; c < C; ++c) {
-            const float e = St[s].e[c];
-            const float eR = (c + 1 < C) ? St[s].e[c + 1] : St[s].er;
+            const float e = Si[s].e[c];
+            const float eR = (c + 1 < C) ? Si[s].e[c + 1] : Si[s].er;
             const float eD = En[c];
             float hx, hy, ax, ay;
             if (PML) {
@@ -380,10 +390,10 @@ __device__ __forceinline__ void iterate(const TbParams &P, const Geom &g, const 
             }
         }
         // carry this level's rows to the next iteration
-        St[s].h = Hin;
+        So[s].h = Hin;
 #pragma unroll
-        for (int c = 0; c < C; ++c) St[s].e[c] = En[c];
-        St[s].er = enr;
+        for (int c = 0; c < C; ++c) So[s].e[c] = En[c];
+        So[s].er = enr;
         if (s + 1 < K) {
             Hin = Hout;          // input of the next level
             Dpend = Dn;
@@ -460,34 +470,42 @@ __device__ __forceinline__ void load_ft(const TbParams &P, int p, int j0, FRow<C
 }
 
 template <int K, int C, bool LOSSY, bool PML, bool VEC, bool BROW, bool BCOL, bool SP, int NF = 0,
-          bool BS = false>
+          bool BS = false, bool PP = false>
 __device__ __forceinline__ void sweep(const TbParams &P, const Geom &g)
 {
     ColCoef<C> cc;
     col_coef<C, PML, BCOL>(P, g.j0, cc);
-    Carry<C> St[K];
-    FRow<C, NF> SF[K];
-    if constexpr (NF > 0) {
+    // PP: two carry sets, St[0] and St[1]; an iteration reads one and writes
+    // the other, so no level's carry is copied from register to register
+    // (fewer instructions, more registers).  Otherwise one set, updated in
+    // place.
+    constexpr int NS = PP ? 2 : 1;
+    Carry<C> St[NS][K];
+    FRow<C, NF> SF[NS][K];
 #pragma unroll
-        for (int s = 0; s < K; ++s)
+    for (int a = 0; a < NS; ++a)
 #pragma unroll
-            for (int f = 0; f < NF; ++f)
+        for (int s = 0; s < K; ++s) {
 #pragma unroll
-                for (int c = 0; c < C; ++c) { SF[s].r[f][c] = 0.f; SF[s].i[f][c] = 0.f; }
-    }
+            for (int c = 0; c < C; ++c) {
+                St[a][s].h.hx[c] = 0.f; St[a][s].h.hy[c] = 0.f;
+                St[a][s].h.ix[c] = 0.f; St[a][s].h.iy[c] = 0.f;
+                St[a][s].e[c] = 0.f;
+                St[a][s].d.dz[c] = 0.f; St[a][s].d.iz[c] = 0.f;
+                St[a][s].d.na[c] = 0.f; St[a][s].d.nb[c] = 0.f;
+            }
+            St[a][s].er = 0.f;
+            if constexpr (NF > 0) {
 #pragma unroll
-    for (int s = 0; s < K; ++s) {
+                for (int f = 0; f < NF; ++f)
 #pragma unroll
-        for (int c = 0; c < C; ++c) {
-            St[s].h.hx[c] = 0.f; St[s].h.hy[c] = 0.f; St[s].h.ix[c] = 0.f; St[s].h.iy[c] = 0.f;
-            St[s].e[c] = 0.f;
-            St[s].d.dz[c] = 0.f; St[s].d.iz[c] = 0.f; St[s].d.na[c] = 0.f; St[s].d.nb[c] = 0.f;
+                    for (int c = 0; c < C; ++c) { SF[a][s].r[f][c] = 0.f; SF[a][s].i[f][c] = 0.f; }
+            }
         }
-        St[s].er = 0.f;
-    }
     const int pbeg = max(g.i0 - K, 0);
     const int pend = g.i1 - 1 + K;
-    // row p+1 is in flight while row p is computed
+    // row p+1 is in flight while row p is computed (rows past the grid read a
+    // clamped row)
     HRow<C> H0, H1;
     DRow<C> D0, D1;
     FRow<C, NF> F0, F1;
@@ -497,13 +515,33 @@ __device__ __forceinline__ void sweep(const TbParams &P, const Geom &g)
     // keeps the loop head free of a wait that would also catch the stores of
     // the previous iteration
     if constexpr (BS) __builtin_amdgcn_s_waitcnt(0);
-    for (int p = pbeg; p <= pend; ++p) {
-        load_row<C, LOSSY, PML, VEC, BCOL>(P, p + 1, g.j0, H1, D1);
-        if constexpr (NF > 0) load_ft<C, NF, VEC, BCOL>(P, p + 1, g.j0, F1);
-        iterate<K, C, LOSSY, PML, VEC, BROW, BCOL, SP, NF, BS>(P, g, cc, p, H0, D0, St, F0, SF);
-        H0 = H1;
-        D0 = D1;
-        if constexpr (NF > 0) F0 = F1;
+    if constexpr (PP) {
+        // two rows per trip, so the row registers alternate as well
+        int p = pbeg;
+        for (; p < pend; p += 2) {
+            load_row<C, LOSSY, PML, VEC, BCOL>(P, p + 1, g.j0, H1, D1);
+            if constexpr (NF > 0) load_ft<C, NF, VEC, BCOL>(P, p + 1, g.j0, F1);
+            iterate<K, C, LOSSY, PML, VEC, BROW, BCOL, SP, NF, BS>(P, g, cc, p, H0, D0, St[0],
+                                                                   St[NS - 1], F0, SF[0], SF[NS - 1]);
+            load_row<C, LOSSY, PML, VEC, BCOL>(P, p + 2, g.j0, H0, D0);
+            if constexpr (NF > 0) load_ft<C, NF, VEC, BCOL>(P, p + 2, g.j0, F0);
+            iterate<K, C, LOSSY, PML, VEC, BROW, BCOL, SP, NF, BS>(P, g, cc, p + 1, H1, D1,
+                                                                   St[NS - 1], St[0], F1, SF[NS - 1],
+                                                                   SF[0]);
+        }
+        if (p == pend)
+            iterate<K, C, LOSSY, PML, VEC, BROW, BCOL, SP, NF, BS>(P, g, cc, p, H0, D0, St[0],
+                                                                   St[NS - 1], F0, SF[0], SF[NS - 1]);
+    } else {
+        for (int p = pbeg; p <= pend; ++p) {
+            load_row<C, LOSSY, PML, VEC, BCOL>(P, p + 1, g.j0, H1, D1);
+            if constexpr (NF > 0) load_ft<C, NF, VEC, BCOL>(P, p + 1, g.j0, F1);
+            iterate<K, C, LOSSY, PML, VEC, BROW, BCOL, SP, NF, BS>(P, g, cc, p, H0, D0, St[0], St[0],
+                                                                   F0, SF[0], SF[0]);
+            H0 = H1;
+            D0 = D1;
+            if constexpr (NF > 0) F0 = F1;
+        }
     }
 }
 
@@ -607,10 +645,10 @@ __global__ void __launch_bounds__(64) tm_tb_kernel(const TbParams P)
     }
     const bool interior = !brow && !bcol && !sp;
     if (interior) {
-        if (want_int) sweep<K, C, LOSSY, PML, VEC, false, false, false, NF>(P, g);
+        if (want_int) sweep<K, C, LOSSY, PML, VEC, false, false, false, NF, NF == 0, NF == 0>(P, g);
     } else if (want_bnd) {
-        if (!brow && !sp) sweep<K, C, LOSSY, PML, VEC, false, true, false, NF>(P, g);
-        else sweep<K, C, LOSSY, PML, VEC, true, true, true, NF>(P, g);
+        if (!brow && !sp) sweep<K, C, LOSSY, PML, VEC, false, true, false, NF, NF == 0>(P, g);
+        else sweep<K, C, LOSSY, PML, VEC, true, true, true, NF, NF == 0>(P, g);
     }
 }
 
