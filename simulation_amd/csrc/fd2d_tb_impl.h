@@ -94,11 +94,18 @@ struct Geom {
 
 // Element offsets below are 32-bit (the host rejects grids of 2^30 cells or
 // more), so every access is a uniform 64-bit array base plus a 32-bit
-// per-lane offset.
-__device__ __forceinline__ const float *at(const float *base, unsigned off)
+// per-lane offset.  Loads go through global-address-space pointers: the
+// arrays are device memory, and where a base pointer is itself read from
+// memory (the batch kernel's grid records) a generic pointer would make the
+// loads FLAT, which also count against the LDS counter, so that every wait
+// for a loaded row becomes a wait for all outstanding memory operations.
+using gfloat = const __attribute__((address_space(1))) float;
+typedef float vf2 __attribute__((ext_vector_type(2)));
+using gfloat2 = const __attribute__((address_space(1))) vf2;
+__device__ __forceinline__ gfloat *at(const float *base, unsigned off)
 {
     const unsigned bytes = off << 2;                  // < 2^32: zero-extended 32-bit offset
-    return reinterpret_cast<const float *>(reinterpret_cast<const char *>(base) + bytes);
+    return (gfloat *)(reinterpret_cast<const char *>(base) + bytes);
 }
 __device__ __forceinline__ float *at(float *base, unsigned off)
 {
@@ -117,7 +124,7 @@ __device__ __forceinline__ void ld(const float *__restrict__ base, unsigned row,
     if (VEC) {
         const int jc = CLAMP ? min(max(j0, 0), ny - C) : j0;
         if constexpr (C == 2) {
-            const float2 q = *reinterpret_cast<const float2 *>(at(base, row + (unsigned)jc));
+            const vf2 q = *(gfloat2 *)(at(base, row + (unsigned)jc));
             v[0] = q.x; v[1] = q.y;
         } else {
 #pragma unroll
