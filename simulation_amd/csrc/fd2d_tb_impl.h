@@ -102,6 +102,7 @@ struct Geom {
 using gfloat = const __attribute__((address_space(1))) float;
 typedef float vf2 __attribute__((ext_vector_type(2)));
 using gfloat2 = const __attribute__((address_space(1))) vf2;
+__device__ __forceinline__ float ldg(const float *p) { return *(gfloat *)p; }
 __device__ __forceinline__ gfloat *at(const float *base, unsigned off)
 {
     const unsigned bytes = off << 2;                  // < 2^32: zero-extended 32-bit offset
@@ -219,8 +220,9 @@ __device__ __forceinline__ void col_coef(const TbParams &P, int j0, ColCoef<C> &
         cc.gy2[c] = 1.f; cc.gy3[c] = 1.f; cc.fy1[c] = 0.f; cc.fy2[c] = 1.f; cc.fy3[c] = 1.f;
         if (PML && BCOL) {
             const int j = min(max(j0 + c, 0), P.ny - 1);
-            cc.gy2[c] = P.pml.gy2[j]; cc.gy3[c] = P.pml.gy3[j];
-            cc.fy1[c] = P.pml.fy1[j]; cc.fy2[c] = P.pml.fy2[j]; cc.fy3[c] = P.pml.fy3[j];
+            cc.gy2[c] = ldg(P.pml.gy2 + j); cc.gy3[c] = ldg(P.pml.gy3 + j);
+            cc.fy1[c] = ldg(P.pml.fy1 + j); cc.fy2[c] = ldg(P.pml.fy2 + j);
+            cc.fy3[c] = ldg(P.pml.fy3 + j);
         }
     }
 }
@@ -268,8 +270,8 @@ __device__ __forceinline__ void iterate(const TbParams &P, const Geom &g, const 
         float gx2r = 1.f, gx3r = 1.f, fx1q = 0.f, fx2q = 1.f, fx3q = 1.f;
         if (PML && BROW) {
             const int rc = min(max(r, 0), nx - 1), qc = min(max(q, 0), nx - 1);
-            gx2r = P.pml.gx2[rc]; gx3r = P.pml.gx3[rc];
-            fx1q = P.pml.fx1[qc]; fx2q = P.pml.fx2[qc]; fx3q = P.pml.fx3[qc];
+            gx2r = ldg(P.pml.gx2 + rc); gx3r = ldg(P.pml.gx3 + rc);
+            fx1q = ldg(P.pml.fx1 + qc); fx2q = ldg(P.pml.fx2 + qc); fx3q = ldg(P.pml.fx3 + qc);
         }
         // ------------------------------------------------ E update, row r
         const float hxl = from_left(Hin.hx[C - 1]);
@@ -288,7 +290,7 @@ __device__ __forceinline__ void iterate(const TbParams &P, const Geom &g, const 
         }
         for (int k = 0; SP && k < P.nsrc; ++k) {
             if (r == P.src_i[k]) {
-                const float v = P.src_wave[(size_t)k * P.wave_len + (t - 1)];
+                const float v = ldg(P.src_wave + (size_t)k * P.wave_len + (t - 1));
 #pragma unroll
                 for (int c = 0; c < C; ++c)
                     if (g.j0 + c == P.src_j[k]) Dn.dz[c] = v;
@@ -338,7 +340,8 @@ __device__ __forceinline__ void iterate(const TbParams &P, const Geom &g, const 
                     for (int c = 0; c < C; ++c) {
                         const int j = g.j0 + c;
                         if (j == P.probe_j[pr] && j >= g.vj0 && j < g.vj1)
-                            P.probe_out[(size_t)pr * P.probe_len + (t - 1)] = En[c];
+                            *(__attribute__((address_space(1))) float *)(P.probe_out +
+                                (size_t)pr * P.probe_len + (t - 1)) = En[c];
                     }
                 }
         }
