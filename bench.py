@@ -288,26 +288,36 @@ def bench_learned(a):
     c = n // 2
     start = ("pulse", c, c, 3.0)
     cyl = [(c - n / 8, c + n / 8, n / 10, 4.0, 0.02)]
-    res = {}
-    for m in ("learned", "pml32"):
-        s, p = LB._solver(m, n, net, cyl, start, W + K, dev, depth=a.depth, rows=a.rows,
-                          cols=a.cols)
-        s.run(W)
-        torch.cuda.synchronize()
-        st = torch.cuda.current_stream()
-        e0 = torch.cuda.Event(enable_timing=True)
-        e1 = torch.cuda.Event(enable_timing=True)
-        t0 = time.perf_counter()
-        e0.record(st)
-        s.run(K)
-        e1.record(st)
-        torch.cuda.synchronize()
-        wall = time.perf_counter() - t0
-        N = n + 2 * p
-        res[m] = {"grid": N, "ms_per_step": wall * 1e3 / K, "events_ms": e0.elapsed_time(e1),
-                  "mcups_grid": N * N * K / wall / 1e6, "mcups_window": n * n * K / wall / 1e6}
-        del s
-        torch.cuda.empty_cache()
+    # Two passes over the two forms, alternating; the second pass is
+    # reported.  The form timed first in a process measured ~10 % slower
+    # than in every later timing of the same form (learned 325k, then
+    # 355-360k Mcell-updates/s; npml=32 316k each time), which biased the
+    # comparison against whichever form went first.
+    passes = []
+    for _ in range(2):
+        res = {}
+        for m in ("learned", "pml32"):
+            s, p = LB._solver(m, n, net, cyl, start, W + K, dev, depth=a.depth, rows=a.rows,
+                              cols=a.cols)
+            s.run(W)
+            torch.cuda.synchronize()
+            st = torch.cuda.current_stream()
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            t0 = time.perf_counter()
+            e0.record(st)
+            s.run(K)
+            e1.record(st)
+            torch.cuda.synchronize()
+            wall = time.perf_counter() - t0
+            N = n + 2 * p
+            res[m] = {"grid": N, "ms_per_step": wall * 1e3 / K, "events_ms": e0.elapsed_time(e1),
+                      "mcups_grid": N * N * K / wall / 1e6,
+                      "mcups_window": n * n * K / wall / 1e6}
+            del s
+            torch.cuda.empty_cache()
+        passes.append(res)
+    first, res = passes
     L = (K + 7) // 8
     L += L % 2
     launch_ms = res["learned"]["events_ms"] / L
@@ -322,7 +332,9 @@ def bench_learned(a):
         "data": "synthetic (point pulse, one lossy cylinder; fields start at zero)",
         "config": {"workload": f"2-D TM {n}x{n} window, learned boundary ({wname}) vs npml=32 PML",
                    "grid": [n, n], "learned": res["learned"], "pml32": res["pml32"],
-                   "speedup_vs_pml32_window": round(ratio, 3)},
+                   "speedup_vs_pml32_window": round(ratio, 3),
+                   "first_pass_mcups_window": {m: round(v["mcups_window"], 1)
+                                               for m, v in first.items()}},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4), "traffic": None,
                      "kernel": "tm_tb_kernel + lbc frame_kernel (learned form, 44 B/cell)",
