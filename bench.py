@@ -47,7 +47,7 @@ def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
-    ap.add_argument("--warmup", type=int, default=40)
+    ap.add_argument("--warmup", type=int, default=400)
     ap.add_argument("--size", type=int, default=4096)
     ap.add_argument("--depth", type=int, default=0)
     ap.add_argument("--rows", type=int, default=0)
