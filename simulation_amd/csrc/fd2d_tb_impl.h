@@ -38,12 +38,13 @@
 //     launches on two streams, beside its frame kernel.
 //   * Launches read one state set and write the other (ping-pong), so waves
 //     never read a cell another wave has already advanced.
-//   * Interior waves keep two sets of level carries and alternate them
-//     between sweep iterations (no register-to-register copies), and all
-//     waves without DFT planes store through buffer resources whose range
-//     check drops the cells a wave does not own, so every iteration issues
-//     the same stores and the wait for the next row's loads does not wait
-//     for them.
+//   * Interior waves of tm_tb_kernel keep two sets of level carries and
+//     alternate them between sweep iterations (no register-to-register
+//     copies; the batch kernel keeps one set, which holds it at 4 waves per
+//     SIMD), and all waves without DFT planes store through buffer resources
+//     whose range check drops the cells a wave does not own, so every
+//     iteration issues the same stores and the wait for the next row's loads
+//     does not wait for them.
 //
 // HBM traffic per cell per launch (lossy PML form): reads dz iz hx hy ihx ihy
 // naz nbz (32 B, x halo/warm-up overlap), writes dz iz hx hy ihx ihy (24 B,
