@@ -236,12 +236,12 @@ __device__ __forceinline__ void col_coef(const TbParams &P, int j0, ColCoef<C> &
 // same for the strip's columns; SP: a source or probe lies in the rows and
 // columns the wave computes.
 template <int K, int C, bool LOSSY, bool PML, bool VEC, bool BROW, bool BCOL, bool SP, int NF,
-          bool BS>
+          bool BS, bool PRO = false>
 __device__ __forceinline__ void iterate(const TbParams &P, const Geom &g, const ColCoef<C> &cc,
                                         int p, const HRow<C> &Hload, const DRow<C> &Dload,
                                         const Carry<C> (&Si)[K], Carry<C> (&So)[K],
                                         const FRow<C, NF> &Fload, const FRow<C, NF> (&SFi)[K],
-                                        FRow<C, NF> (&SFo)[K])
+                                        FRow<C, NF> (&SFo)[K], int j = 0)
 {
     const int nx = P.nx, ny = P.ny;
     HRow<C> Hin = Hload;
@@ -266,6 +266,15 @@ __device__ __forceinline__ void iterate(const TbParams &P, const Geom &g, const 
                 Fin = SFi[s - 1];
                 SFo[s - 1] = Fpend;
             }
+        }
+        if (PRO && j < 2 * s + 1) {
+            // warm-up position j of the sweep: level s contributes to the
+            // tile's outputs only from position 2s+1 on (its E row p-s lies
+            // above the dependency cone of the first output row), so it only
+            // passes the H row it would carry; the other carries it leaves
+            // are not read before it has written them
+            So[s].h = Hin;
+            continue;
         }
         const HRow<C> Hp = Si[s].h;
         float gx2r = 1.f, gx3r = 1.f, fx1q = 0.f, fx2q = 1.f, fx3q = 1.f;
@@ -527,8 +536,21 @@ __device__ __forceinline__ void sweep(const TbParams &P, const Geom &g)
     // the previous iteration
     if constexpr (BS) __builtin_amdgcn_s_waitcnt(0);
     if constexpr (PP) {
-        // two rows per trip, so the row registers alternate as well
+        static_assert(!PP || (!BROW && NF == 0), "two carry sets: interior tiles only");
         int p = pbeg;
+        // interior tiles start K rows above their first row (pbeg = i0-K);
+        // in the first 2K-1 positions level s only starts at position
+        // i0-K+2s+1 (one carry set; the sweep then continues in St[0])
+        const int c0 = g.i0 - K;
+#pragma unroll 1
+        for (; p < c0 + 2 * K - 1; ++p) {
+            load_row<C, LOSSY, PML, VEC, BCOL>(P, p + 1, g.j0, H1, D1);
+            iterate<K, C, LOSSY, PML, VEC, BROW, BCOL, SP, NF, BS, true>(
+                P, g, cc, p, H0, D0, St[0], St[0], F0, SF[0], SF[0], p - c0);
+            H0 = H1;
+            D0 = D1;
+        }
+        // two rows per trip, so the row registers alternate as well
         for (; p < pend; p += 2) {
             load_row<C, LOSSY, PML, VEC, BCOL>(P, p + 1, g.j0, H1, D1);
             if constexpr (NF > 0) load_ft<C, NF, VEC, BCOL>(P, p + 1, g.j0, F1);
