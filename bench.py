@@ -3,26 +3,33 @@
     python bench.py [--steps K] [--warmup W] [--size N] [--depth D] [--rows H] [--cols C]
 
 Workload (the 2-D TM configuration the metric is quoted on): N x N FP32 grid
-(default 4096), npml=8 Sullivan PML, one lossy cylinder of radius 400 cells
+(default 8192), npml=8 Sullivan PML, one lossy cylinder of radius 400 cells
 (eps_r=4, sigma=0.02) at (N/2-1, N/2-1), hard Gaussian source (t0=20,
 spread 6) at (N/2, N/2), four probe traces recorded every step.  All fields
 start at zero and live in HBM before timing.  A "step" is one leapfrog time
 step of the whole grid; K timed steps run through fd2d_tm_run (temporally
-blocked launches) after W untimed warm-up steps, bracketed by
-torch.cuda.synchronize().
+blocked launches, each advancing the grid by up to 8 steps) after W untimed
+warm-up steps, bracketed by torch.cuda.synchronize().  Every device code
+object of the library is loaded before the warm-up (fdtd_preload), so no
+launch in the timed region carries a code-object load, whatever K and W.
 
 The JSON line adds:
-  roofline     for the dominant kernel (tm_tb_kernel): algorithmic bytes per
-               launch (60 B per cell-update x cells x fused steps) over its
-               average launch duration (HIP events on the launch stream over
-               the timed region / number of launches), against 8 TB/s;
-               traffic = HBM bytes per launch from rocprofv3 FETCH_SIZE +
-               WRITE_SIZE when a summary for this workload is committed under
-               profiles/, else null.
+  roofline     for the dominant kernel (tm_tb_kernel), per launch: the
+               minimum HBM bytes of one launch (60 B per cell: every state
+               array read and written once, naz and nbz read once, whatever
+               the number of steps the launch fuses) over its average launch
+               duration (HIP events on the launch stream over the timed
+               region / number of launches), against 8 TB/s; traffic = HBM
+               bytes per launch from rocprofv3 FETCH_SIZE + WRITE_SIZE when a
+               record of the same workload and launch shape (steps per
+               launch, rows, columns per lane) is committed under profiles/,
+               else null.  The per-step "effective" rate (60 B per
+               cell-update) is reported beside it, under its own keys.
   cpu_baseline the reference's own FP32 C kernels (fd2d/clang/test_3_4.c,
                OpenMP, built into oracle/_ref/) -- or this project's C
                restatement when that build is absent -- timed on the host for a
-               bounded number of steps of the same grid.
+               bounded number of steps of the same grid, with the CPU model
+               and the thread count.
 """
 from __future__ import annotations
 
@@ -48,16 +55,19 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=400)
-    ap.add_argument("--size", type=int, default=4096)
+    ap.add_argument("--size", type=int, default=8192)
     ap.add_argument("--depth", type=int, default=0)
     ap.add_argument("--rows", type=int, default=0)
     ap.add_argument("--cols", type=int, default=0)
+    ap.add_argument("--form", choices=("auto", "strip", "multiwave"), default="auto",
+                    help="tm/large: auto = the library's choice, strip = single-wave strips "
+                         "everywhere, multiwave = interior blocks of four waves")
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", choices=("tm", "large", "batch", "learned", "tfsf", "dform"),
                     default="tm",
-                    help="tm: the N x N single-cylinder configuration (default, N=4096); "
+                    help="tm: the N x N single-cylinder configuration (default, N=8192); "
                          "large: 16384^2, 16 cylinders, 4 sources, 5000 steps; "
                          "batch: 256 uneven grids; learned: 2048^2 learned boundary against "
                          "npml=32 PML; tfsf: the plane-wave program (1024^2, npml=80, TF/SF "
@@ -78,12 +88,30 @@ def parse():
                          "counter_collection.csv files, scaled by the copy calibration, into "
                          "profiles/traffic.json under KEY")
     a = ap.parse_args()
+    a.size_given = any(x == "--size" or x.startswith("--size=") for x in sys.argv[1:])
     if a.workload == "large":
-        if a.size == 4096:
+        if not a.size_given:
             a.size = 16384
         if a.steps == 2000:
             a.steps = 5000
     return a
+
+
+def host_cpu() -> dict:
+    """CPU model and the number of logical CPUs this process may run on."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return {"cpu_model": model, "nproc": n}
 
 
 def cpu_baseline(n: int, budget_s: float) -> dict:
@@ -136,10 +164,12 @@ def cpu_baseline(n: int, budget_s: float) -> dict:
     rate = n * n * steps / el / 1e6
     return {"value": round(rate, 2), "unit": "Mcell-updates/s", "cores": threads, "kind": kind,
             "sample": f"{steps} steps of the {n}x{n} FP32 TM step (dfield+source+efield+hfield), "
-                      f"{el:.1f} s, OMP_NUM_THREADS={threads}"}
+                      f"{el:.1f} s, OMP_NUM_THREADS={threads}", **host_cpu()}
 
 
-def traffic_for(workload: str):
+def traffic_for(key: str):
+    """HBM bytes per launch of the record `key` (workload and launch shape)
+    in profiles/traffic.json, or None."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(path):
         return None
@@ -147,8 +177,16 @@ def traffic_for(workload: str):
         data = json.load(open(path))
     except ValueError:
         return None
-    rec = data.get(workload)
+    rec = data.get(key)
     return None if rec is None else rec.get("bytes_per_launch")
+
+
+def launch_depths(nsteps: int, launches: int) -> list:
+    """Steps of each fused launch of an fd2d_tm_run of nsteps (the library
+    splits the steps evenly over its even number of launches)."""
+    if launches <= 0:
+        return []
+    return [nsteps // launches + (1 if l < nsteps % launches else 0) for l in range(launches)]
 
 
 def copy_calibration() -> None:
@@ -277,7 +315,7 @@ def bench_learned(a):
     import torch
     from simulation_amd import learned as LB
     dev = torch.device("cuda", 0)
-    n = a.size if a.size != 4096 else 2048
+    n = a.size if a.size_given else 2048
     K = a.steps if a.steps != 2000 else 1600
     W = a.warmup
     wpath = a.weights or LB.WEIGHTS
@@ -365,7 +403,7 @@ def bench_tfsf(a):
     import torch
     from simulation_amd import fd2d
     dev = torch.device("cuda", 0)
-    n = a.size if a.size != 4096 else 1024
+    n = a.size if a.size_given else 1024
     K = a.steps if a.steps != 2000 else 1000
     W = a.warmup
     npml, freq = 80, (50e6, 300e6, 700e6)
@@ -436,7 +474,7 @@ def bench_dform(a):
     import torch
     from simulation_amd import fd1d
     dev = torch.device("cuda", 0)
-    nx = a.size if a.size != 4096 else 38000
+    nx = a.size if a.size_given else 38000
     K = a.steps if a.steps != 2000 else 8000
     W = a.warmup
     freq = (50e6, 200e6, 500e6)
@@ -631,7 +669,8 @@ def main():
         desc = (f"2-D TM {n}x{n} FP32, npml=8 Sullivan PML, lossy cylinder r=400 "
                 f"eps_r=4 sigma=0.02, hard Gaussian t0=20 spread=6 at centre, 4 probes")
     cfg.depth, cfg.rows, cfg.cols = a.depth, a.rows, a.cols
-    s = fd2d.build_solver(cfg, dev)
+    cfg.form = {"auto": 0, "strip": 1, "multiwave": 2}[a.form]
+    s = fd2d.build_solver(cfg, dev)          # loads every code object (fdtd_preload)
     tuned = None
     if a.tune:
         from simulation_amd import autotune
@@ -652,9 +691,16 @@ def main():
     ev_ms = e0.elapsed_time(e1)
     cells = n * n
     value = cells * K / wall / 1e6
-    launch_ms = ev_ms / max(plan["launches"], 1)
-    bytes_per_launch = B_ALG * cells * K / max(plan["launches"], 1)
-    achieved = bytes_per_launch / (launch_ms * 1e-3) / 1e9
+    L = max(plan["launches"], 1)
+    depths = launch_depths(K, plan["launches"])
+    launch_ms = ev_ms / L
+    # minimum HBM bytes of one launch: each array moved once, however many
+    # steps the launch fuses
+    min_bytes = B_ALG * cells
+    achieved = min_bytes / (launch_ms * 1e-3) / 1e9
+    effective = B_ALG * cells * K / (ev_ms * 1e-3) / 1e9
+    shape = (f"d{depths[0]}_r{plan['rows']}_c{plan['cols']}" + ("_mw" if a.form == "multiwave" else "")
+             if depths and len(set(depths)) == 1 else None)
     rec = {
         "metric": METRIC,
         "value": round(value, 1),
@@ -671,7 +717,9 @@ def main():
         "config": {
             "workload": desc,
             "grid": [n, n],
-            "depth": plan["depth"], "rows": plan["rows"], "cols": plan["cols"],
+            "steps_per_launch": sorted(set(depths)),
+            "max_depth": plan["depth"], "rows": plan["rows"], "cols": plan["cols"],
+            "form": a.form,
             "launches": plan["launches"],
             "autotuned": tuned,
         },
@@ -681,10 +729,13 @@ def main():
             "peak": HBM_PEAK,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK, 4),
-            "traffic": traffic_for(f"{workload}_d{plan['depth']}_r{plan['rows']}_c{plan['cols']}"),
-            "kernel": "tm_tb_kernel",
+            "traffic": traffic_for(f"{workload}_{shape}") if shape else None,
+            "traffic_key": f"{workload}_{shape}" if shape else None,
+            "kernel": "tm_mw_kernel" if a.form == "multiwave" else "tm_tb_kernel",
             "avg_launch_ms": round(launch_ms, 4),
-            "alg_bytes_per_launch": int(bytes_per_launch),
+            "min_bytes_per_launch": int(min_bytes),
+            "effective_GBps_per_step": round(effective, 1),
+            "effective_frac": round(effective / HBM_PEAK, 4),
         },
         "cpu_baseline": None,
     }

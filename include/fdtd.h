@@ -39,7 +39,7 @@
 extern "C" {
 #endif
 
-#define FDTD_ABI_VERSION 1
+#define FDTD_ABI_VERSION 2
 
 #define FDTD_OK 0
 #define FDTD_EINVAL (-1)
@@ -75,6 +75,13 @@ typedef struct fdtd_medium {
 int fdtd_abi_version(void);
 /* Message of the last failed call on this host thread ("" if none). */
 const char *fdtd_last_error(void);
+/* Loads every device code object of the library on the current device.  The
+ * HIP runtime otherwise loads a translation unit's code at the first launch
+ * of one of its kernels, so that first launch would carry the load.  No
+ * counterpart in the reference (PyCUDA compiles and loads its module up
+ * front, SourceModule at fd2d/pycuda/fd2d_3_4.py:353).  Optional; callable
+ * any number of times.                                                      */
+int fdtd_preload(void);
 
 /* ---------------------------------------------------- host waveforms
  * Evaluated on the host with the host libm, in the reference's expression
@@ -210,6 +217,13 @@ int fd2d_energy_diff(int wn, int wm, int nxa, int nya, int ia0, int ja0, const f
  *   space (ihx, ihy untouched).
  *   depth/rows/cols: temporal depth, rows per block and columns per lane;
  *   0 selects the built-in defaults (see fd2d_tm_plan).
+ *   form: FDTD_TM_FORM_AUTO (0) lets the library choose (currently the
+ *   single-wave strips, the faster form on MI355X); FDTD_TM_FORM_STRIP (1)
+ *   sweeps everything in single-wave strips (any cols);
+ *   FDTD_TM_FORM_MULTIWAVE (2) sweeps the interior of the band in workgroups
+ *   of four waves that exchange their edge columns through LDS (one column
+ *   per lane; `rows` = rows of such a block) and the rest in single-wave
+ *   strips.  All forms give the same bits.
  *   band_rows/band_cols: the caller's promise that row indices
  *   [band_rows[0], band_rows[1]) hold the default profile values
  *   (fx1 = 0, fx2 = fx3 = gx2 = gx3 = 1) and column indices
@@ -233,13 +247,20 @@ typedef struct fd2d_tm_desc {
     int probe_len;
     int depth, rows, cols;
     int band_rows[2], band_cols[2];
+    int form;                     /* FDTD_TM_FORM_* */
 } fd2d_tm_desc;
+
+#define FDTD_TM_FORM_AUTO 0
+#define FDTD_TM_FORM_STRIP 1
+#define FDTD_TM_FORM_MULTIWAVE 2
 
 int fd2d_tm_run(const fd2d_tm_desc *d, int t0, int nsteps, void *stream);
 
 /* The launch plan fd2d_tm_run would use for this descriptor and nsteps:
  * number of launches and the chosen depth/rows/cols (defaults: depth 8, one
- * column per lane, rows = nx/220 clamped to [16, 512]).  Host only.         */
+ * column per lane; rows = nx/220 clamped to [16, 512] for single-wave
+ * strips, the multi-wave default below for the multi-wave form).  Host
+ * only.                                                                    */
 int fd2d_tm_plan(const fd2d_tm_desc *d, int nsteps,
                  int *launches, int *depth, int *rows, int *cols);
 
@@ -273,6 +294,10 @@ typedef struct fd2d_batch_desc {
     int depth, rows, cols;        /* fixed when the work list is made */
     int waves;                    /* workgroups of the persistent launch */
     int vec;                      /* 1: every grid allows vector loads */
+    int wave_len;                 /* smallest wave_len over grids with a source
+                                     (INT_MAX: none), set by fd2d_batch_upload */
+    int probe_len;                /* smallest probe_len over grids with a probe
+                                     (INT_MAX: none), set by fd2d_batch_upload */
 } fd2d_batch_desc;
 
 /* Number of work units (and so ints/4 of units_dev) a batch needs for the
@@ -289,9 +314,10 @@ int fd2d_batch_upload(const fd2d_grid *grids, int ngrids, int depth, int rows, i
                       int waves, fd2d_grid *grids_dev, int *units_dev, int max_units,
                       fd2d_batch_desc *out, void *stream);
 
-/* Advance every grid by nsteps (t = t0+1 .. t0+nsteps).  max_wave_len /
- * max_probe_len: the smallest wave_len / probe_len over the grids (checked
- * against t0+nsteps; pass INT_MAX when no grid has a source / probe).  The
+/* Advance every grid by nsteps (t = t0+1 .. t0+nsteps).  t0+nsteps is
+ * checked against the smallest wave_len / probe_len of the grids recorded in
+ * *b by fd2d_batch_upload, and against the caller's max_wave_len /
+ * max_probe_len (pass INT_MAX to rely on the recorded ones alone).  The
  * steps are split into an even number of launches of at most `depth` steps,
  * so that the result lands in the grids' own state sets: nsteps = 1, or an
  * odd nsteps with depth 1, is rejected.                                    */

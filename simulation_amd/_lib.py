@@ -61,7 +61,11 @@ class Fd2dTmDesc(C.Structure):
         ("probe_out", _f), ("probe_len", C.c_int),
         ("depth", C.c_int), ("rows", C.c_int), ("cols", C.c_int),
         ("band_rows", C.c_int * 2), ("band_cols", C.c_int * 2),
+        ("form", C.c_int),
     ]
+
+
+TM_FORM_AUTO, TM_FORM_STRIP, TM_FORM_MULTIWAVE = 0, 1, 2
 
 
 class Fd2dGrid(C.Structure):
@@ -129,6 +133,7 @@ class Fd2dBatchDesc(C.Structure):
         ("grids_dev", C.c_void_p), ("units_dev", C.c_void_p),
         ("nunits", C.c_int), ("depth", C.c_int), ("rows", C.c_int), ("cols", C.c_int),
         ("waves", C.c_int), ("vec", C.c_int),
+        ("wave_len", C.c_int), ("probe_len", C.c_int),
     ]
 
 
@@ -143,6 +148,7 @@ _DP = C.POINTER(C.c_double)
 SIGNATURES = {
     "fdtd_abi_version": (_I, []),
     "fdtd_last_error": (C.c_char_p, []),
+    "fdtd_preload": (_I, []),
     "fdtd_gaussian_f32": (_F, [_I, _I, _F]),
     "fdtd_gaussian_f64": (_D, [_I, _I, _D]),
     "fdtd_sinusoidal_f32": (_F, [_I, _F, _F]),
@@ -232,6 +238,18 @@ def lib():
             fn.argtypes = args
         _lib = L
     return _lib
+
+
+_preloaded = False
+
+
+def preload() -> None:
+    """Load every device code object of the library (once per process), so
+    that no later launch carries a code-object load (fdtd_preload)."""
+    global _preloaded
+    if not _preloaded:
+        call("fdtd_preload")
+        _preloaded = True
 
 
 def check(name: str, rc: int) -> None:

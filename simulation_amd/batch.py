@@ -72,6 +72,7 @@ class BatchSolver:
     def __init__(self, specs, nsteps_max: int, depth: int = 0, rows: int = 0, cols: int = 0,
                  waves: int = 0, device=None):
         dev = device or require_gpu()
+        _lib.preload()
         self.device = dev
         self.specs = list(specs)
         self.nsteps_max = int(nsteps_max)

@@ -261,6 +261,8 @@ inline int nblk(long n) { return (int)((n + 255) / 256); }
 
 }  // namespace
 
+int preload_dform1d() { return preload_kernel(reinterpret_cast<const void *>(&dxfield_kernel)); }
+
 int launch_dxfield(int nx, float *dx, const float *hy, int src_pos, float src_val, int src_on,
                    hipStream_t s)
 {

@@ -130,6 +130,8 @@ inline int grid1d(int n, int block) { int g = (n + block - 1) / block; return g 
 
 int fd1d_run_lds_limit() { return kLdsLimit; }
 
+int preload_fd1d() { return preload_kernel(reinterpret_cast<const void *>(&fd1d_dielectric_kernel<float>)); }
+
 template <typename T>
 int launch_fd1d_exfield(int nx, const T *ca, const T *cb, T *ex, const T *hy,
                         int src_mode, int src_pos, T src_val, hipStream_t s)

@@ -264,15 +264,19 @@ fd2d_energy_diff_partial(int wn, int wm, int nya, int ia0, int ja0, const float 
     if (threadIdx.x == 0) work[1 + blockIdx.x] = r;
 }
 
-__global__ void __launch_bounds__(ET) fd2d_energy_final(double *__restrict__ work)
+// Sums the partials work[1 .. EB] in index order into *out (out = work for
+// the result in work[0]).
+__global__ void __launch_bounds__(ET) fd2d_energy_final(const double *work, double *out)
 {
     double acc = 0.0;
     for (int k = threadIdx.x; k < EB; k += ET) acc += work[1 + k];
     const double r = block_sum(acc);
-    if (threadIdx.x == 0) work[0] = r;
+    if (threadIdx.x == 0) out[0] = r;
 }
 
 }  // namespace
+
+int preload_fd2d() { return preload_kernel(reinterpret_cast<const void *>(&fd2d_pmlparam_kernel)); }
 
 int launch_fd2d_dfield(int nx, int ny, const PmlPtrs *pml, float *dz, const float *hx,
                        const float *hy, const SrcList &src, hipStream_t s)
@@ -350,7 +354,7 @@ int launch_fd2d_energy(int nx, int ny, int i0, int i1, int j0, int j1, const flo
                        ny, i0, i1, j0, j1, ez, hx, hy, work);
     int rc = hip_check(hipGetLastError(), "fd2d_energy launch");
     if (rc) return rc;
-    hipLaunchKernelGGL(fd2d_energy_final, dim3(1), dim3(ET), 0, s, work);
+    hipLaunchKernelGGL(fd2d_energy_final, dim3(1), dim3(ET), 0, s, work, work);
     return hip_check(hipGetLastError(), "fd2d_energy launch");
 }
 
@@ -363,7 +367,7 @@ int launch_fd2d_energy_diff(int wn, int wm, int nya, int ia0, int ja0, const flo
                        eza, hxa, hya, nyb, ib0, jb0, ezb, hxb, hyb, work);
     int rc = hip_check(hipGetLastError(), "fd2d_energy_diff launch");
     if (rc) return rc;
-    hipLaunchKernelGGL(fd2d_energy_final, dim3(1), dim3(ET), 0, s, work);
+    hipLaunchKernelGGL(fd2d_energy_final, dim3(1), dim3(ET), 0, s, work, work);
     return hip_check(hipGetLastError(), "fd2d_energy_diff launch");
 }
 

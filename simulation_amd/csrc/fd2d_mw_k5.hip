@@ -1,0 +1,7 @@
+// Instantiation of the multi-wave temporally blocked TM kernel for depth 5 (fd2d_mw_impl.h).
+#include "fd2d_mw_impl.h"
+
+namespace fdtd {
+template int launch_fd2d_mw_depth<5>(const TbParams &, bool, bool, int, int, hipStream_t);
+template int preload_mw_depth<5>();
+}  // namespace fdtd

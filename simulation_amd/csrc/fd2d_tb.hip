@@ -7,6 +7,50 @@ namespace fdtd {
 
 int tb_max_depth() { return 8; }
 
+int preload_tb()
+{
+    int rc = 0;
+    rc = rc ? rc : preload_tb_depth<1>();
+    rc = rc ? rc : preload_tb_depth<2>();
+    rc = rc ? rc : preload_tb_depth<3>();
+    rc = rc ? rc : preload_tb_depth<4>();
+    rc = rc ? rc : preload_tb_depth<5>();
+    rc = rc ? rc : preload_tb_depth<6>();
+    rc = rc ? rc : preload_tb_depth<7>();
+    rc = rc ? rc : preload_tb_depth<8>();
+    return rc;
+}
+
+int launch_fd2d_mw(const TbParams &p, int depth, bool lossy, bool pml, int rows, int frows,
+                   hipStream_t s)
+{
+    switch (depth) {
+    case 1: return launch_fd2d_mw_depth<1>(p, lossy, pml, rows, frows, s);
+    case 2: return launch_fd2d_mw_depth<2>(p, lossy, pml, rows, frows, s);
+    case 3: return launch_fd2d_mw_depth<3>(p, lossy, pml, rows, frows, s);
+    case 4: return launch_fd2d_mw_depth<4>(p, lossy, pml, rows, frows, s);
+    case 5: return launch_fd2d_mw_depth<5>(p, lossy, pml, rows, frows, s);
+    case 6: return launch_fd2d_mw_depth<6>(p, lossy, pml, rows, frows, s);
+    case 7: return launch_fd2d_mw_depth<7>(p, lossy, pml, rows, frows, s);
+    case 8: return launch_fd2d_mw_depth<8>(p, lossy, pml, rows, frows, s);
+    default: return fail("temporal depth %d not built", depth);
+    }
+}
+
+int preload_mw()
+{
+    int rc = 0;
+    rc = rc ? rc : preload_mw_depth<1>();
+    rc = rc ? rc : preload_mw_depth<2>();
+    rc = rc ? rc : preload_mw_depth<3>();
+    rc = rc ? rc : preload_mw_depth<4>();
+    rc = rc ? rc : preload_mw_depth<5>();
+    rc = rc ? rc : preload_mw_depth<6>();
+    rc = rc ? rc : preload_mw_depth<7>();
+    rc = rc ? rc : preload_mw_depth<8>();
+    return rc;
+}
+
 bool tb_supported(int depth, int cols)
 {
     return depth >= 1 && depth <= tb_max_depth() && (cols == 1 || cols == 2);

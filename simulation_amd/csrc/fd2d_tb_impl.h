@@ -947,6 +947,12 @@ int fd2d_batch_resident_depth(int cols, bool vec, int *waves)
 }
 
 template <int K>
+int preload_tb_depth()
+{
+    return preload_kernel(reinterpret_cast<const void *>(&tb::tm_tb_kernel<K, 1, true, true, false, TB_WHOLE>));
+}
+
+template <int K>
 int launch_fd2d_tb_depth(const TbParams &p, int cols, bool lossy, bool pml, int part, hipStream_t s)
 {
     if (cols == 2) return tb::launch_part<K, 2>(p, lossy, pml, part, s);

@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstring>
 #include <algorithm>
+#include <climits>
 #include <vector>
 
 #include "fdtd_internal.h"
@@ -98,7 +99,20 @@ struct Plan {
     int depth, rows, cols, halo;
     int launches;          // ping-pong launches
     int tail_unfused;      // 1 if the last step runs as dfield/efield/hfield in place
+    int mw;                // 1: multi-wave form (rows = rows of its interior blocks)
+    int frows;             // rows of the single-wave tiles (strips, or the frame of the
+                           // multi-wave form)
 };
+
+// Rows of an interior block of the multi-wave form when the caller gives
+// none.
+int mw_default_rows(int nx)
+{
+    long h = nx / 32;
+    if (h < 32) h = 32;
+    if (h > 512) h = 512;
+    return (int)h;
+}
 
 int plan_tm(const fd2d_tm_desc *d, int nsteps, Plan *pl)
 {
@@ -106,19 +120,23 @@ int plan_tm(const fd2d_tm_desc *d, int nsteps, Plan *pl)
     int C = d->cols ? d->cols : 1;
     if (!tb_supported(D, C)) return fail("fd2d_tm: depth=%d cols=%d not supported", D, C);
     if (64 * C - 2 * D < 8) return fail("fd2d_tm: depth %d too large for %d columns/lane", D, C);
-    int H = d->rows;
-    if (H == 0) {
-        // default from row sweeps on MI355X (depth 8, one column per lane,
-        // ~220 row blocks per strip: 4096^2 and 16384^2 at their best,
-        // 8192^2 within 1 %)
-        long h = d->nx / 220;
-        if (h < 16) h = 16;
-        if (h > 512) h = 512;
-        H = (int)h;
-    }
+    if (d->form != FDTD_TM_FORM_AUTO && d->form != FDTD_TM_FORM_STRIP &&
+        d->form != FDTD_TM_FORM_MULTIWAVE)
+        return fail("fd2d_tm: form=%d", d->form);
+    if (d->form == FDTD_TM_FORM_MULTIWAVE && C != 1)
+        return fail("fd2d_tm: the multi-wave form takes one column per lane (cols=%d)", C);
+    // single-wave strips: default from row sweeps on MI355X (depth 8, one
+    // column per lane, ~220 row blocks per strip: 4096^2 and 16384^2 at
+    // their best, 8192^2 within 1 %)
+    long h = d->nx / 220;
+    if (h < 16) h = 16;
+    if (h > 512) h = 512;
+    const int mw = d->form == FDTD_TM_FORM_MULTIWAVE;
+    int H = d->rows ? d->rows : (mw ? mw_default_rows(d->nx) : (int)h);
     if (H < 1) return fail("fd2d_tm: rows=%d", H);
     // strip halo: the deepest launch's cone, rounded up to whole lanes
     pl->depth = D; pl->cols = C; pl->rows = H; pl->halo = ((D + C - 1) / C) * C;
+    pl->mw = mw; pl->frows = mw ? (int)h : H;
     pl->launches = 0; pl->tail_unfused = 0;
     if (nsteps <= 0) return 0;
     if (nsteps == 1) { pl->tail_unfused = 1; return 0; }
@@ -208,6 +226,11 @@ int batch_launches(const fd2d_batch_desc *b, int t0, int nsteps, int max_wave_le
         return fail("fd2d_batch_run: incomplete batch descriptor");
     if (!tb_supported(b->depth, b->cols)) return fail("fd2d_batch_run: depth/cols");
     if (t0 < 0 || nsteps < 0) return fail("fd2d_batch_run: t0=%d nsteps=%d", t0, nsteps);
+    // the device indexes each grid's waveform and probe trace at t-1 with
+    // that grid's own lengths: the recorded minima bound every grid
+    if (t0 + nsteps > b->wave_len || t0 + nsteps > b->probe_len)
+        return fail("fd2d_batch_run: steps up to %d exceed a grid's waveform (%d) or probe (%d) length",
+                    t0 + nsteps, b->wave_len, b->probe_len);
     if (t0 + nsteps > max_wave_len || t0 + nsteps > max_probe_len)
         return fail("fd2d_batch_run: steps up to %d exceed a waveform or probe length", t0 + nsteps);
     if (nsteps == 0) return 0;
@@ -238,6 +261,20 @@ extern "C" {
 int fdtd_abi_version(void) { return FDTD_ABI_VERSION; }
 
 const char *fdtd_last_error(void) { return g_err; }
+
+int fdtd_preload(void)
+{
+    clear_error();
+    int rc = preload_fd1d();
+    if (!rc) rc = preload_fd2d();
+    if (!rc) rc = preload_tb();
+    if (!rc) rc = preload_mw();
+    if (!rc) rc = preload_dform1d();
+    if (!rc) rc = preload_tfsf();
+    if (!rc) rc = preload_lbc();
+    if (!rc) rc = preload_lbc_frame();
+    return rc;
+}
 
 // ------------------------------------------------------------ waveforms
 
@@ -561,7 +598,8 @@ int fd2d_tm_run(const fd2d_tm_desc *d, int t0, int nsteps, void *stream)
         p.dz_o = out[0]; p.iz_o = out[1]; p.hx_o = out[2]; p.hy_o = out[3]; p.ihx_o = out[4]; p.ihy_o = out[5];
         p.ez_o = (l == L - 1 && !pl.tail_unfused) ? d->ez : nullptr;
         p.t_first = t0 + done + 1;
-        rc = launch_fd2d_tb(p, k, pl.cols, lossy, pml, TB_WHOLE, S(stream));
+        if (pl.mw) rc = launch_fd2d_mw(p, k, lossy, pml, pl.rows, pl.frows, S(stream));
+        else rc = launch_fd2d_tb(p, k, pl.cols, lossy, pml, TB_WHOLE, S(stream));
         if (rc) return rc;
         done += k;
     }
@@ -843,6 +881,7 @@ static int tfsf_fused(const fd2d_tm_desc *d, const fd2d_tfsf_desc *b, int t0, in
     // frequencies: 38k Mcell-updates/s at depth 8 rows 16, 60k at depth 4
     // rows 4)
     fd2d_tm_desc dd = *d;
+    dd.form = FDTD_TM_FORM_STRIP;          // TF/SF corrections and DFT: strip form
     if (b->nf && !dd.depth) dd.depth = 4;
     if (b->nf && !dd.rows) {
         const int h = d->nx / 220;
@@ -1011,7 +1050,9 @@ int fd2d_lbc_run(const fd2d_tm_desc *d, const fd2d_lbc_desc *b, int t0, int nste
     if (nsteps == 0) return 0;
     if (nsteps < 2) return fail("fd2d_lbc_run: nsteps=%d (at least 2: launches come in pairs)", nsteps);
     Plan pl;
-    rc = plan_tm(d, 2 * 8, &pl);
+    fd2d_tm_desc ds = *d;
+    ds.form = FDTD_TM_FORM_STRIP;         // the split launches are single-wave strips
+    rc = plan_tm(&ds, 2 * 8, &pl);
     if (rc) return rc;
     if (pl.depth > 8) return fail("fd2d_lbc_run: depth %d above 8", pl.depth);
     const bool lossy = d->md.nbz != nullptr;
@@ -1138,10 +1179,13 @@ int fd2d_batch_upload(const fd2d_grid *grids, int ngrids, int depth, int rows, i
     int rc = batch_geom(depth, rows, cols, &g);
     if (rc) return rc;
     bool vec = g.cols == 2;
+    int min_wave = INT_MAX, min_probe = INT_MAX;
     for (int k = 0; k < ngrids; ++k) {
         rc = check_grid(grids[k], k);
         if (rc) return rc;
         vec = vec && (grids[k].ny % 2 == 0);
+        if (grids[k].src_i >= 0) min_wave = std::min(min_wave, std::max(grids[k].wave_len, 0));
+        if (grids[k].probe_i >= 0) min_probe = std::min(min_probe, std::max(grids[k].probe_len, 0));
     }
     const int w = 64 * g.cols - 2 * g.halo;
     // work list: units holding a grid edge, PML band, source or probe first
@@ -1191,6 +1235,8 @@ int fd2d_batch_upload(const fd2d_grid *grids, int ngrids, int depth, int rows, i
     out->cols = g.cols;
     out->waves = std::max(1, std::min(waves, n));
     out->vec = vec ? 1 : 0;
+    out->wave_len = min_wave;
+    out->probe_len = min_probe;
     return 0;
 }
 

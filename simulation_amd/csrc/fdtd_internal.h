@@ -21,6 +21,27 @@ int hip_check(hipError_t e, const char *what);
 // Clears the per-thread message (called at the start of every entry point).
 void clear_error();
 
+// ------------------------------------------------------ code-object loading
+// The HIP runtime loads a translation unit's device code object at the first
+// use of one of its kernels.  Asking for a kernel's attributes is such a use;
+// each kernel translation unit exposes one preload_* function that does it
+// for a kernel of its own, so fdtd_preload() can load every code object
+// before a timed or latency-sensitive run.
+inline int preload_kernel(const void *kernel)
+{
+    hipFuncAttributes a;
+    return hip_check(hipFuncGetAttributes(&a, kernel), "fdtd_preload");
+}
+int preload_fd1d();
+int preload_fd2d();
+int preload_dform1d();
+int preload_tfsf();
+int preload_lbc();
+int preload_lbc_frame();
+int preload_tb();                    // every depth (fd2d_tb.hip)
+template <int K>
+int preload_tb_depth();
+
 // ------------------------------------------------------ kernel arguments
 
 struct PmlPtrs {
@@ -215,6 +236,21 @@ int launch_fd2d_tb(const TbParams &p, int depth, int cols, bool lossy, bool pml,
 template <int K>
 int launch_fd2d_tb_depth(const TbParams &p, int cols, bool lossy, bool pml, int part,
                          hipStream_t s);
+
+// Multi-wave form of the whole-grid launch (fd2d_mw_impl.h): interior
+// super-tiles swept by MW_WAVES waves together, frame tiles by single waves;
+// one column per lane, no plane wave, no DFT.  rows: rows of an interior
+// super-tile; frows: rows of a frame tile.
+constexpr int MW_WAVES = 4;
+int launch_fd2d_mw(const TbParams &p, int depth, bool lossy, bool pml, int rows, int frows,
+                   hipStream_t s);
+int preload_mw();
+// one instantiation per depth (fd2d_mw_k<depth>.hip)
+template <int K>
+int launch_fd2d_mw_depth(const TbParams &p, bool lossy, bool pml, int rows, int frows,
+                         hipStream_t s);
+template <int K>
+int preload_mw_depth();
 
 // Batched grids (lossy PML form): persistent launch over a work list of
 // 4-int units {grid, strip, row block, 0}; parity 1 reads the partner sets.

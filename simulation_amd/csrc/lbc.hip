@@ -547,6 +547,8 @@ __global__ void __launch_bounds__(256) fit_kernel(const double *gram, double rho
 
 }  // namespace
 }  // namespace lbc
+
+int preload_lbc() { return preload_kernel(reinterpret_cast<const void *>(&lbc::pack_kernel)); }
 }  // namespace fdtd
 
 using namespace fdtd;

@@ -530,6 +530,11 @@ __global__ void __launch_bounds__(NT) frame_kernel(LbcFrameParams P)
 }  // namespace
 }  // namespace lbc
 
+int preload_lbc_frame()
+{
+    return preload_kernel(reinterpret_cast<const void *>(&lbc::frame_setup_kernel));
+}
+
 size_t lbc_frame_smem() { return sizeof(lbc::FrameSmem); }
 
 size_t lbc_frame_table_bytes(int nx, int ny, int K)

@@ -205,6 +205,8 @@ __global__ void __launch_bounds__(1024) inc_chunk_lds_kernel(int ny, float *ezi,
 
 }  // namespace
 
+int preload_tfsf() { return preload_kernel(reinterpret_cast<const void *>(&set_kernel)); }
+
 int launch_inc_chunk(int ny, float *ezi, float *hxi, float *bc, float *r_in, float *i_in,
                      float *table, const IncChunk &a, hipStream_t s)
 {

@@ -233,6 +233,7 @@ class DFormSolver:
     def __init__(self, nx, md: medium1d, nsteps_max, freq=(), dt=None, src_pos=1,
                  src_wave=None, probes=(), device=None, fused=True):
         dev = device or require_gpu()
+        _lib.preload()
         self.nx, self.md, self.device = int(nx), md, dev
         z = lambda: torch.zeros(nx, dtype=torch.float32, device=dev)  # noqa: E731
         self.dx, self.ix, self.ex, self.hy = z(), z(), z(), z()

@@ -261,12 +261,16 @@ class TMSolver:
     sources: [(i, j, wave)] with wave a host array of the hard-source values of
     steps 1..wave_len; probes: [(i, j)] recorded every step up to probe_len.
     The state arrays follow fd2d/pycuda/fd2d_3_4.py:279-286; a partner set of
-    the same shapes is allocated for the ping-pong launches.
+    the same shapes is allocated for the ping-pong launches.  form:
+    _lib.TM_FORM_AUTO (the library's choice), _lib.TM_FORM_STRIP (single-wave
+    strips) or _lib.TM_FORM_MULTIWAVE (interior blocks swept by four waves
+    that exchange their edge columns); all give the same bits.
     """
 
     def __init__(self, nx, ny, pml, md, sources=(), probes=(), nsteps_max=None,
-                 depth=0, rows=0, cols=0, device=None):
+                 depth=0, rows=0, cols=0, device=None, form=_lib.TM_FORM_AUTO):
         dev = device or require_gpu()
+        _lib.preload()
         self.nx, self.ny, self.device = int(nx), int(ny), dev
         self.pml, self.md = pml, md
         self.lossy = md.nbz is not None
@@ -296,6 +300,7 @@ class TMSolver:
         self.probe_out = (torch.zeros((len(probes), self.probe_len), dtype=torch.float32, device=dev)
                           if probes else None)
         self.depth, self.rows, self.cols = depth, rows, cols
+        self.form = int(form)
         self.band_rows, self.band_cols = self.default_band()
         self.t = 0
         self._work = torch.empty(_lib.ENERGY_WORK, dtype=torch.float64, device=dev)
@@ -347,6 +352,7 @@ class TMSolver:
         d.depth, d.rows, d.cols = self.depth, self.rows, self.cols
         d.band_rows[0], d.band_rows[1] = self.band_rows
         d.band_cols[0], d.band_cols[1] = self.band_cols
+        d.form = self.form
         return d
 
     def plan(self, nsteps: int) -> dict:
@@ -407,6 +413,7 @@ class TMConfig:
     depth: int = 0
     rows: int = 0
     cols: int = 0
+    form: int = 0
 
 
 def c2_config(n: int = 4096, ns: int = 2000) -> TMConfig:
@@ -461,7 +468,7 @@ def build_solver(cfg: TMConfig, device=None) -> TMSolver:
     srcs = [(i, j, waveform("gaussian", cfg.ns, 0, np.float32, center=t0, spread=sp))
             for i, j, t0, sp in cfg.sources]
     return TMSolver(cfg.nx, cfg.ny, pml, md, srcs, cfg.probes, nsteps_max=cfg.ns,
-                    depth=cfg.depth, rows=cfg.rows, cols=cfg.cols, device=dev)
+                    depth=cfg.depth, rows=cfg.rows, cols=cfg.cols, device=dev, form=cfg.form)
 
 
 def simulate(cfg: TMConfig, device=None, snapshot_fields=("ez", "hy"), out_dir=None) -> dict:

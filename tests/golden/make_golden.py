@@ -19,10 +19,14 @@ and non-square grids):
   fd1d_c1_{f32,f64}    test_1_2.c / fd1d_1_2.c   nx=200, 500 steps, hard Gaussian at 100,
                                                   probe ex[150], snapshots every 100 steps
   fd1d_1_3_{f32,f64}   test_1_3.c / fd1d_1_3.c   nx=200, 300 steps, cb, additive Gaussian
+  fd1d_1_4_{f32,f64}   test_1_4.c / fd1d_1_4.c   nx=200, 300 steps, cb, additive 700 MHz sinusoid
   fd1d_1_5_{f32,f64}   test_1_5.c / fd1d_1_5.c   nx=200, 400 steps, ca/cb, additive sinusoid
   fd1d_debye_{f32,f64} test_2_3.c / fd1d_2_3.c   nx=300, 600 steps, D/E form, Debye half space,
                                                   additive Gaussian at 1, DFT at 3 frequencies
   fd1d_lossy_{f32,f64} test_2_2.c / fd1d_2_2.c   the same with the lossy half space (no Debye)
+  fd1d_2_1_{f32,f64}   test_2_1.c / fd1d_2_1.c   nx=300, 500 steps, D/E form, lossy half space
+                                                  (eps_r 4, sigma 0.04), additive 700 MHz sinusoid
+                                                  at dx[1], no DFT
   fd2d_3_1_{f32,f64}   test_3_1.c / fd2d_3_1.c   60x70, 60 steps, free space
   fd2d_3_2_{f32,f64}   test_3_2.c / fd2d_3_2.c   64x80, npml=8, 100 steps, PML, sinusoid
   fd2d_c2_{f32,f64}    test_3_4.c / fd2d_3_4.c   97x131, npml=8, 150 steps: PML dfield/hfield,
@@ -31,6 +35,9 @@ and non-square grids):
   fd2d_tfsf_{f32,f64}  test_3_4.c / fd2d_3_4.c   70x90, npml=8, 160 steps: plane wave through the
                                                   TF/SF box (incident grid + two-step boundary),
                                                   PML, lossy cylinder, running DFT at 3 frequencies
+  fd2d_3_3_{f32,f64}   test_3_3.c / fd2d_3_3.c   70x90, npml=8, 160 steps: plane wave through the
+                                                  TF/SF box, PML, lossless medium (efield ez =
+                                                  naz*dz; naz of a lossless cylinder), no DFT
   fd2d_dielectric_c    fd2d_3_4.c dielectric (FP64, integer sub-sampling), 97x131
   fd2d_pmlparam_c      fd2d_3_2.c / test_3_2.c pmlparam (FP64 / FP32) for npml 8, 32, 64
 """
@@ -300,11 +307,13 @@ def case_fd2d_tfsf(tag, prog, prog64, dt, nx, ny, ns, npml, rgrid, epsr, sigma, 
          **{f"out_ft_{k}": v for k, v in ft.items()})
 
 
-def case_fd1d_dform(tag, prog, dt, nx, ns, debye, freqs, probes):
-    """The 1-D D/E-form program's main loop (dxfield with its additive
-    gaussian(t, 50, 10) at cell 1, exfield, fourier, hyfield) with its own
-    dielectric: lossy (test_2_2.c / fd1d_2_2.c) or Debye (test_2_3.c /
-    fd1d_2_3.c), running DFT at freqs."""
+def case_fd1d_dform(tag, prog, dt, nx, ns, debye, freqs, probes, source=None, epsr=2.0,
+                    sigma=0.01):
+    """The 1-D D/E-form program's main loop (dxfield with its additive source
+    at cell 1 -- gaussian(t, 50, 10), or the 700 MHz sinusoid of test_2_1.c
+    -- exfield, fourier when the program has a DFT, hyfield) with its own
+    dielectric: lossy (test_2_1.c, test_2_2.c) or Debye (test_2_3.c), running
+    DFT at freqs (none for test_2_1.c)."""
     lib = load(prog)
     R = real(dt)
     ptr = C.POINTER(R)
@@ -319,7 +328,7 @@ def case_fd1d_dform(tag, prog, dt, nx, ns, debye, freqs, probes):
         dtv = np.float32(np.float32(0.01) / np.float32(6e8))
     else:
         dtv = np.float64(0.01 / 6e8)
-    chi, tau, epsr, sigma = 2.0, 0.001e-6, 2.0, 0.01
+    chi, tau = 2.0, 0.001e-6
     lib.dielectric.restype = Medium
     if debye:
         md = lib.dielectric(C.c_int(nx), R(dtv), R(chi), R(tau), R(epsr), R(sigma))
@@ -341,17 +350,57 @@ def case_fd1d_dform(tag, prog, dt, nx, ns, debye, freqs, probes):
             lib.exfield(I(nx), C.byref(md), P(f["dx"]), P(f["ix"]), P(f["sx"]), P(f["ex"]))
         else:
             lib.exfield(I(nx), C.byref(md), P(f["dx"]), P(f["ix"]), P(f["ex"]))
-        lib.fourier(I(t), I(nf), I(nx), R(dtv), P(freq), P(f["ex"]), C.byref(fts))
+        if nf:
+            lib.fourier(I(t), I(nf), I(nx), R(dtv), P(freq), P(f["ex"]), C.byref(fts))
         lib.hyfield(I(nx), P(f["ex"]), P(f["hy"]), P(bc))
         for p, k in enumerate(probes):
             probe_out[p, t - 1] = f["ex"][k]
     meta = dict(program=prog, nx=nx, ns=ns, debye=debye, dt=float(dtv), chi=chi, tau=tau,
                 epsr=epsr, sigma=sigma, dtype=str(np.dtype(dt)), freqs=list(freqs),
-                source=dict(kind="gaussian", pos=1, t0=50, spread=10.0, mode="add"),
+                source=source or dict(kind="gaussian", pos=1, t0=50, spread=10.0, mode="add"),
                 dft_probe=10, probes=list(probes))
     outs = {f"out_{k}": v for k, v in f.items() if debye or k != "sx"}
     save(tag, meta, **{f"out_md_{k}": v for k, v in mda.items()}, in_freq=freq, **outs,
          out_bc=bc, out_probe=probe_out, **{f"out_ft_{k}": v for k, v in ft.items()})
+
+
+def case_fd2d_3_3(tag, prog, prog_coef, dt, nx, ny, ns, npml, rgrid, epsr):
+    """The lossless plane-wave program's main loop (ezinct, dfield with the
+    incident source ezi[3] = gaussian(t, 20, 8), inctdz, efield ez = naz*dz,
+    hxinct, hfield, incthx, incthy) with its own pmlparam; naz is that of a
+    lossless cylinder (the FP64 lossy program's dielectric with sigma = 0,
+    stored in the program's precision) so that the E-from-D factor varies."""
+    lib = load(prog)
+    pml = ref_pml(lib, dt, nx, ny, npml)
+    lib64 = load(prog_coef)
+    M64 = medium_struct(np.float64)
+    lib64.dielectric.restype = M64
+    dtv = float(np.float32(np.float32(0.01) / np.float32(6e8)))
+    md = lib64.dielectric(C.c_int(nx), C.c_int(ny), C.c_int(npml), C.c_int(rgrid),
+                          C.c_double(dtv), C.c_double(epsr), C.c_double(0.0))
+    naz = np.ctypeslib.as_array(md.naz, shape=(nx, ny)).copy().astype(dt)
+    S = pml_struct(dt)
+    s = S(*[P(pml[k]) for k in PML_KEYS])
+    f = {k: np.zeros((nx, ny), dt) for k in ("dz", "ez", "hx", "hy", "ihx", "ihy")}
+    ezi = np.zeros(ny, dt)
+    hxi = np.zeros(ny, dt)
+    bc = np.zeros(4, dt)
+    I = C.c_int
+    for t in range(1, ns + 1):
+        lib.ezinct(I(ny), P(ezi), P(hxi), P(bc))
+        lib.dfield(I(t), I(nx), I(ny), C.byref(s), P(ezi), P(f["dz"]), P(f["hx"]), P(f["hy"]))
+        lib.inctdz(I(nx), I(ny), I(npml), P(hxi), P(f["dz"]))
+        lib.efield(I(nx), I(ny), P(naz), P(f["dz"]), P(f["ez"]))
+        lib.hxinct(I(ny), P(ezi), P(hxi))
+        lib.hfield(I(nx), I(ny), C.byref(s), P(f["ez"]), P(f["ihx"]), P(f["ihy"]), P(f["hx"]),
+                   P(f["hy"]))
+        lib.incthx(I(nx), I(ny), I(npml), P(ezi), P(f["hx"]))
+        lib.incthy(I(nx), I(ny), I(npml), P(ezi), P(f["hy"]))
+    meta = dict(program=prog, coef_program=prog_coef, nx=nx, ny=ny, ns=ns, npml=npml,
+                rgrid=rgrid, epsr=epsr, sigma=0.0, dt=dtv, dtype=str(np.dtype(dt)), freqs=[],
+                incident=dict(pos=3, t0=20, spread=8.0), dft_probe=6)
+    save(tag, meta, in_naz=naz, in_freq=np.zeros(0, dt), **{f"in_{k}": v for k, v in pml.items()},
+         **{f"out_{k}": v for k, v in f.items()}, out_ezi=ezi, out_hxi=hxi, out_bc=bc)
 
 
 def case_dielectric_c(tag, nx, ny, npml, rgrid, epsr, sigma):
@@ -384,6 +433,8 @@ def main():
         case_fd1d(f"{p}_1_2", f"fd1d_c1_{suf}", dt, 200, 500, "abc", probes=(150,), snap_every=100)
         case_fd1d(f"{p}_1_3", f"fd1d_1_3_{suf}", dt, 200, 300, "cb", probes=(150,),
                   extra=dict(epsr=4.0))
+        case_fd1d(f"{p}_1_4", f"fd1d_1_4_{suf}", dt, 200, 300, "cb", probes=(150,),
+                  extra=dict(epsr=4.0, source="sinusoidal"))
         case_fd1d(f"{p}_1_5", f"fd1d_1_5_{suf}", dt, 200, 400, "cacb", probes=(150,),
                   extra=dict(epsr=4.0, sigma=0.04))
         q = "test" if dt == np.float32 else "fd2d"
@@ -391,6 +442,10 @@ def main():
                         (50e6, 200e6, 500e6), probes=(100, 200))
         case_fd1d_dform(f"fd1d_lossy_{suf}", f"{p}_2_2", dt, 300, 600, False,
                         (50e6, 200e6, 500e6), probes=(100, 200))
+        case_fd1d_dform(f"fd1d_2_1_{suf}", f"{p}_2_1", dt, 300, 500, False, (), probes=(100, 200),
+                        source=dict(kind="sinusoidal", pos=1, freq=700e6, mode="add"),
+                        epsr=4.0, sigma=0.04)
+        case_fd2d_3_3(f"fd2d_3_3_{suf}", f"{q}_3_3", "fd2d_3_4", dt, 70, 90, 160, 8, 12, 4.0)
         case_fd2d_3_1(f"fd2d_3_1_{suf}", f"{q}_3_1", dt, 60, 70, 60,
                       probes=[(30, 35), (10, 20), (45, 60)])
         case_fd2d_3_2(f"fd2d_3_2_{suf}", f"{q}_3_2", dt, 64, 80, 100, 8,

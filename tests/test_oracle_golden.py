@@ -105,11 +105,16 @@ def test_tfsf_oracle_matches_reference(oracle, name):
                 c["freq"], c["dt"], ft, inc_pos=c["inc_pos"])
     e = c["exp"]
     for k, v in list(st.items()) + list(inc.items()):
+        if c["nbz"] is None and k == "iz":
+            continue
         assert np.array_equal(v, e["out_" + k]), k
     for k, v in ft.items():
-        assert np.array_equal(v, e["out_ft_" + k]), k
+        if nf:
+            assert np.array_equal(v, e["out_ft_" + k]), k
     # the plane wave crossed the box and the cylinder scattered it
-    assert np.abs(e["out_ez"]).max() > 0.1 and np.abs(e["out_ft_r_pt"]).max() > 1.0
+    assert np.abs(e["out_ez"]).max() > 0.1
+    if nf:
+        assert np.abs(e["out_ft_r_pt"]).max() > 1.0
 
 
 @pytest.mark.parametrize("name", DFORM_CASES)
@@ -118,7 +123,7 @@ def test_dform_oracle_matches_reference(oracle, name):
     NumPy restatement (coefficients included) against the reference
     program's own functions, bit-exact."""
     from oracle import numpy_oracle as NO
-    c = dform_case(name, oracle.gaussian)
+    c = dform_case(name, oracle.gaussian, oracle.sinusoidal)
     dt, nx, m = c["dtype"], c["nx"], c["meta"]
     md = NO.dielectric_dform(nx, c["dt"], m["chi"], m["tau"], m["epsr"], m["sigma"], c["debye"], dt)
     e = c["exp"]
