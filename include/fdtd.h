@@ -190,6 +190,20 @@ int fd2d_energy(int nx, int ny, int i0, int i1, int j0, int j1,
                 const float *ez, const float *hx, const float *hy,
                 double *work, double *result, void *stream);
 
+/* fd2d_energy without a work area, for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops).  No
+ * counterpart in the reference.  The partial sums live in a library-owned
+ * static device array of FDTD_ENERGY_WORK doubles (8200 B per GPU, held until
+ * the process ends); the reduction needs exactly that for any window, so no
+ * window exceeds it.  result: device pointer to one double, written
+ * asynchronously on `stream` (the call neither synchronises nor copies to
+ * the host).  Allocates nothing.  Every call on a GPU shares that GPU's
+ * array: calls that overlap on one GPU (other streams, other host threads)
+ * race.  Nothing in the array needs to start at zero.                        */
+int fd2d_energy_noalloc(int nx, int ny, int i0, int i1, int j0, int j1,
+                        const float *ez, const float *hx, const float *hy,
+                        double *result, void *stream);
+
 /* FP64 energy of the difference of two field sets over equal wn x wm
  * windows -- window (ia0, ja0) of grid a (nxa x nya) against (ib0, jb0) of
  * grid b: sum (ez_a - ez_b)^2 + (hx_a - hx_b)^2 + (hy_a - hy_b)^2.  Used to

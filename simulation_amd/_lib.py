@@ -169,6 +169,7 @@ SIGNATURES = {
     "fd2d_pmlparam": (_I, [_I, _I, _I, C.POINTER(FdtdPml), _P]),
     "fd2d_dielectric": (_I, [_I, _I, _I, _D, _I, _DP, _I, _P, _P, _P]),
     "fd2d_energy": (_I, [_I, _I, _I, _I, _I, _I, _P, _P, _P, _P, _DP, _P]),
+    "fd2d_energy_noalloc": (_I, [_I, _I, _I, _I, _I, _I, _P, _P, _P, _P, _P]),
     "fd2d_energy_diff": (_I, [_I, _I, _I, _I, _I, _I, _P, _P, _P, _I, _I, _I, _I, _P, _P, _P, _P,
                               _DP, _P]),
     "fd2d_tm_run": (_I, [C.POINTER(Fd2dTmDesc), _I, _I, _P]),

@@ -6,6 +6,8 @@
 // (fd2d/pycuda/fd2d_3_2.py:219-260, fd2d/pycuda/fd2d_3_4.py:175-185); the
 // library is built with -ffp-contract=off so each product and sum rounds on
 // its own, as in the reference's C programs.
+#include <mutex>
+
 #include "fdtd_internal.h"
 
 namespace fdtd {
@@ -369,6 +371,49 @@ int launch_fd2d_energy_diff(int wn, int wm, int nya, int ia0, int ja0, const flo
     if (rc) return rc;
     hipLaunchKernelGGL(fd2d_energy_final, dim3(1), dim3(ET), 0, s, work, work);
     return hip_check(hipGetLastError(), "fd2d_energy_diff launch");
+}
+
+// ------------------------------------------------ energy without a workspace
+// The partial sums of fd2d_energy_noalloc live in this file-scope device
+// array (one copy per GPU, held until the process ends).  Host code never
+// passes `scratch` itself to a kernel (in host code that name is a host-side
+// stand-in): it passes the device address hipGetSymbolAddress returns for the
+// current device, looked up once per device and kept.
+static __device__ __align__(256) double scratch[FDTD_ENERGY_WORK];
+
+int energy_scratch(double **out)
+{
+    static std::mutex mu;
+    static double *addr[64] = {};
+    int dev = 0;
+    int rc = hip_check(hipGetDevice(&dev), "hipGetDevice");
+    if (rc) return rc;
+    if (dev < 0 || dev >= 64) return fail("device ordinal %d out of range", dev);
+    std::lock_guard<std::mutex> lock(mu);
+    if (!addr[dev]) {
+        void *p = nullptr;
+        rc = hip_check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "scratch address");
+        if (rc) return rc;
+        addr[dev] = static_cast<double *>(p);
+    }
+    *out = addr[dev];
+    return 0;
+}
+
+size_t energy_scratch_elems() { return FDTD_ENERGY_WORK; }
+
+int launch_fd2d_energy_to(int nx, int ny, int i0, int i1, int j0, int j1, const float *ez,
+                          const float *hx, const float *hy, double *work, double *out,
+                          hipStream_t s)
+{
+    (void)nx;
+    static_assert(1 + EB <= FDTD_ENERGY_WORK, "partials exceed the work area");
+    hipLaunchKernelGGL(fd2d_energy_partial, dim3(EB), dim3(ET), 0, s,
+                       ny, i0, i1, j0, j1, ez, hx, hy, work);
+    int rc = hip_check(hipGetLastError(), "fd2d_energy launch");
+    if (rc) return rc;
+    hipLaunchKernelGGL(fd2d_energy_final, dim3(1), dim3(ET), 0, s, work, out);
+    return hip_check(hipGetLastError(), "fd2d_energy launch");
 }
 
 }  // namespace fdtd

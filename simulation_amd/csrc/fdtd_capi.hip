@@ -273,6 +273,8 @@ int fdtd_preload(void)
     if (!rc) rc = preload_tfsf();
     if (!rc) rc = preload_lbc();
     if (!rc) rc = preload_lbc_frame();
+    double *scr = nullptr;          // device address of the energy scratch
+    if (!rc) rc = energy_scratch(&scr);
     return rc;
 }
 
@@ -473,6 +475,22 @@ int fd2d_energy(int nx, int ny, int i0, int i1, int j0, int j1, const float *ez,
                    "fd2d_energy copy");
     if (rc) return rc;
     return hip_check(hipStreamSynchronize(S(stream)), "fd2d_energy sync");
+}
+
+int fd2d_energy_noalloc(int nx, int ny, int i0, int i1, int j0, int j1, const float *ez,
+                        const float *hx, const float *hy, double *result, void *stream)
+{
+    clear_error();
+    if (!ez || !hx || !hy || !result) return fail("fd2d_energy_noalloc: ez, hx, hy, result required");
+    if (nx < 1 || ny < 1) return fail("fd2d_energy_noalloc: grid %dx%d", nx, ny);
+    if (i0 < 0 || j0 < 0 || i1 > nx || j1 > ny || i0 > i1 || j0 > j1)
+        return fail("fd2d_energy_noalloc: window [%d,%d)x[%d,%d) outside %dx%d", i0, i1, j0, j1, nx, ny);
+    // the partial sums need 1 + 1024 doubles whatever the window: the
+    // scratch holds exactly that, so no window can exceed it
+    double *scr = nullptr;
+    int rc = energy_scratch(&scr);
+    if (rc) return rc;
+    return launch_fd2d_energy_to(nx, ny, i0, i1, j0, j1, ez, hx, hy, scr, result, S(stream));
 }
 
 int fd2d_energy_diff(int wn, int wm, int nxa, int nya, int ia0, int ja0, const float *eza,

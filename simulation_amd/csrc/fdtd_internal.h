@@ -222,6 +222,14 @@ int launch_fd2d_dielectric(int nx, int ny, int npml, double dt, const CylList &c
                            int int_sub, float *naz, float *nbz, hipStream_t s);
 int launch_fd2d_energy(int nx, int ny, int i0, int i1, int j0, int j1, const float *ez,
                        const float *hx, const float *hy, double *work, hipStream_t s);
+// The same reduction with the result written to *out (device).
+int launch_fd2d_energy_to(int nx, int ny, int i0, int i1, int j0, int j1, const float *ez,
+                          const float *hx, const float *hy, double *work, double *out,
+                          hipStream_t s);
+// Device address of the library's static energy scratch on the current
+// device (FDTD_ENERGY_WORK doubles; looked up once per device).
+int energy_scratch(double **out);
+size_t energy_scratch_elems();
 
 // Temporally blocked TM launch (fd2d_tb_impl.h): depth in [1,
 // tb_max_depth()], cols in {1,2}; part TB_WHOLE: every tile in one launch
