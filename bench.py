@@ -77,8 +77,9 @@ def parse():
                     help="learned: network parameters (.npz from simulation_amd.learned); "
                          "default: the committed weights, else the zero network")
     ap.add_argument("--graph", action="store_true", help="batch: time one captured graph launch")
-    ap.add_argument("--tune", action="store_true",
-                    help="choose depth/rows/cols with the autotuner before timing")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="tm/large: skip the autotuner (default: it chooses depth/rows/cols/form "
+                         "before the warm-up, unless --depth/--rows/--cols/--form fix them)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a plain 1 GiB device copy three times (rocprofv3 FETCH_SIZE/WRITE_SIZE "
                          "scale) and exit")
@@ -672,9 +673,14 @@ def main():
     cfg.form = {"auto": 0, "strip": 1, "multiwave": 2}[a.form]
     s = fd2d.build_solver(cfg, dev)          # loads every code object (fdtd_preload)
     tuned = None
-    if a.tune:
+    fixed = a.depth or a.rows or a.cols or a.form != "auto"
+    if not (a.no_tune or fixed):
         from simulation_amd import autotune
-        tuned = autotune.tune(s)
+        # timed at the run's own length (up to 48 steps), so the chosen shape
+        # is the fastest for this plan and its kernel has run before the
+        # timed region
+        tuned = autotune.tune(s, steps=min(K, 48), use_cache=False)
+        a.form = {0: "auto", 1: "strip", 2: "multiwave"}[tuned.get("form", 0)]
     plan = s.plan(K)
     s.run(W)
     torch.cuda.synchronize()

@@ -271,10 +271,11 @@ typedef struct fd2d_tm_desc {
 int fd2d_tm_run(const fd2d_tm_desc *d, int t0, int nsteps, void *stream);
 
 /* The launch plan fd2d_tm_run would use for this descriptor and nsteps:
- * number of launches and the chosen depth/rows/cols (defaults: depth 8, one
- * column per lane; rows = nx/220 clamped to [16, 512] for single-wave
- * strips, the multi-wave default below for the multi-wave form).  Host
- * only.                                                                    */
+ * number of launches and the chosen depth/rows/cols.  Defaults (measured on
+ * MI355X): one column per lane; depth 10 from nx = 6144 on, else 8; rows =
+ * nx/220 clamped to [16, 512], except 64 for 6144 <= nx < 12288; the
+ * multi-wave form: depth 8, rows = nx/32 clamped to [32, 512].  Depths go up
+ * to 12.  Host only.                                                       */
 int fd2d_tm_plan(const fd2d_tm_desc *d, int nsteps,
                  int *launches, int *depth, int *rows, int *cols);
 
@@ -368,7 +369,10 @@ int fdtd_graph_destroy(void *graph_exec);
  * Parameters: one FP32 vector of lbc_param_count() values, in the order
  * W1 (64x108, row-major), b1 (64), W2 (64x64), b2, W3 (3x64), b3, L (3x108);
  * the hidden layers read bf16 copies from a buffer of lbc_wbuf_elems()
- * uint16 values written by lbc_pack (and kept current by lbc_adam).
+ * uint16 values written by lbc_pack (and kept current by lbc_adam); its last
+ * elements hold a flag, non-zero when W3 or b3 has a non-zero value: with a
+ * zero output layer the hidden layers add exactly nothing and fd2d_lbc_run
+ * skips them.
  *
  * Canonical sample (30 floats): 27 features at index (f*3 + d)*3 + o + 1 for
  * field f, feature depth d = 0..2 (grid depth D = d + 3: the three cells just

@@ -25,13 +25,18 @@ CACHE = os.environ.get("FDTD_TUNE_CACHE",
 
 
 def candidates(nx: int):
-    """(depth, rows, cols) shapes tried for an nx-row grid."""
-    base = min(512, max(16, nx // 110))
-    rows = sorted({max(16, base // 2), base, min(512, base * 2)})
-    for depth in (8, 6, 4):
-        for cols in (1, 2):
-            for r in rows:
-                yield depth, r, cols
+    """(depth, rows, cols, form) shapes tried for an nx-row grid: single-wave
+    strips at depths 10 and 8 with row blocks of nx/220 and nx/128 rows
+    (MI355X sweeps put the optimum there: 4096^2 at depth 8 and ~nx/220,
+    8192^2 at depth 10 and ~nx/128, 16384^2 at depth 10 and ~nx/220), two
+    columns per lane at the first of those, and the multi-wave form with
+    short blocks."""
+    rows = sorted({min(512, max(16, nx // d)) for d in (220, 128)})
+    for depth in (10, 8):
+        for r in rows:
+            yield depth, r, 1, 1
+    yield 8, rows[0], 2, 1
+    yield 8, 32, 1, 2
 
 
 def batch_candidates():
@@ -64,21 +69,25 @@ def _save(db):
         pass
 
 
-def _time(fn, reps: int = 1) -> float:
-    fn()                                   # warm-up: code load, first touch
+def _time(fn, reps: int = 2) -> float:
+    """Best of `reps` timed calls after one untimed call."""
+    fn()                                   # warm-up: first touch of the shape
     torch.cuda.synchronize()
-    t0 = time.perf_counter()
+    best = float("inf")
     for _ in range(reps):
+        t0 = time.perf_counter()
         fn()
-    torch.cuda.synchronize()
-    return (time.perf_counter() - t0) / reps
+        torch.cuda.synchronize()
+        best = min(best, time.perf_counter() - t0)
+    return best
 
 
 def tune(solver: fd2d.TMSolver, steps: int = 48, use_cache: bool = True, configs=None,
          verbose: bool = False) -> dict:
-    """Time (depth, rows, cols) candidates on this solver's grid and set the
-    fastest on it.  The solver's state, probe traces and step are restored."""
-    key = (f"tm|{_device()}|{solver.nx}x{solver.ny}|lossy={int(solver.lossy)}"
+    """Time (depth, rows, cols[, form]) candidates on this solver's grid and
+    set the fastest on it.  The solver's state, probe traces and step are
+    restored."""
+    key = (f"tm2|{_device()}|{solver.nx}x{solver.ny}|lossy={int(solver.lossy)}"
            f"|pml={int(solver.pml is not None)}")
     db = _load() if use_cache and configs is None else {}
     if key in db:
@@ -90,29 +99,34 @@ def tune(solver: fd2d.TMSolver, steps: int = 48, use_cache: bool = True, configs
         n = steps if not solver.probes else min(steps, solver.probe_len)
         n -= n % 2
         results = []
-        for depth, rows, cols in (configs or candidates(solver.nx)):
-            solver.depth, solver.rows, solver.cols = depth, rows, cols
+        for cand in (configs or candidates(solver.nx)):
+            depth, rows, cols = cand[:3]
+            form = cand[3] if len(cand) > 3 else 0
+            solver.depth, solver.rows, solver.cols, solver.form = depth, rows, cols, form
 
             def go():
                 solver.t = 0
                 solver.run(n)
             dt = _time(go)
             rate = solver.nx * solver.ny * n / dt
-            results.append((rate, depth, rows, cols))
+            results.append((rate, depth, rows, cols, form))
             if verbose:
-                print(f"  tm depth={depth} rows={rows} cols={cols}: {rate / 1e9:.1f} G/s", flush=True)
+                print(f"  tm depth={depth} rows={rows} cols={cols} form={form}: "
+                      f"{rate / 1e9:.1f} G/s", flush=True)
         for k, v in saved.items():
             getattr(solver, k).copy_(v)
         if probe_saved is not None:
             solver.probe_out.copy_(probe_saved)
         solver.t = t_saved
-        rate, depth, rows, cols = max(results)
-        best = {"depth": depth, "rows": rows, "cols": cols, "gcups": round(rate / 1e9, 2)}
+        rate, depth, rows, cols, form = max(results)
+        best = {"depth": depth, "rows": rows, "cols": cols, "form": form,
+                "gcups": round(rate / 1e9, 2)}
         if use_cache and configs is None:
             db = _load()
             db[key] = best
             _save(db)
     solver.depth, solver.rows, solver.cols = best["depth"], best["rows"], best["cols"]
+    solver.form = best.get("form", 0)
     return best
 
 

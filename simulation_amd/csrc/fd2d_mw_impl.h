@@ -243,6 +243,11 @@ __device__ __forceinline__ void level(const TbParams &P, const MwGeom &g, const 
     }
 }
 
+__device__ __forceinline__ float uniform(float v)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, v)));
+}
+
 // Level s is active at positions [beg(s), end(s)): E rows [i0-K+1+s,
 // i1+K-s) and H rows [i0-K+1+s, i1+K-1-s) -- the rows the written rows of
 // the last level depend on.  The H update of the first active position
@@ -315,40 +320,50 @@ __device__ __forceinline__ void sweep(const TbParams &P, const MwGeom &g, Xch<K>
                 xb[b ^ 1] = *at(P.hx_i, (unsigned)rn * ny + xcol);
             }
             const int sl = b ^ 1;                // LDS slot of the previous position
+            // neighbours' edge values of the previous position, wave-uniform
+            // (held in scalar registers): the left wave's hx of each level
+            // (index w) and the right wave's ez of each level (index w+2)
+            float hxl[K], ezr[K];
+#pragma unroll
+            for (int s = 0; s < K; ++s) {
+                hxl[s] = uniform(X.hx[sl][g.w][s]);
+                ezr[s] = uniform(X.ez[sl][g.w + 2][s]);
+            }
             float hxo[K], ezo[K];
+            // the coefficients of level s's row come from the ring, read one
+            // level ahead of their use
+            float2 nab_next = make_float2(0.f, 0.f);
+            if (K > 1) nab_next = ring[(pp - 2 * (K - 1)) & (RING - 1)][g.lane];
 #pragma unroll
             for (int s = K - 1; s >= 0; --s) {
                 hxo[s] = 0.f;
                 ezo[s] = 0.f;
+                const float2 nab_s = nab_next;
+                if (s >= 2) nab_next = ring[(pp - 2 * (s - 1)) & (RING - 1)][g.lane];
                 const bool act = pp >= lv_beg<K>(g.i0, s) && pp < lv_end<K>(g.i1, s);
                 if (!act) continue;
-                // neighbours' edge values of the previous position: the left
-                // wave's hx of level s-1 (index w), the right wave's ez of
-                // level s (index w+2)
-                const float ezr = X.ez[sl][g.w + 2][s];
                 if (s == 0) {
                     const float2 nab = make_float2(Dcur.na[0], Dcur.nb[0]);
                     if (K == 1) {
                         DRow<1> dd; HRow<1> hh;
                         level<K, LOSSY, PML, SP, true>(P, g, X, 0, pp, Dcur, nab, Hcur, Hprev, xcur,
-                                                       ezr, Ec[0], dd, hh, ezo[0]);
+                                                       ezr[0], Ec[0], dd, hh, ezo[0]);
                     } else {
                         level<K, LOSSY, PML, SP, false>(P, g, X, 0, pp, Dcur, nab, Hcur, Hprev, xcur,
-                                                        ezr, Ec[0], Dc[0][b], Hc[0][b], ezo[0]);
+                                                        ezr[0], Ec[0], Dc[0][b], Hc[0][b], ezo[0]);
                         hxo[0] = Hc[0][b].hx[0];
                     }
                 } else {
-                    const float hxl = X.hx[sl][g.w][s - 1];
-                    const float2 nab = ring[(pp - 2 * s) & (RING - 1)][g.lane];
+                    const float2 nab = nab_s;
                     if (s == K - 1) {
                         DRow<1> dd; HRow<1> hh;
                         level<K, LOSSY, PML, SP, true>(P, g, X, s, pp, Dc[s - 1][b], nab,
-                                                       Hc[s - 1][b ^ 1], Hc[s - 1][b], hxl, ezr,
-                                                       Ec[s], dd, hh, ezo[s]);
+                                                       Hc[s - 1][b ^ 1], Hc[s - 1][b], hxl[s - 1],
+                                                       ezr[s], Ec[s], dd, hh, ezo[s]);
                     } else {
                         level<K, LOSSY, PML, SP, false>(P, g, X, s, pp, Dc[s - 1][b], nab,
-                                                        Hc[s - 1][b ^ 1], Hc[s - 1][b], hxl, ezr,
-                                                        Ec[s], Dc[s][b], Hc[s][b], ezo[s]);
+                                                        Hc[s - 1][b ^ 1], Hc[s - 1][b], hxl[s - 1],
+                                                        ezr[s], Ec[s], Dc[s][b], Hc[s][b], ezo[s]);
                         hxo[s] = Hc[s][b].hx[0];
                     }
                 }

@@ -5,7 +5,7 @@
 
 namespace fdtd {
 
-int tb_max_depth() { return 8; }
+int tb_max_depth() { return TB_MAX_DEPTH; }
 
 int preload_tb()
 {
@@ -18,6 +18,10 @@ int preload_tb()
     rc = rc ? rc : preload_tb_depth<6>();
     rc = rc ? rc : preload_tb_depth<7>();
     rc = rc ? rc : preload_tb_depth<8>();
+    rc = rc ? rc : preload_tb_depth<9>();
+    rc = rc ? rc : preload_tb_depth<10>();
+    rc = rc ? rc : preload_tb_depth<11>();
+    rc = rc ? rc : preload_tb_depth<12>();
     return rc;
 }
 
@@ -68,6 +72,10 @@ int launch_fd2d_tb(const TbParams &p, int depth, int cols, bool lossy, bool pml,
     case 6: return launch_fd2d_tb_depth<6>(p, cols, lossy, pml, part, s);
     case 7: return launch_fd2d_tb_depth<7>(p, cols, lossy, pml, part, s);
     case 8: return launch_fd2d_tb_depth<8>(p, cols, lossy, pml, part, s);
+    case 9: return launch_fd2d_tb_depth<9>(p, cols, lossy, pml, part, s);
+    case 10: return launch_fd2d_tb_depth<10>(p, cols, lossy, pml, part, s);
+    case 11: return launch_fd2d_tb_depth<11>(p, cols, lossy, pml, part, s);
+    case 12: return launch_fd2d_tb_depth<12>(p, cols, lossy, pml, part, s);
     default: return fail("temporal depth %d not built", depth);
     }
 }
@@ -83,6 +91,7 @@ int launch_fd2d_batch(const fd2d_grid *grids, const int *units, int nunits, int 
     switch (depth) {
         FDTD_BATCH_CASE(1) FDTD_BATCH_CASE(2) FDTD_BATCH_CASE(3) FDTD_BATCH_CASE(4)
         FDTD_BATCH_CASE(5) FDTD_BATCH_CASE(6) FDTD_BATCH_CASE(7) FDTD_BATCH_CASE(8)
+        FDTD_BATCH_CASE(9) FDTD_BATCH_CASE(10) FDTD_BATCH_CASE(11) FDTD_BATCH_CASE(12)
     default: return fail("temporal depth %d not built", depth);
     }
 #undef FDTD_BATCH_CASE
@@ -99,6 +108,10 @@ int fd2d_batch_resident(int depth, int cols, bool vec, int *waves)
     case 6: return fd2d_batch_resident_depth<6>(cols, vec, waves);
     case 7: return fd2d_batch_resident_depth<7>(cols, vec, waves);
     case 8: return fd2d_batch_resident_depth<8>(cols, vec, waves);
+    case 9: return fd2d_batch_resident_depth<9>(cols, vec, waves);
+    case 10: return fd2d_batch_resident_depth<10>(cols, vec, waves);
+    case 11: return fd2d_batch_resident_depth<11>(cols, vec, waves);
+    case 12: return fd2d_batch_resident_depth<12>(cols, vec, waves);
     default: return fail("temporal depth %d not built", depth);
     }
 }

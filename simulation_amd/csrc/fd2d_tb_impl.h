@@ -678,7 +678,7 @@ __global__ void __launch_bounds__(64) tm_tb_kernel(const TbParams P)
     }
     const bool interior = !brow && !bcol && !sp;
     if (interior) {
-        if (want_int) sweep<K, C, LOSSY, PML, VEC, false, false, false, NF, NF == 0, NF == 0>(P, g);
+        if (want_int) sweep<K, C, LOSSY, PML, VEC, false, false, false, NF, NF == 0, NF == 0 && K <= 8>(P, g);
     } else if (want_bnd) {
         if (!brow && !sp) sweep<K, C, LOSSY, PML, VEC, false, true, false, NF, NF == 0>(P, g);
         else sweep<K, C, LOSSY, PML, VEC, true, true, true, NF, NF == 0>(P, g);

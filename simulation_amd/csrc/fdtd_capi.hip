@@ -114,23 +114,41 @@ int mw_default_rows(int nx)
     return (int)h;
 }
 
+// Default depth and rows of the single-wave strips, from sweeps on MI355X
+// (lossy PML form, one column per lane; Mcell-updates/s at 2000 steps):
+//   4096^2:  depth 8, rows 18 (446k; depth 10 at rows 16-26: 362k-407k)
+//   8192^2:  depth 10, rows 64 (595k; rows 56/72: 556k/576k; depth 8 rows
+//            37/64: 561k/582k)
+//   16384^2: depth 10, rows 74 (618k; rows 100/128: 606k/588k; depth 8 rows
+//            74: 592k)
+// Deeper launches pay off from 8192^2 on (more steps per pass over HBM,
+// one carry set at 3 waves per SIMD); rows follow ~nx/220 except around
+// 8192^2, where blocks of 64 rows are best.
+void tm_defaults(int nx, int *depth, int *rows)
+{
+    *depth = nx >= 6144 ? 10 : 8;
+    long h = nx / 220;
+    if (nx >= 6144 && nx < 12288) h = 64;
+    if (h < 16) h = 16;
+    if (h > 512) h = 512;
+    *rows = (int)h;
+}
+
 int plan_tm(const fd2d_tm_desc *d, int nsteps, Plan *pl)
 {
-    int D = d->depth ? d->depth : 8;
+    int D0 = 8, H0 = 16;
+    tm_defaults(d->nx, &D0, &H0);
+    int D = d->depth ? d->depth : (d->form == FDTD_TM_FORM_MULTIWAVE ? 8 : D0);
     int C = d->cols ? d->cols : 1;
     if (!tb_supported(D, C)) return fail("fd2d_tm: depth=%d cols=%d not supported", D, C);
     if (64 * C - 2 * D < 8) return fail("fd2d_tm: depth %d too large for %d columns/lane", D, C);
     if (d->form != FDTD_TM_FORM_AUTO && d->form != FDTD_TM_FORM_STRIP &&
         d->form != FDTD_TM_FORM_MULTIWAVE)
         return fail("fd2d_tm: form=%d", d->form);
-    if (d->form == FDTD_TM_FORM_MULTIWAVE && C != 1)
-        return fail("fd2d_tm: the multi-wave form takes one column per lane (cols=%d)", C);
-    // single-wave strips: default from row sweeps on MI355X (depth 8, one
-    // column per lane, ~220 row blocks per strip: 4096^2 and 16384^2 at
-    // their best, 8192^2 within 1 %)
-    long h = d->nx / 220;
-    if (h < 16) h = 16;
-    if (h > 512) h = 512;
+    if (d->form == FDTD_TM_FORM_MULTIWAVE && (C != 1 || D > MW_MAX_DEPTH))
+        return fail("fd2d_tm: the multi-wave form takes one column per lane and depth <= %d "
+                    "(cols=%d depth=%d)", MW_MAX_DEPTH, C, D);
+    const long h = H0;
     const int mw = d->form == FDTD_TM_FORM_MULTIWAVE;
     int H = d->rows ? d->rows : (mw ? mw_default_rows(d->nx) : (int)h);
     if (H < 1) return fail("fd2d_tm: rows=%d", H);
@@ -1070,6 +1088,7 @@ int fd2d_lbc_run(const fd2d_tm_desc *d, const fd2d_lbc_desc *b, int t0, int nste
     Plan pl;
     fd2d_tm_desc ds = *d;
     ds.form = FDTD_TM_FORM_STRIP;         // the split launches are single-wave strips
+    if (!ds.depth) ds.depth = 8;          // and fuse at most 8 steps (the frame's tables)
     rc = plan_tm(&ds, 2 * 8, &pl);
     if (rc) return rc;
     if (pl.depth > 8) return fail("fd2d_lbc_run: depth %d above 8", pl.depth);

@@ -42,7 +42,16 @@ __global__ void __launch_bounds__(256) sample_kernel(int nx, int ny, int ld, con
 __global__ void __launch_bounds__(256) pack_kernel(const float *p, uint16_t *w)
 {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= WBUF) return;
+    if (q >= WBUF_ALL) return;
+    if (q >= WBUF) {
+        // the flag: does the hidden layers' output layer hold a non-zero
+        // parameter (W3 and b3 are contiguous in the FP32 layout)?
+        uint16_t any = 0;
+        if (q == W_MLP)
+            for (int k = P_W3; k < P_L; ++k) any |= p[k] != 0.0f;
+        w[q] = any;
+        return;
+    }
     float v = 0.0f;
     if (q < W_W2) {
         int j = q / K1, k = q % K1;
@@ -349,6 +358,10 @@ __global__ void __launch_bounds__(256) adam_kernel(int nblocks, const float *par
     v[q] = vv;
     const float val = p[q] - lr * (mm / c1) / (sqrtf(vv / c2) + eps);
     p[q] = val;
+    // the hidden layers' output turned non-zero: the time-stepping kernel
+    // must evaluate them (the flag is only ever raised here; lbc_pack
+    // recomputes it)
+    if (q >= P_W3 && q < P_L && val != 0.0f) w[W_MLP] = 1;
     const uint16_t h = f2bf(val);
     const int fs = frm_slot(q);
     if (fs >= 0) w[fs] = h;
@@ -559,7 +572,7 @@ static hipStream_t S(void *s) { return reinterpret_cast<hipStream_t>(s); }
 extern "C" {
 
 int lbc_param_count(void) { return NPARAM; }
-int lbc_wbuf_elems(void) { return WBUF; }
+int lbc_wbuf_elems(void) { return WBUF_ALL; }
 size_t lbc_train_smem(void) { return sizeof(TrainSmem); }
 
 int lbc_sample(int nx, int ny, int ld, const float *ez, const float *hx, const float *hy,
@@ -576,7 +589,7 @@ int lbc_sample(int nx, int ny, int ld, const float *ez, const float *hx, const f
 int lbc_pack(const float *params, uint16_t *wbuf, void *stream)
 {
     if (!params || !wbuf) return fail("lbc_pack: null pointer");
-    hipLaunchKernelGGL(pack_kernel, dim3((WBUF + 255) / 256), dim3(256), 0, S(stream), params,
+    hipLaunchKernelGGL(pack_kernel, dim3((WBUF_ALL + 255) / 256), dim3(256), 0, S(stream), params,
                        wbuf);
     return hip_check(hipGetLastError(), "lbc_pack");
 }

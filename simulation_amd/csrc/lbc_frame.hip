@@ -353,6 +353,10 @@ __global__ void __launch_bounds__(NT) frame_kernel(LbcFrameParams P)
     }
     LaneW LW;
     load_lane_w(LW, P.w, lane);
+    // a network whose hidden layers' output layer is zero adds exactly
+    // nothing (y = 0, so the odd part 0.5 (y - y') is +0, as computed): skip
+    // those layers
+    const bool mlp = P.w[W_MLP] != 0;
     // cell loops without divisions: thread tid starts at cell tid and steps
     // by NT = DR rows + DC columns
     const int DR = NT / RW, DC = NT % RW;
@@ -405,6 +409,8 @@ __global__ void __launch_bounds__(NT) frame_kernel(LbcFrameParams P)
                     lin[o] = a;
                 }
             }
+            f32x4 y = {0.0f, 0.0f, 0.0f, 0.0f};
+            if (mlp) {
             // layer 1 input: column c < 8 the features, c >= 8 their negation
             const float sc = c < 8 ? inv : -inv;
             bf16x8 xb[4];
@@ -438,9 +444,9 @@ __global__ void __launch_bounds__(NT) frame_kernel(LbcFrameParams P)
             }
             bf16x8 a2[2];
             act_pack(z, S.b2, g, a2);
-            f32x4 y = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
             for (int ks = 0; ks < 2; ++ks) y = mfma(LW.w3[ks], a2[ks], y);
+            }   // mlp
             // outputs v = 0..2 of column c in lanes g = 0: odd part from the
             // column pair (c, c + 8)
 #pragma unroll

@@ -52,6 +52,10 @@ constexpr int FRM_OPS = 26;     // per lane: W1 16 (tile mt, K step ks), W2 8, W
 constexpr int W_W1 = 0, W_W2 = W_W1 + HID * K1, W_W3 = W_W2 + HID * HID,
               W_W2T = W_W3 + 16 * HID, W_W3T = W_W2T + HID * HID, W_FRM = W_W3T + HID * 32,
               WBUF = W_FRM + 64 * FRM_OPS * 8;
+// After the operands: one flag, 1 if the hidden layers' output (W3, b3) has
+// a non-zero parameter, 0 if the network reduces to its linear path (the
+// time-stepping kernel then skips the hidden layers); padded to 8 elements.
+constexpr int W_MLP = WBUF, WBUF_ALL = WBUF + 8;
 
 // Transposed network (positions as MFMA columns, hidden units as rows): lane
 // (c = lane & 15, g = lane >> 4) holds, as A operands,
@@ -212,7 +216,7 @@ __device__ __forceinline__ float odd_out(float yk)
 }
 
 struct Net {
-    const uint16_t *w;      // bf16 operand buffer (WBUF elements)
+    const uint16_t *w;      // bf16 operand buffer (WBUF_ALL elements)
     const float *p;         // FP32 parameters (biases and L read from here)
 };
 

@@ -240,6 +240,187 @@ class LearnedTMSolver:
         return self.hist.view(-1, 4, NSAMP).cpu().numpy()
 
 
+# ------------------------------------------------------------ PML fallback
+
+@dataclass
+class FallbackPolicy:
+    """When a guarded learned-boundary run hands over to a PML.
+
+    Every `every` steps the FP64 energy of the grid (fd2d_energy) is taken.
+    The run hands over when that energy is not finite, or -- from step
+    `quiet_after` on (when the sources have stopped injecting), once the
+    largest energy seen exceeds `floor` -- when it exceeds `growth` times the
+    largest energy seen at earlier checks: an absorbing boundary only removes
+    energy, so growth past that factor means the boundary went unstable.
+    `force_at`: hand over at the first check at or after that step whatever
+    the energy (None: never).  The PML it hands over to is the reference's
+    Sullivan layer (fd2d/pycuda/fd2d_3_2.py:126-135) of `npml` cells, placed
+    outside the window as in the npml=32 comparison form."""
+    every: int = 64
+    growth: float = 4.0
+    quiet_after: int = 0
+    floor: float = 0.0
+    npml: int = 32
+    force_at: int | None = None
+
+
+def fallback_reason(policy: FallbackPolicy, energy: float, emax: float, t: int) -> str | None:
+    """Why a check at step t with this energy (largest earlier: emax) hands
+    over, or None to carry on."""
+    if policy.force_at is not None and t >= policy.force_at:
+        return f"handover forced at step {t}"
+    if not math.isfinite(energy):
+        return f"energy {energy} at step {t}"
+    if t >= policy.quiet_after and emax > policy.floor and energy > policy.growth * emax:
+        return f"energy {energy:.3e} at step {t} above {policy.growth} x {emax:.3e}"
+    return None
+
+
+class GuardedLearnedSolver:
+    """A learned-boundary run (LearnedTMSolver on an n x m window) with the
+    plain PML kept as fallback.
+
+    run() advances in chunks of policy.every steps; before each chunk the
+    state (dz, iz, ez, hx, hy, the boundary-sample history, probe traces) is
+    copied on the device, after it the energy is checked (fallback_reason).
+    On a handover the state before the failing chunk is restored and copied
+    into the centre of an (n + 2 npml) x (m + 2 npml) grid whose outer npml
+    cells are a Sullivan PML (pmlparam), with the window's medium, free space
+    around it, zero fields outside the window and zero PML accumulators; the
+    run continues there (a TMSolver), from the same step, with the sources
+    and probes shifted by npml.  `events` records each handover (step and
+    reason); `pml` is the PML solver once handed over."""
+
+    def __init__(self, nx, ny, net: Net, md, sources=(), probes=(), nsteps_max=None,
+                 policy: FallbackPolicy | None = None, device=None, **kw):
+        self.policy = policy or FallbackPolicy()
+        self.learned = LearnedTMSolver(nx, ny, net, md, sources, probes, nsteps_max, device=device,
+                                       **kw)
+        self.nx, self.ny = int(nx), int(ny)
+        self.md = md
+        self.kw = kw
+        self.pml = None
+        self.events = []
+        self.emax = 0.0
+        self.energies = []
+
+    @property
+    def t(self) -> int:
+        return self.pml.t if self.pml is not None else self.learned.t
+
+    @property
+    def active(self):
+        """The solver currently advancing: the learned one, or the PML one."""
+        return self.pml if self.pml is not None else self.learned
+
+    def _snapshot(self):
+        f = self.learned.tm
+        keep = {k: getattr(f, k).clone() for k in ("dz", "iz", "ez", "hx", "hy")
+                if getattr(f, k) is not None}
+        keep["hist"] = self.learned.hist.clone()
+        keep["hist2"] = self.learned.hist2.clone()
+        if f.probe_out is not None:
+            keep["probe_out"] = f.probe_out.clone()
+        return f.t, keep
+
+    def _restore(self, snap):
+        t, keep = snap
+        f = self.learned.tm
+        for k, v in keep.items():
+            if k in ("hist", "hist2"):
+                getattr(self.learned, k).copy_(v)
+            else:
+                getattr(f, k).copy_(v)
+        f.t = t
+
+    def handover_state(self):
+        """The learned run's current state as the PML grid's initial state:
+        (dict of (N, M) arrays dz iz ez hx hy ihx ihy, step, pad)."""
+        f = self.learned.tm
+        npml = self.policy.npml
+        N, M = self.nx + 2 * npml, self.ny + 2 * npml
+        dev = f.device
+        out = {}
+        for k in ("dz", "iz", "ez", "hx", "hy", "ihx", "ihy"):
+            src = getattr(f, k, None)
+            if k == "iz" and src is None:
+                out[k] = None
+                continue
+            big = torch.zeros((N, M), dtype=torch.float32, device=dev)
+            if src is not None:
+                big[npml:npml + self.nx, npml:npml + self.ny].copy_(src)
+            out[k] = big
+        return out, f.t, npml
+
+    def _hand_over(self, reason: str) -> None:
+        f = self.learned.tm
+        state, t, npml = self.handover_state()
+        N, M = self.nx + 2 * npml, self.ny + 2 * npml
+        dev = f.device
+        pml = fd2d.new_pmlayer(N, M, dev)
+        fd2d.pmlparam(N, M, npml, pml)
+        naz = torch.ones((N, M), dtype=torch.float32, device=dev)
+        naz[npml:npml + self.nx, npml:npml + self.ny].copy_(self.md.naz)
+        nbz = None
+        if self.md.nbz is not None:
+            nbz = torch.zeros((N, M), dtype=torch.float32, device=dev)
+            nbz[npml:npml + self.nx, npml:npml + self.ny].copy_(self.md.nbz)
+        srcs = []
+        if f.sources:
+            waves = f.src_wave.cpu().numpy()
+            srcs = [(i + npml, j + npml, waves[k]) for k, (i, j) in enumerate(f.sources)]
+        probes = [(i + npml, j + npml) for i, j in f.probes]
+        s = fd2d.TMSolver(N, M, pml, fd2d.medium(naz, nbz), srcs, probes,
+                          f.probe_len or None, device=dev, **self.kw)
+        for k, v in state.items():
+            if v is not None and getattr(s, k) is not None:
+                getattr(s, k).copy_(v)
+        if f.probe_out is not None:
+            s.probe_out[:, :t].copy_(f.probe_out[:, :t])
+        s.t = t
+        self.pml = s
+        self.events.append({"step": t, "reason": reason, "npml": npml})
+
+    def check(self) -> str | None:
+        """Energy check of the learned run at its current step."""
+        f = self.learned.tm
+        e = f.energy()
+        self.energies.append((f.t, e))
+        reason = fallback_reason(self.policy, e, self.emax, f.t)
+        if math.isfinite(e):
+            self.emax = max(self.emax, e)
+        return reason
+
+    def run(self, nsteps: int) -> None:
+        left = int(nsteps)
+        while left > 0 and self.pml is None:
+            n = min(self.policy.every, left)
+            if left - n == 1:
+                n = left                  # a learned run advances at least two steps
+            snap = self._snapshot()
+            self.learned.run(n)
+            reason = self.check()
+            if reason is not None:
+                self._restore(snap)
+                self._hand_over(reason)
+                break
+            left -= n
+        if self.pml is not None:
+            # whatever the learned run did not finish (the failing chunk
+            # included) runs on the PML form
+            self.pml.run(left)
+
+    def window(self, k: str) -> torch.Tensor:
+        """Field k on the n x m window, whichever form holds it."""
+        if self.pml is None:
+            return getattr(self.learned.tm, k)
+        p = self.policy.npml
+        return getattr(self.pml, k)[p:p + self.nx, p:p + self.ny]
+
+    def probe_traces(self):
+        return self.active.probe_traces()
+
+
 # ----------------------------------------------------------------- training
 
 @dataclass

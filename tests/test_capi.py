@@ -135,10 +135,12 @@ def test_lbc_work_and_operand_buffer_sizes():
     """The frame tables (one set per depth 1..8) need a work buffer that grows
     with the perimeter; grids below 64 x 64 are rejected (size 0).  The bf16
     buffer holds the row-major operands (W1 128-wide, W2, W3 16 rows, W2^T,
-    W3^T 32-wide) and 64 lanes x 26 per-lane operands of 8 values."""
+    W3^T 32-wide), 64 lanes x 26 per-lane operands of 8 values and the flag
+    that says whether the hidden layers' output layer is non-zero."""
     from simulation_amd import _lib
     L = _lib.lib()
     assert L.lbc_work_bytes(32, 32) == 0
     a, b = L.lbc_work_bytes(512, 512), L.lbc_work_bytes(2048, 2048)
     assert 0 < a < b < 8 * a
-    assert L.lbc_wbuf_elems() == 64 * 128 + 64 * 64 + 16 * 64 + 64 * 64 + 64 * 32 + 64 * 26 * 8
+    # operands, then the hidden-output flag padded to 8 elements
+    assert L.lbc_wbuf_elems() == 64 * 128 + 64 * 64 + 16 * 64 + 64 * 64 + 64 * 32 + 64 * 26 * 8 + 8

@@ -55,7 +55,8 @@ def test_fd2d_step_kernels_bit_exact(gpu, name):
     _check(c, s.state(), s.probe_traces())
 
 
-FUSED = [(1, 1, 0), (2, 2, 0), (3, 1, 7), (5, 2, 16), (8, 2, 0), (8, 1, 32), (4, 2, 200)]
+FUSED = [(1, 1, 0), (2, 2, 0), (3, 1, 7), (5, 2, 16), (8, 2, 0), (8, 1, 32), (4, 2, 200),
+         (9, 1, 13), (10, 1, 0), (11, 2, 21), (12, 1, 40)]
 
 
 @pytest.mark.parametrize("name", F32_CASES)

@@ -257,18 +257,28 @@ def _frame_operands(p):
 
 def test_frame_operand_layout(gpu):
     """The bf16 buffer's per-lane operands: written by lbc_pack, kept current
-    by lbc_adam (the whole buffer equals a fresh pack after a step)."""
+    by lbc_adam (the whole buffer equals a fresh pack after a step); the
+    flag after them (8 elements) says whether W3/b3 hold a non-zero value."""
     from simulation_amd import learned
     p, X, T = _problem(9, 256)
     net = learned.Net(p, gpu)
     w = net.wbuf.cpu().numpy().view(np.uint16)
     nfrm = 64 * 26 * 8
-    assert w.size == net.wbuf.numel() and w.size >= nfrm
-    got = (w[-nfrm:].astype(np.uint32) << 16).view(np.float32)
+    assert w.size == net.wbuf.numel() and w.size >= nfrm + 8
+    got = (w[-nfrm - 8:-8].astype(np.uint32) << 16).view(np.float32)
     assert np.array_equal(got, _frame_operands(p))
+    assert w[-8] == 1 and not w[-7:].any()
     src, offs, ts = _rows(X, gpu)
     net.train_step(src, offs, ts, 1e-2, tgt=torch.from_numpy(T).to(gpu), nblocks=3)
     fresh = learned.Net(net.numpy(), gpu)
     assert torch.equal(net.wbuf, fresh.wbuf)
-    got = (net.wbuf.cpu().numpy().view(np.uint16)[-nfrm:].astype(np.uint32) << 16).view(np.float32)
+    got = (net.wbuf.cpu().numpy().view(np.uint16)[-nfrm - 8:-8].astype(np.uint32) << 16).view(np.float32)
     assert np.array_equal(got, _frame_operands(net.numpy()))
+    # a zero output layer: the flag is clear, and an Adam step that moves W3
+    # off zero raises it
+    q = p.copy()
+    q[learned.P_W3:learned.P_L] = 0.0
+    z = learned.Net(q, gpu)
+    assert z.wbuf.cpu().numpy().view(np.uint16)[-8] == 0
+    z.train_step(src, offs, ts, 1e-2, tgt=torch.from_numpy(T).to(gpu), nblocks=3)
+    assert z.wbuf.cpu().numpy().view(np.uint16)[-8] == 1
