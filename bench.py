@@ -676,10 +676,10 @@ def main():
     fixed = a.depth or a.rows or a.cols or a.form != "auto"
     if not (a.no_tune or fixed):
         from simulation_amd import autotune
-        # timed at the run's own length (up to 48 steps), so the chosen shape
-        # is the fastest for this plan and its kernel has run before the
-        # timed region
-        tuned = autotune.tune(s, steps=min(K, 48), use_cache=False)
+        # timed at the run's own length (up to 80 steps, whole launches of
+        # depth 8 and 10), so the chosen shape is the fastest for this plan
+        # and its kernel has run before the timed region
+        tuned = autotune.tune(s, steps=min(K, 80), use_cache=False)
         a.form = {0: "auto", 1: "strip", 2: "multiwave"}[tuned.get("form", 0)]
     plan = s.plan(K)
     s.run(W)

@@ -82,7 +82,7 @@ def _time(fn, reps: int = 2) -> float:
     return best
 
 
-def tune(solver: fd2d.TMSolver, steps: int = 48, use_cache: bool = True, configs=None,
+def tune(solver: fd2d.TMSolver, steps: int = 80, use_cache: bool = True, configs=None,
          verbose: bool = False) -> dict:
     """Time (depth, rows, cols[, form]) candidates on this solver's grid and
     set the fastest on it.  The solver's state, probe traces and step are
