@@ -65,6 +65,8 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-learned", action="store_true",
+                    help="large: skip the learned-boundary form (with its PML fallback)")
     ap.add_argument("--workload", choices=("tm", "large", "batch", "learned", "tfsf", "dform"),
                     default="tm",
                     help="tm: the N x N single-cylinder configuration (default, N=8192); "
@@ -76,7 +78,11 @@ def parse():
     ap.add_argument("--weights", default="",
                     help="learned: network parameters (.npz from simulation_amd.learned); "
                          "default: the committed weights, else the zero network")
-    ap.add_argument("--graph", action="store_true", help="batch: time one captured graph launch")
+    ap.add_argument("--waves", type=int, default=0,
+                    help="batch: waves of the batch launch (0: as many as stay resident, "
+                         "a persistent sweep of the unit list; -1: one wave per unit)")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="batch: time the eager launches instead of one captured graph launch")
     ap.add_argument("--no-tune", action="store_true",
                     help="tm/large: skip the autotuner (default: it chooses depth/rows/cols/form "
                          "before the warm-up, unless --depth/--rows/--cols/--form fix them)")
@@ -247,33 +253,80 @@ def record_traffic(key, kernel, fetch_csv, write_csv, cfetch_csv, cwrite_csv) ->
     return rec
 
 
+def bench_large_learned(cfg, W: int, K: int, dev) -> dict:
+    """The large configuration with the learned boundary in place of the PML,
+    the plain PML kept as fallback (learned.GuardedLearnedSolver): the same
+    n x n grid, cylinders, sources and probes, no PML cells; the shipped
+    network (or the zero network, the reference's two-step boundary, if no
+    weights are committed).  Energy checked every 256 steps; growth is
+    judged once the sources have stopped injecting (step 200).  Returns the
+    window rate, the launch timing and the hand-over record."""
+    import torch
+    from simulation_amd import fd1d, fd2d, learned as LB
+    n = cfg.nx
+    net = (LB.load(LB.WEIGHTS, dev) if os.path.exists(LB.WEIGHTS)
+           else LB.Net(np.zeros(LB.NPARAM, np.float32), dev))
+    md = fd2d.dielectric(n, n, 0, 0, fd2d.time_step(), 1.0, 0.0, dev, cylinders=cfg.cylinders)
+    srcs = [(i, j, fd1d.waveform("gaussian", W + K, 0, np.float32, center=t0, spread=sp))
+            for i, j, t0, sp in cfg.sources]
+    pol = LB.FallbackPolicy(every=256, quiet_after=200)
+    g = LB.GuardedLearnedSolver(n, n, net, md, srcs, cfg.probes, nsteps_max=W + K, policy=pol,
+                                device=dev)
+    g.run(W)
+    torch.cuda.synchronize()
+    st = torch.cuda.current_stream()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record(st)
+    g.run(K)
+    e1.record(st)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    out = {"workload": f"{n}x{n} window, no PML cells, learned boundary "
+                       f"({'shipped network' if os.path.exists(LB.WEIGHTS) else 'zero network'}), "
+                       f"npml={pol.npml} PML fallback, energy check every {pol.every} steps",
+           "value": round(n * n * K / wall / 1e6, 1), "unit": "Mcell-updates/s (window)",
+           "ms_per_step": round(wall * 1e3 / K, 4), "events_ms": round(e0.elapsed_time(e1), 3),
+           "handovers": g.events, "energy_checks": len(g.energies),
+           "last_energy": g.energies[-1][1] if g.energies else None}
+    del g
+    torch.cuda.empty_cache()
+    return out
+
+
 def bench_batch(a):
     """Batched workload: 256 square grids, sides uniform in [128, 1024] (seed
     0), 1-5 lossy cylinders each, npml=8, hard Gaussian point source, one probe
-    per grid; `steps` fused steps per grid (default 3000), timed as one run of
-    fd2d_batch_run launches (or one captured graph launch with --graph)."""
+    per grid; `steps` fused steps per grid (default 3000), timed as one
+    captured HIP graph launch (--no-graph: the eager fd2d_batch_run
+    launches).  Each launch of the persistent kernel advances every grid by
+    up to `depth` steps; its minimum HBM bytes are 60 B per cell of the batch
+    (every state array read and written once, naz and nbz read once)."""
     import torch
     from simulation_amd.batch import BatchSolver, c3_specs
 
     dev = torch.device("cuda", 0)
-    if a.graph:
+    graph = not a.no_graph
+    if graph:
         # graph capture needs a stream other than the null stream
         torch.cuda.set_stream(torch.cuda.Stream())
     K = a.steps if a.steps != 2000 else 3000
     specs = c3_specs(0)
-    b = BatchSolver(specs, K + a.warmup, depth=a.depth, rows=a.rows, cols=a.cols, device=dev)
+    b = BatchSolver(specs, K + a.warmup, depth=a.depth, rows=a.rows, cols=a.cols,
+                    waves=(1 << 30) if a.waves < 0 else a.waves, device=dev)
     if a.warmup:
         b.run(a.warmup if a.warmup > 1 else 2)
     torch.cuda.synchronize()
-    if a.graph:
-        h = b.graph(K)
+    if graph:
+        b.graph(K)
         torch.cuda.synchronize()
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
     stream = torch.cuda.current_stream()
     t0 = time.perf_counter()
     e0.record(stream)
-    if a.graph:
+    if graph:
         b.run_graph(K)
     else:
         b.run(K)
@@ -282,11 +335,17 @@ def bench_batch(a):
     wall = time.perf_counter() - t0
     ev_ms = e0.elapsed_time(e1)
     cells = b.cells
-    L = (K + b.desc.depth - 1) // b.desc.depth
+    D = b.desc.depth
+    L = (K + D - 1) // D
     L += L % 2
+    depths = launch_depths(K, L)
     launch_ms = ev_ms / L
-    bytes_per_launch = B_ALG * cells * K / L
-    achieved = bytes_per_launch / (launch_ms * 1e-3) / 1e9
+    min_bytes = B_ALG * cells
+    achieved = min_bytes / (launch_ms * 1e-3) / 1e9
+    effective = B_ALG * cells * K / (ev_ms * 1e-3) / 1e9
+    # a launch reads and writes every state array once whatever its number of
+    # fused steps, so launches of 7 and 8 steps share the record of depth 8
+    key = f"fd2d_batch_c3_seed0_d{D}_r{b.desc.rows or 64}_c{b.desc.cols}"
     rec = {
         "metric": METRIC, "value": round(cells * K / wall / 1e6, 1), "unit": "Mcell-updates/s",
         "n_gpus": 1, "steps": K, "warmup": a.warmup, "ms_per_step": round(wall * 1e3 / K, 4),
@@ -294,14 +353,18 @@ def bench_batch(a):
         "data": "synthetic (seeded grid sizes, cylinders and sources; fields start at zero)",
         "config": {"workload": "256 batched 2-D TM grids, sides U[128,1024] (seed 0), 1-5 lossy "
                                "cylinders, npml=8, hard Gaussian source, persistent launches"
-                               + (", one captured HIP graph" if a.graph else ""),
-                   "cells": cells, "grids": len(specs), "depth": b.desc.depth,
-                   "rows": b.desc.rows, "cols": b.desc.cols, "waves": b.desc.waves,
-                   "units": b.desc.nunits, "launches": L},
+                               + (", one captured HIP graph" if graph else ", eager launches"),
+                   "cells": cells, "grids": len(specs), "steps_per_launch": sorted(set(depths)),
+                   "rows": b.desc.rows or 64,
+                   "cols": b.desc.cols, "waves": b.desc.waves, "units": b.desc.nunits,
+                   "launches": L},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4),
+                     "traffic": traffic_for(key), "traffic_key": key,
                      "kernel": "tm_tb_batch_kernel", "avg_launch_ms": round(launch_ms, 4),
-                     "alg_bytes_per_launch": int(bytes_per_launch)},
+                     "min_bytes_per_launch": int(min_bytes),
+                     "effective_GBps_per_step": round(effective, 1),
+                     "effective_frac": round(effective / HBM_PEAK, 4)},
         "cpu_baseline": None,
     }
     print(json.dumps(rec), flush=True)
@@ -745,6 +808,10 @@ def main():
         },
         "cpu_baseline": None,
     }
+    if a.workload == "large" and not a.no_learned:
+        del s
+        torch.cuda.empty_cache()
+        rec["config"]["learned_boundary"] = bench_large_learned(cfg, W, K, dev)
     if not a.no_cpu:
         try:
             rec["cpu_baseline"] = cpu_baseline(n, a.cpu_seconds)

@@ -306,7 +306,8 @@ typedef struct fd2d_batch_desc {
     const fd2d_grid *grids_dev;   /* device copy of the grid records */
     const int *units_dev;         /* device work list, 4 ints per unit */
     int nunits;
-    int depth, rows, cols;        /* fixed when the work list is made */
+    int depth, rows, cols;        /* fixed when the work list is made (rows 0:
+                                     the default, see fd2d_batch_units) */
     int waves;                    /* workgroups of the persistent launch */
     int vec;                      /* 1: every grid allows vector loads */
     int wave_len;                 /* smallest wave_len over grids with a source
@@ -316,7 +317,9 @@ typedef struct fd2d_batch_desc {
 } fd2d_batch_desc;
 
 /* Number of work units (and so ints/4 of units_dev) a batch needs for the
- * given depth (0 = 8), rows per block (0 = 32) and columns per lane (0 = 1). */
+ * given depth (0 = 8), rows per block (0 = 64, or the whole grid when it has
+ * fewer rows) and columns per lane (0 = 1).  A unit is 4 ints: grid, strip,
+ * first row, one past the last row.                                       */
 int fd2d_batch_units(const fd2d_grid *grids, int ngrids, int depth, int rows, int cols,
                      int *nunits);
 

@@ -40,10 +40,13 @@ def candidates(nx: int):
 
 
 def batch_candidates():
-    for depth in (8, 6, 4):
-        for rows in (16, 32, 64):
-            for cols in (1, 2):
-                yield depth, rows, cols, 0
+    """(depth, rows, cols, waves) shapes tried for a batch: depths 8 and 10
+    with 48-80 rows (on MI355X the 256-grid batch of sides 128-1024 peaks
+    there, all within a few per cent), and two columns per lane at depth 8."""
+    for depth in (8, 10):
+        for rows in (48, 64, 80):
+            yield depth, rows, 1, 0
+    yield 8, 64, 2, 0
 
 
 def _device():

@@ -104,6 +104,14 @@ using gfloat = const __attribute__((address_space(1))) float;
 typedef float vf2 __attribute__((ext_vector_type(2)));
 using gfloat2 = const __attribute__((address_space(1))) vf2;
 __device__ __forceinline__ float ldg(const float *p) { return *(gfloat *)p; }
+// Element i of a read-only array at an index that is the same in every lane
+// (a row profile value): a scalar load through the constant address space,
+// so it costs no vector memory instruction and no wait on the vector counter.
+using cfloat = const __attribute__((address_space(4))) float;
+__device__ __forceinline__ float ldu(const float *base, int i)
+{
+    return ((cfloat *)base)[__builtin_amdgcn_readfirstlane(i)];
+}
 __device__ __forceinline__ gfloat *at(const float *base, unsigned off)
 {
     const unsigned bytes = off << 2;                  // < 2^32: zero-extended 32-bit offset
@@ -183,8 +191,8 @@ __device__ __forceinline__ void st_rb(__amdgpu_buffer_rsrc_t rs, unsigned row, i
     }
 }
 
-template <int C, bool LOSSY, bool PML, bool VEC, bool CLAMP>
-__device__ __forceinline__ void load_row(const TbParams &P, int p, int j0, HRow<C> &H, DRow<C> &D)
+template <int C, bool LOSSY, bool PML, bool VEC, bool CLAMP, class PR>
+__device__ __forceinline__ void load_row(const PR &P, int p, int j0, HRow<C> &H, DRow<C> &D)
 {
     const int pc = min(max(p, 0), P.nx - 1);     // rows outside the grid: any in-bounds row
     const unsigned row = (unsigned)pc * (unsigned)P.ny;
@@ -213,8 +221,8 @@ __device__ __forceinline__ void load_row(const TbParams &P, int p, int j0, HRow<
 template <int C>
 struct ColCoef { float gy2[C], gy3[C], fy1[C], fy2[C], fy3[C]; };
 
-template <int C, bool PML, bool BCOL>
-__device__ __forceinline__ void col_coef(const TbParams &P, int j0, ColCoef<C> &cc)
+template <int C, bool PML, bool BCOL, class PR>
+__device__ __forceinline__ void col_coef(const PR &P, int j0, ColCoef<C> &cc)
 {
 #pragma unroll
     for (int c = 0; c < C; ++c) {
@@ -236,8 +244,8 @@ __device__ __forceinline__ void col_coef(const TbParams &P, int j0, ColCoef<C> &
 // same for the strip's columns; SP: a source or probe lies in the rows and
 // columns the wave computes.
 template <int K, int C, bool LOSSY, bool PML, bool VEC, bool BROW, bool BCOL, bool SP, int NF,
-          bool BS, bool PRO = false>
-__device__ __forceinline__ void iterate(const TbParams &P, const Geom &g, const ColCoef<C> &cc,
+          bool BS, bool PRO = false, class PR>
+__device__ __forceinline__ void iterate(const PR &P, const Geom &g, const ColCoef<C> &cc,
                                         int p, const HRow<C> &Hload, const DRow<C> &Dload,
                                         const Carry<C> (&Si)[K], Carry<C> (&So)[K],
                                         const FRow<C, NF> &Fload, const FRow<C, NF> (&SFi)[K],
@@ -280,8 +288,8 @@ __device__ __forceinline__ void iterate(const TbParams &P, const Geom &g, const 
         float gx2r = 1.f, gx3r = 1.f, fx1q = 0.f, fx2q = 1.f, fx3q = 1.f;
         if (PML && BROW) {
             const int rc = min(max(r, 0), nx - 1), qc = min(max(q, 0), nx - 1);
-            gx2r = ldg(P.pml.gx2 + rc); gx3r = ldg(P.pml.gx3 + rc);
-            fx1q = ldg(P.pml.fx1 + qc); fx2q = ldg(P.pml.fx2 + qc); fx3q = ldg(P.pml.fx3 + qc);
+            gx2r = ldu(P.pml.gx2, rc); gx3r = ldu(P.pml.gx3, rc);
+            fx1q = ldu(P.pml.fx1, qc); fx2q = ldu(P.pml.fx2, qc); fx3q = ldu(P.pml.fx3, qc);
         }
         // ------------------------------------------------ E update, row r
         const float hxl = from_left(Hin.hx[C - 1]);
@@ -471,8 +479,8 @@ __device__ __forceinline__ void iterate(const TbParams &P, const Geom &g, const 
 
 // DFT accumulators of row p (planes f < P.nf; rows outside the grid read an
 // in-bounds row, never stored)
-template <int C, int NF, bool VEC, bool CLAMP>
-__device__ __forceinline__ void load_ft(const TbParams &P, int p, int j0, FRow<C, NF> &F)
+template <int C, int NF, bool VEC, bool CLAMP, class PR>
+__device__ __forceinline__ void load_ft(const PR &P, int p, int j0, FRow<C, NF> &F)
 {
     const int pc = min(max(p, 0), P.nx - 1);
     const unsigned row = (unsigned)pc * (unsigned)P.ny;
@@ -490,8 +498,8 @@ __device__ __forceinline__ void load_ft(const TbParams &P, int p, int j0, FRow<C
 }
 
 template <int K, int C, bool LOSSY, bool PML, bool VEC, bool BROW, bool BCOL, bool SP, int NF = 0,
-          bool BS = false, bool PP = false>
-__device__ __forceinline__ void sweep(const TbParams &P, const Geom &g)
+          bool BS = false, bool PP = false, class PR>
+__device__ __forceinline__ void sweep(const PR &P, const Geom &g)
 {
     ColCoef<C> cc;
     col_coef<C, PML, BCOL>(P, g.j0, cc);
@@ -819,13 +827,45 @@ int launch_part(const TbParams &p, bool lossy, bool pml, int part, hipStream_t s
 }
 
 // ----------------------------------------------------------- batched grids
+// A unit's parameters.  The sweep reads the same fields as from TbParams, but
+// a batch grid has at most one source and one probe, so the lists are single
+// values (One ignores the index) and the plane-wave and DFT terms are absent
+// at compile time: nothing in the record is indexed at run time, and it stays
+// in registers.  (A TbParams built in the kernel would live in scratch memory,
+// since its source and probe lists are indexed by a loop counter.)
+struct One {
+    int v;
+    __device__ int operator[](int) const { return v; }
+};
+
+struct BatchParams {
+    int nx, ny;
+    const float *dz_i, *iz_i, *hx_i, *hy_i, *ihx_i, *ihy_i;
+    float *dz_o, *iz_o, *hx_o, *hy_o, *ihx_o, *ihy_o, *ez_o;
+    const float *naz, *nbz;
+    PmlPtrs pml;
+    int t_first;
+    int nsrc;
+    One src_i, src_j;
+    const float *src_wave;
+    int wave_len;
+    int nprobe;
+    One probe_i, probe_j;
+    float *probe_out;
+    int probe_len;
+    static constexpr int skip_w = 0;
+    static constexpr int tf_npml = 0, inc_t0 = 0, inc_stride = 0;
+    static constexpr const float *inc = nullptr;
+    static constexpr int nf = 0;
+};
+
 // A persistent kernel over a work list of (grid, strip, row block) units of
 // many independent grids (fd2d_grid records in device memory); each unit is
 // swept exactly as the single-grid kernels sweep it.  Lossy PML form only.
 template <int K, int C, bool VEC>
 __global__ void __launch_bounds__(64)
 tm_tb_batch_kernel(const fd2d_grid *__restrict__ grids, const int *__restrict__ units, int nunits,
-                   int parity, int t_first, int write_ez, int halo, int rows)
+                   int parity, int t_first, int write_ez, int halo)
 {
     constexpr int S = 64 * C;
     // XCD-aware slots: wave w (on XCD w % 8) takes the units slot, slot +
@@ -835,15 +875,11 @@ tm_tb_batch_kernel(const fd2d_grid *__restrict__ grids, const int *__restrict__ 
     const unsigned nw = gridDim.x, xcd = blockIdx.x % 8u, q = nw / 8u, rem = nw % 8u;
     const unsigned slot = xcd * q + min(xcd, rem) + blockIdx.x / 8u;
     for (int u = (int)slot; u < nunits; u += (int)nw) {
-        const int gi = units[4 * u], sx = units[4 * u + 1], by = units[4 * u + 2];
+        // unit: grid, strip, rows [i0, i1) (each grid has its own row blocks)
+        const int gi = units[4 * u], sx = units[4 * u + 1];
+        const int ui0 = units[4 * u + 2], ui1 = units[4 * u + 3];
         const fd2d_grid &G = grids[gi];
-        TbParams P;
-        P.tf_npml = 0;                        // no plane wave, no DFT in a batch
-        P.nf = 0;
-        P.inc = nullptr;
-        P.n_rb = -1;                          // (tile lists: single-grid launches only)
-        P.n_sx = P.n_t = 0;
-        P.nb = P.nb_pad = 0;
+        BatchParams P;
         P.nx = G.nx;
         P.ny = G.ny;
         const bool swap = parity != 0;        // odd launches read the partner set
@@ -858,41 +894,38 @@ tm_tb_batch_kernel(const fd2d_grid *__restrict__ grids, const int *__restrict__ 
         P.nbz = G.md.nbz;
         P.pml = PmlPtrs{G.pml.fx1, G.pml.fx2, G.pml.fx3, G.pml.fy1, G.pml.fy2, G.pml.fy3,
                         G.pml.gx2, G.pml.gx3, G.pml.gy2, G.pml.gy3};
-        P.halo = halo;
-        P.rows = rows;
         const int *br = G.band_rows, *bc = G.band_cols;
-        P.rlo = br[1] > br[0] ? max(br[0], 1) : 0;
-        P.rhi = br[1] > br[0] ? min(br[1], G.nx - 1) : 0;
-        P.clo = bc[1] > bc[0] ? max(bc[0], 1) : 0;
-        P.chi = bc[1] > bc[0] ? min(bc[1], G.ny - 1) : 0;
+        const int rlo = br[1] > br[0] ? max(br[0], 1) : 0;
+        const int rhi = br[1] > br[0] ? min(br[1], G.nx - 1) : 0;
+        const int clo = bc[1] > bc[0] ? max(bc[0], 1) : 0;
+        const int chi = bc[1] > bc[0] ? min(bc[1], G.ny - 1) : 0;
         P.t_first = t_first;
         P.nsrc = G.src_i >= 0 ? 1 : 0;
-        P.src_i[0] = G.src_i;
-        P.src_j[0] = G.src_j;
+        P.src_i.v = G.src_i;
+        P.src_j.v = G.src_j;
         P.src_wave = G.src_wave;
         P.wave_len = G.wave_len;
         P.nprobe = G.probe_i >= 0 ? 1 : 0;
-        P.probe_i[0] = G.probe_i;
-        P.probe_j[0] = G.probe_j;
+        P.probe_i.v = G.probe_i;
+        P.probe_j.v = G.probe_j;
         P.probe_out = G.probe_out;
         P.probe_len = G.probe_len;
-        P.skip_w = 0;
         const int js = sx * (S - 2 * halo) - halo;
         Geom g;
         g.j0 = js + (int)threadIdx.x * C;
-        g.i0 = by * rows;
-        g.i1 = min(g.i0 + rows, P.nx);
+        g.i0 = ui0;
+        g.i1 = ui1;
         g.vj0 = js + halo;
         g.vj1 = min(js + S - halo, P.ny);
-        const bool brow = !(g.i0 - 2 * K >= P.rlo && g.i1 - 1 + K < P.rhi);
-        const bool bcol = !(js >= P.clo && js + S <= P.chi);
+        const bool brow = !(g.i0 - 2 * K >= rlo && g.i1 - 1 + K < rhi);
+        const bool bcol = !(js >= clo && js + S <= chi);
         bool sp = false;
         if (P.nsrc)
-            sp |= P.src_i[0] >= g.i0 - 2 * K && P.src_i[0] <= g.i1 + K && P.src_j[0] >= js &&
-                  P.src_j[0] < js + S;
+            sp |= P.src_i.v >= g.i0 - 2 * K && P.src_i.v <= g.i1 + K && P.src_j.v >= js &&
+                  P.src_j.v < js + S;
         if (P.nprobe)
-            sp |= P.probe_i[0] >= g.i0 && P.probe_i[0] < g.i1 && P.probe_j[0] >= js &&
-                  P.probe_j[0] < js + S;
+            sp |= P.probe_i.v >= g.i0 && P.probe_i.v < g.i1 && P.probe_j.v >= js &&
+                  P.probe_j.v < js + S;
         if (!brow && !bcol && !sp) sweep<K, C, true, true, VEC, false, false, false, 0, true>(P, g);
         else sweep<K, C, true, true, VEC, true, true, true, 0, true>(P, g);
     }
@@ -900,10 +933,10 @@ tm_tb_batch_kernel(const fd2d_grid *__restrict__ grids, const int *__restrict__ 
 
 template <int K, int C, bool VEC>
 int launch_batch_kv(const fd2d_grid *grids, const int *units, int nunits, int waves, int parity,
-                    int t_first, int write_ez, int halo, int rows, hipStream_t s)
+                    int t_first, int write_ez, int halo, hipStream_t s)
 {
     hipLaunchKernelGGL((tm_tb_batch_kernel<K, C, VEC>), dim3(waves), dim3(64), 0, s, grids, units,
-                       nunits, parity, t_first, write_ez, halo, rows);
+                       nunits, parity, t_first, write_ez, halo);
     return hip_check(hipGetLastError(), "fd2d_batch_run launch");
 }
 
@@ -926,14 +959,14 @@ int batch_resident_kv(int *waves)
 template <int K>
 int launch_fd2d_batch_depth(const fd2d_grid *grids, const int *units, int nunits, int waves,
                             int cols, bool vec, int parity, int t_first, int write_ez, int halo,
-                            int rows, hipStream_t s)
+                            hipStream_t s)
 {
     if (cols == 2 && vec)
-        return tb::launch_batch_kv<K, 2, true>(grids, units, nunits, waves, parity, t_first, write_ez, halo, rows, s);
+        return tb::launch_batch_kv<K, 2, true>(grids, units, nunits, waves, parity, t_first, write_ez, halo, s);
     if (cols == 2)
-        return tb::launch_batch_kv<K, 2, false>(grids, units, nunits, waves, parity, t_first, write_ez, halo, rows, s);
+        return tb::launch_batch_kv<K, 2, false>(grids, units, nunits, waves, parity, t_first, write_ez, halo, s);
     if (cols == 1)
-        return tb::launch_batch_kv<K, 1, false>(grids, units, nunits, waves, parity, t_first, write_ez, halo, rows, s);
+        return tb::launch_batch_kv<K, 1, false>(grids, units, nunits, waves, parity, t_first, write_ez, halo, s);
     return fail("columns per lane %d not built", cols);
 }
 

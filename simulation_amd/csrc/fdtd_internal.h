@@ -262,15 +262,16 @@ template <int K>
 int preload_mw_depth();
 
 // Batched grids (lossy PML form): persistent launch over a work list of
-// 4-int units {grid, strip, row block, 0}; parity 1 reads the partner sets.
+// 4-int units {grid, strip, first row, one past the last row}; parity 1
+// reads the partner sets.
 int launch_fd2d_batch(const fd2d_grid *grids, const int *units, int nunits, int waves, int depth,
                       int cols, bool vec, int parity, int t_first, int write_ez, int halo,
-                      int rows, hipStream_t s);
+                      hipStream_t s);
 int fd2d_batch_resident(int depth, int cols, bool vec, int *waves);
 template <int K>
 int launch_fd2d_batch_depth(const fd2d_grid *grids, const int *units, int nunits, int waves,
                             int cols, bool vec, int parity, int t_first, int write_ez, int halo,
-                            int rows, hipStream_t s);
+                            hipStream_t s);
 template <int K>
 int fd2d_batch_resident_depth(int cols, bool vec, int *waves);
 

@@ -102,3 +102,37 @@ def test_batch_rejects_bad_grids(gpu):
         b.run(1)
     with pytest.raises(_lib.FdtdError, match="exceed"):
         b.run(12)
+
+
+def test_full_c3_batch_in_one_graph(gpu):
+    """The whole seeded batch of 256 grids (sides 128..1024, per-grid row
+    blocks) advanced 64 steps by one captured graph; 34 of its grids -- the
+    smallest, the largest and every eighth of the list --
+    against their stand-alone fd2d_tm_run, bit for bit, probes included."""
+    from simulation_amd.batch import BatchSolver, c3_specs, single_solver
+    specs = c3_specs(0)
+    ns = 64
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        b = BatchSolver(specs, ns, device=gpu)
+        print(f"[batch] full C3: {len(specs)} grids, {b.desc.nunits} units, {b.cells} cells",
+              flush=True)
+        b.run_graph(ns)
+        torch.cuda.synchronize()
+        sizes = [sp.nx * sp.ny for sp in specs]
+        pick = {int(np.argmin(sizes)), int(np.argmax(sizes))}
+        pick |= set(range(0, len(specs), len(specs) // 30))
+        pick = sorted(pick)
+        assert len(pick) >= 32
+        for k in pick:
+            sp = specs[k]
+            one = single_solver(sp, ns, gpu)
+            one.run(ns)
+            ref = one.state()
+            got = b.grid_state(k)
+            for f in FIELDS:
+                assert np.array_equal(got[f], ref[f]), (k, sp.nx, f)
+            if sp.probe:
+                assert np.array_equal(b.probe_trace(k), one.probe_traces()[0]), k
+            del one
+        b.close()
