@@ -98,7 +98,7 @@ struct Xch {
 // positions earlier; a ring of 16 rows covers every depth up to 8.  Each
 // wave writes and reads only its own ring, in order, so no barrier guards it.
 constexpr int RING = 16;
-struct CoefRing {
+struct alignas(16) CoefRing {
     float2 v[W][RING][64];
 };
 
@@ -423,7 +423,11 @@ __global__ void __launch_bounds__(NT) tm_mw_kernel(const MwParams M)
         g.j0 = g.vj0 - P.halo + lane;
         g.i0 = M.band[b][0] + y * M.frows;
         g.i1 = min(M.band[b][1], g.i0 + M.frows);
-        tb::sweep<K, 1, LOSSY, PML, false, true, true, true, 0, true>(P, g);
+        // the frame workgroups use no coefficient ring of their own: its LDS
+        // holds the waves' single-wave profile rings
+        static_assert(sizeof(CoefRing) >= W * sizeof(tb::NaRing<K, 1>), "frame rings");
+        tb::sweep<K, 1, LOSSY, PML, false, true, true, true, 0, true>(
+            P, g, reinterpret_cast<tb::NaRing<K, 1> *>(&R)[w]);
         return;
     }
     // interior super-tile, XCD-aware order

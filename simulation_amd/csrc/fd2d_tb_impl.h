@@ -28,8 +28,9 @@
 //     in which those values are compile-time constants and boundary tests are
 //     absent; the few waves at the grid edges or holding a source or probe
 //     run the general body.  Multiplying by a constant 1 is exact, so the
-//     compiler may drop it without changing any result; products with a
-//     constant 0 are kept.
+//     compiler may drop it without changing any result; the terms 0 * ihx
+//     and 0 * ihy of the H update are left out in the interior body, where
+//     they cannot change a stored bit (see iterate).
 //   * One launch covers a K-step group: the host lists the tiles that are
 //     not interior (row blocks, strips, source and probe tiles), their
 //     workgroups come first, then one workgroup per tile of the grid in an
@@ -41,7 +42,9 @@
 //   * Interior waves of tm_tb_kernel keep two sets of level carries and
 //     alternate them between sweep iterations (no register-to-register
 //     copies; the batch kernel keeps one set, which holds it at 4 waves per
-//     SIMD), and all waves without DFT planes store through buffer resources
+//     SIMD).  The profile values naz, nbz a level needs are not carried: the
+//     sweep keeps those of its last rows in a per-wave LDS ring (NaRing).
+//     All waves without DFT planes store through buffer resources
 //     whose range check drops the cells a wave does not own, so every
 //     iteration issues the same stores and the wait for the next row's loads
 //     does not wait for them.
@@ -78,13 +81,26 @@ template <int C> struct DRow { float dz[C], iz[C], na[C], nb[C]; };
 // once per launch.
 template <int C, int NF> struct FRow { float r[NF ? NF : 1][C], i[NF ? NF : 1][C]; };
 
+// D state a level hands to the next one.
+template <int C> struct DSt { float dz[C], iz[C]; };
+
 // Registers one time level carries from one sweep iteration to the next.
 template <int C>
 struct Carry {
     HRow<C> h;        // H input of this level at its E row (next iteration: the row above)
     float e[C];       // ez of this level at its E row
-    float er;         // the same, column right of the lane's last
-    DRow<C> d;        // D/E state of this level at its E row (input of the next level)
+    DSt<C> d;         // D state of this level at its E row (input of the next level)
+};
+
+// The naz and nbz values of the last R rows the sweep loaded, per lane, in
+// LDS (R >= K, a power of two): level s reads those of its row p-s there
+// instead of receiving them through the carries of levels 0 .. s-1, which
+// would take 2 registers per level and two copies per level and row.  Each
+// lane reads only what it wrote itself.
+template <int K> constexpr int ring_len() { return K <= 8 ? 8 : 16; }
+template <int K, int C>
+struct alignas(16) NaRing {
+    float v[ring_len<K>()][64][2 * C];
 };
 
 struct Geom {
@@ -246,14 +262,26 @@ __device__ __forceinline__ void col_coef(const PR &P, int j0, ColCoef<C> &cc)
 template <int K, int C, bool LOSSY, bool PML, bool VEC, bool BROW, bool BCOL, bool SP, int NF,
           bool BS, bool PRO = false, class PR>
 __device__ __forceinline__ void iterate(const PR &P, const Geom &g, const ColCoef<C> &cc,
+                                        NaRing<K, C> &ring,
                                         int p, const HRow<C> &Hload, const DRow<C> &Dload,
                                         const Carry<C> (&Si)[K], Carry<C> (&So)[K],
                                         const FRow<C, NF> &Fload, const FRow<C, NF> (&SFi)[K],
                                         FRow<C, NF> (&SFo)[K], int j = 0)
 {
     const int nx = P.nx, ny = P.ny;
+    constexpr unsigned RM = ring_len<K>() - 1;
+    const unsigned lane = threadIdx.x & 63u;
+    {
+        // row p's profile values (level 0's E row) for the levels below
+        float *w = ring.v[(unsigned)p & RM][lane];
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+            w[c] = Dload.na[c];
+            w[C + c] = Dload.nb[c];
+        }
+    }
     HRow<C> Hin = Hload;
-    DRow<C> Dpend;                        // this iteration's D output of the previous level
+    DSt<C> Dpend;                         // this iteration's D output of the previous level
     FRow<C, NF> Fpend;
 #pragma unroll
     for (int s = 0; s < K; ++s) {
@@ -268,7 +296,14 @@ __device__ __forceinline__ void iterate(const PR &P, const Geom &g, const ColCoe
             Din = Dload;
             if constexpr (NF > 0) Fin = Fload;
         } else {
-            Din = Si[s - 1].d;
+            const float *rd = ring.v[(unsigned)r & RM][lane];
+#pragma unroll
+            for (int c = 0; c < C; ++c) {
+                Din.dz[c] = Si[s - 1].d.dz[c];
+                Din.iz[c] = Si[s - 1].d.iz[c];
+                Din.na[c] = rd[c];
+                Din.nb[c] = LOSSY ? rd[C + c] : 0.f;
+            }
             So[s - 1].d = Dpend;
             if constexpr (NF > 0) {
                 Fin = SFi[s - 1];
@@ -293,7 +328,7 @@ __device__ __forceinline__ void iterate(const PR &P, const Geom &g, const ColCoe
         }
         // ------------------------------------------------ E update, row r
         const float hxl = from_left(Hin.hx[C - 1]);
-        DRow<C> Dn;
+        DSt<C> Dn;
         float En[C];
 #pragma unroll
         for (int c = 0; c < C; ++c) {
@@ -336,8 +371,6 @@ __device__ __forceinline__ void iterate(const PR &P, const Geom &g, const ColCoe
                 ez = Din.na[c] * dz;
             }
             Dn.iz[c] = iz;
-            Dn.na[c] = Din.na[c];
-            Dn.nb[c] = Din.nb[c];
             En[c] = ez;
         }
         FRow<C, NF> Fn;
@@ -350,7 +383,6 @@ __device__ __forceinline__ void iterate(const PR &P, const Geom &g, const ColCoe
                     Fn.i[f][c] = Fin.i[f][c] - P.tw_s[s][f] * En[c];
                 }
         }
-        const float enr = from_right(En[0]);
         if (SP && P.nprobe && r >= g.i0 && r < g.i1) {
             for (int pr = 0; pr < P.nprobe; ++pr)
                 if (r == P.probe_i[pr]) {
@@ -368,10 +400,26 @@ __device__ __forceinline__ void iterate(const PR &P, const Geom &g, const ColCoe
 #pragma unroll
         for (int c = 0; c < C; ++c) {
             const float e = Si[s].e[c];
-            const float eR = (c + 1 < C) ? Si[s].e[c + 1] : Si[s].er;
+            // the column right of the lane's last: the next lane's first,
+            // shifted across when it is used rather than carried
+            const float eR = (c + 1 < C) ? Si[s].e[c + 1] : from_right(Si[s].e[0]);
             const float eD = En[c];
             float hx, hy, ax, ay;
-            if (PML) {
+            if (PML && !BROW && !BCOL) {
+                // every profile value of the tile is its default (fx1 = fy1
+                // = 0, the others 1): the terms 0 * ihx and 0 * ihy only add
+                // a zero.  x + (+-0) is x for every x but a zero, and only
+                // the sign of a zero result could differ: an interior hx or
+                // hy is never -0 (they start at +0, and hx + t or hy - t is
+                // -0 only from -0), so the stored bits are the same as with
+                // the terms (the sum differs from the reference only when
+                // ihx or ihy is infinite or NaN, in a run that has already
+                // blown up).
+                ax = Hp.ix[c] + (e - eR);
+                ay = Hp.iy[c] + (e - eD);
+                hx = Hp.hx[c] + (0.5f * e - 0.5f * eR);
+                hy = Hp.hy[c] - (0.5f * e - 0.5f * eD);
+            } else if (PML) {
                 ax = Hp.ix[c] + (e - eR);
                 ay = Hp.iy[c] + (e - eD);
                 hx = cc.fy3[c] * Hp.hx[c] + cc.fy2[c] * (0.5f * e - 0.5f * eR + fx1q * ax);
@@ -421,10 +469,13 @@ __device__ __forceinline__ void iterate(const PR &P, const Geom &g, const ColCoe
         So[s].h = Hin;
 #pragma unroll
         for (int c = 0; c < C; ++c) So[s].e[c] = En[c];
-        So[s].er = enr;
         if (s + 1 < K) {
             Hin = Hout;          // input of the next level
-            Dpend = Dn;
+#pragma unroll
+            for (int c = 0; c < C; ++c) {
+                Dpend.dz[c] = Dn.dz[c];
+                Dpend.iz[c] = Dn.iz[c];
+            }
             if constexpr (NF > 0) Fpend = Fn;
         } else {
             // last level: D/E of row r and H of row q are final for this launch
@@ -499,10 +550,18 @@ __device__ __forceinline__ void load_ft(const PR &P, int p, int j0, FRow<C, NF> 
 
 template <int K, int C, bool LOSSY, bool PML, bool VEC, bool BROW, bool BCOL, bool SP, int NF = 0,
           bool BS = false, bool PP = false, class PR>
-__device__ __forceinline__ void sweep(const PR &P, const Geom &g)
+__device__ __forceinline__ void sweep(const PR &P, const Geom &g, NaRing<K, C> &ring)
 {
     ColCoef<C> cc;
     col_coef<C, PML, BCOL>(P, g.j0, cc);
+    {
+        // rows above the sweep's first read zeros, as the carries start
+        const unsigned lane = threadIdx.x & 63u;
+#pragma unroll
+        for (int k = 0; k < ring_len<K>(); ++k)
+#pragma unroll
+            for (int c = 0; c < 2 * C; ++c) ring.v[k][lane][c] = 0.f;
+    }
     // PP: two carry sets, St[0] and St[1]; an iteration reads one and writes
     // the other, so no level's carry is copied from register to register
     // (fewer instructions, more registers).  Otherwise one set, updated in
@@ -520,9 +579,7 @@ __device__ __forceinline__ void sweep(const PR &P, const Geom &g)
                 St[a][s].h.ix[c] = 0.f; St[a][s].h.iy[c] = 0.f;
                 St[a][s].e[c] = 0.f;
                 St[a][s].d.dz[c] = 0.f; St[a][s].d.iz[c] = 0.f;
-                St[a][s].d.na[c] = 0.f; St[a][s].d.nb[c] = 0.f;
             }
-            St[a][s].er = 0.f;
             if constexpr (NF > 0) {
 #pragma unroll
                 for (int f = 0; f < NF; ++f)
@@ -554,7 +611,7 @@ __device__ __forceinline__ void sweep(const PR &P, const Geom &g)
         for (; p < c0 + 2 * K - 1; ++p) {
             load_row<C, LOSSY, PML, VEC, BCOL>(P, p + 1, g.j0, H1, D1);
             iterate<K, C, LOSSY, PML, VEC, BROW, BCOL, SP, NF, BS, true>(
-                P, g, cc, p, H0, D0, St[0], St[0], F0, SF[0], SF[0], p - c0);
+                P, g, cc, ring, p, H0, D0, St[0], St[0], F0, SF[0], SF[0], p - c0);
             H0 = H1;
             D0 = D1;
         }
@@ -562,22 +619,22 @@ __device__ __forceinline__ void sweep(const PR &P, const Geom &g)
         for (; p < pend; p += 2) {
             load_row<C, LOSSY, PML, VEC, BCOL>(P, p + 1, g.j0, H1, D1);
             if constexpr (NF > 0) load_ft<C, NF, VEC, BCOL>(P, p + 1, g.j0, F1);
-            iterate<K, C, LOSSY, PML, VEC, BROW, BCOL, SP, NF, BS>(P, g, cc, p, H0, D0, St[0],
+            iterate<K, C, LOSSY, PML, VEC, BROW, BCOL, SP, NF, BS>(P, g, cc, ring, p, H0, D0, St[0],
                                                                    St[NS - 1], F0, SF[0], SF[NS - 1]);
             load_row<C, LOSSY, PML, VEC, BCOL>(P, p + 2, g.j0, H0, D0);
             if constexpr (NF > 0) load_ft<C, NF, VEC, BCOL>(P, p + 2, g.j0, F0);
-            iterate<K, C, LOSSY, PML, VEC, BROW, BCOL, SP, NF, BS>(P, g, cc, p + 1, H1, D1,
+            iterate<K, C, LOSSY, PML, VEC, BROW, BCOL, SP, NF, BS>(P, g, cc, ring, p + 1, H1, D1,
                                                                    St[NS - 1], St[0], F1, SF[NS - 1],
                                                                    SF[0]);
         }
         if (p == pend)
-            iterate<K, C, LOSSY, PML, VEC, BROW, BCOL, SP, NF, BS>(P, g, cc, p, H0, D0, St[0],
+            iterate<K, C, LOSSY, PML, VEC, BROW, BCOL, SP, NF, BS>(P, g, cc, ring, p, H0, D0, St[0],
                                                                    St[NS - 1], F0, SF[0], SF[NS - 1]);
     } else {
         for (int p = pbeg; p <= pend; ++p) {
             load_row<C, LOSSY, PML, VEC, BCOL>(P, p + 1, g.j0, H1, D1);
             if constexpr (NF > 0) load_ft<C, NF, VEC, BCOL>(P, p + 1, g.j0, F1);
-            iterate<K, C, LOSSY, PML, VEC, BROW, BCOL, SP, NF, BS>(P, g, cc, p, H0, D0, St[0], St[0],
+            iterate<K, C, LOSSY, PML, VEC, BROW, BCOL, SP, NF, BS>(P, g, cc, ring, p, H0, D0, St[0], St[0],
                                                                    F0, SF[0], SF[0]);
             H0 = H1;
             D0 = D1;
@@ -685,11 +742,13 @@ __global__ void __launch_bounds__(64) tm_tb_kernel(const TbParams P)
         sp |= (row_in(m - 2) || row_in(P.nx - m)) && js + S > m - 1 && js <= P.ny - m;
     }
     const bool interior = !brow && !bcol && !sp;
+    __shared__ NaRing<K, C> ring;
     if (interior) {
-        if (want_int) sweep<K, C, LOSSY, PML, VEC, false, false, false, NF, NF == 0, NF == 0 && K <= 8>(P, g);
+        if (want_int)
+            sweep<K, C, LOSSY, PML, VEC, false, false, false, NF, NF == 0, NF == 0>(P, g, ring);
     } else if (want_bnd) {
-        if (!brow && !sp) sweep<K, C, LOSSY, PML, VEC, false, true, false, NF, NF == 0>(P, g);
-        else sweep<K, C, LOSSY, PML, VEC, true, true, true, NF, NF == 0>(P, g);
+        if (!brow && !sp) sweep<K, C, LOSSY, PML, VEC, false, true, false, NF, NF == 0>(P, g, ring);
+        else sweep<K, C, LOSSY, PML, VEC, true, true, true, NF, NF == 0>(P, g, ring);
     }
 }
 
@@ -868,6 +927,7 @@ tm_tb_batch_kernel(const fd2d_grid *__restrict__ grids, const int *__restrict__ 
                    int parity, int t_first, int write_ez, int halo)
 {
     constexpr int S = 64 * C;
+    __shared__ NaRing<K, C> ring;
     // XCD-aware slots: wave w (on XCD w % 8) takes the units slot, slot +
     // gridDim.x, ...; the slots of one XCD are contiguous, so in each round
     // the neighbouring units of a grid (strips fastest in the list) run on
@@ -926,8 +986,8 @@ tm_tb_batch_kernel(const fd2d_grid *__restrict__ grids, const int *__restrict__ 
         if (P.nprobe)
             sp |= P.probe_i.v >= g.i0 && P.probe_i.v < g.i1 && P.probe_j.v >= js &&
                   P.probe_j.v < js + S;
-        if (!brow && !bcol && !sp) sweep<K, C, true, true, VEC, false, false, false, 0, true>(P, g);
-        else sweep<K, C, true, true, VEC, true, true, true, 0, true>(P, g);
+        if (!brow && !bcol && !sp) sweep<K, C, true, true, VEC, false, false, false, 0, true>(P, g, ring);
+        else sweep<K, C, true, true, VEC, true, true, true, 0, true>(P, g, ring);
     }
 }
 
