@@ -92,15 +92,16 @@ struct Carry {
     DSt<C> d;         // D state of this level at its E row (input of the next level)
 };
 
-// The naz and nbz values of the last R rows the sweep loaded, per lane, in
-// LDS (R >= K, a power of two): level s reads those of its row p-s there
-// instead of receiving them through the carries of levels 0 .. s-1, which
-// would take 2 registers per level and two copies per level and row.  Each
-// lane reads only what it wrote itself.
-template <int K> constexpr int ring_len() { return K <= 8 ? 8 : 16; }
+// The naz and nbz values of the last K rows the sweep loaded, per lane, in
+// LDS: level s reads those of its row p-s there instead of receiving them
+// through the carries of levels 0 .. s-1, which would take 2 registers per
+// level and two copies per level and row.  Row p goes to slots p % K and
+// p % K + K, so level s finds row p-s at slot p % K + K - s: one address per
+// sweep position and a constant offset per level.  Each lane reads only what
+// it wrote itself.
 template <int K, int C>
 struct alignas(16) NaRing {
-    float v[ring_len<K>()][64][2 * C];
+    float v[2 * K][64][2 * C];
 };
 
 struct Geom {
@@ -269,15 +270,18 @@ __device__ __forceinline__ void iterate(const PR &P, const Geom &g, const ColCoe
                                         FRow<C, NF> (&SFo)[K], int j = 0)
 {
     const int nx = P.nx, ny = P.ny;
-    constexpr unsigned RM = ring_len<K>() - 1;
     const unsigned lane = threadIdx.x & 63u;
+    const unsigned slot = (unsigned)p % (unsigned)K;   // p >= 0
     {
         // row p's profile values (level 0's E row) for the levels below
-        float *w = ring.v[(unsigned)p & RM][lane];
 #pragma unroll
-        for (int c = 0; c < C; ++c) {
-            w[c] = Dload.na[c];
-            w[C + c] = Dload.nb[c];
+        for (int h = 0; h < 2; ++h) {
+            float *w = ring.v[slot + h * K][lane];
+#pragma unroll
+            for (int c = 0; c < C; ++c) {
+                w[c] = Dload.na[c];
+                w[C + c] = Dload.nb[c];
+            }
         }
     }
     HRow<C> Hin = Hload;
@@ -296,7 +300,7 @@ __device__ __forceinline__ void iterate(const PR &P, const Geom &g, const ColCoe
             Din = Dload;
             if constexpr (NF > 0) Fin = Fload;
         } else {
-            const float *rd = ring.v[(unsigned)r & RM][lane];
+            const float *rd = ring.v[slot + K - s][lane];
 #pragma unroll
             for (int c = 0; c < C; ++c) {
                 Din.dz[c] = Si[s - 1].d.dz[c];
@@ -406,6 +410,11 @@ __device__ __forceinline__ void iterate(const PR &P, const Geom &g, const ColCoe
             const float eD = En[c];
             float hx, hy, ax, ay;
             if (PML && !BROW && !BCOL) {
+                // 0.5 * eR is the right column's 0.5 * e, shifted across as
+                // it is (the same bits): the shift then folds into the
+                // subtraction, as the one of eR does into e - eR
+                const float he = 0.5f * e;
+                const float heR = (c + 1 < C) ? 0.5f * Si[s].e[c + 1] : from_right(0.5f * Si[s].e[0]);
                 // every profile value of the tile is its default (fx1 = fy1
                 // = 0, the others 1): the terms 0 * ihx and 0 * ihy only add
                 // a zero.  x + (+-0) is x for every x but a zero, and only
@@ -417,8 +426,8 @@ __device__ __forceinline__ void iterate(const PR &P, const Geom &g, const ColCoe
                 // blown up).
                 ax = Hp.ix[c] + (e - eR);
                 ay = Hp.iy[c] + (e - eD);
-                hx = Hp.hx[c] + (0.5f * e - 0.5f * eR);
-                hy = Hp.hy[c] - (0.5f * e - 0.5f * eD);
+                hx = Hp.hx[c] + (he - heR);
+                hy = Hp.hy[c] - (he - 0.5f * eD);
             } else if (PML) {
                 ax = Hp.ix[c] + (e - eR);
                 ay = Hp.iy[c] + (e - eD);
@@ -558,7 +567,7 @@ __device__ __forceinline__ void sweep(const PR &P, const Geom &g, NaRing<K, C> &
         // rows above the sweep's first read zeros, as the carries start
         const unsigned lane = threadIdx.x & 63u;
 #pragma unroll
-        for (int k = 0; k < ring_len<K>(); ++k)
+        for (int k = 0; k < 2 * K; ++k)
 #pragma unroll
             for (int c = 0; c < 2 * C; ++c) ring.v[k][lane][c] = 0.f;
     }
@@ -651,7 +660,8 @@ __device__ __forceinline__ void sweep(const PR &P, const Geom &g, NaRing<K, C> &
 // listed others, as two launches that write disjoint cells and may run
 // concurrently.  A wave returns at once if its tile is not its part's.
 template <int K, int C, bool LOSSY, bool PML, bool VEC, int PART, int NF = 0>
-__global__ void __launch_bounds__(64) tm_tb_kernel(const TbParams P)
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(C == 1 && K <= 10 ? 3 : 1)))
+tm_tb_kernel(const TbParams P)
 {
     constexpr int S = 64 * C;
     const int w = S - 2 * P.halo;
@@ -745,7 +755,7 @@ __global__ void __launch_bounds__(64) tm_tb_kernel(const TbParams P)
     __shared__ NaRing<K, C> ring;
     if (interior) {
         if (want_int)
-            sweep<K, C, LOSSY, PML, VEC, false, false, false, NF, NF == 0, NF == 0>(P, g, ring);
+            sweep<K, C, LOSSY, PML, VEC, false, false, false, NF, NF == 0, NF == 0 && C == 1>(P, g, ring);
     } else if (want_bnd) {
         if (!brow && !sp) sweep<K, C, LOSSY, PML, VEC, false, true, false, NF, NF == 0>(P, g, ring);
         else sweep<K, C, LOSSY, PML, VEC, true, true, true, NF, NF == 0>(P, g, ring);

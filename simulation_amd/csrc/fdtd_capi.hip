@@ -930,6 +930,10 @@ static int tfsf_fused(const fd2d_tm_desc *d, const fd2d_tfsf_desc *b, int t0, in
     fd2d_tm_desc dd = *d;
     dd.form = FDTD_TM_FORM_STRIP;          // TF/SF corrections and DFT: strip form
     if (b->nf && !dd.depth) dd.depth = 4;
+    // the DFT bodies keep their level loop unrolled up to depth 8 only (past
+    // it the twiddle table is indexed at run time and the launch parameters
+    // go to scratch memory); the depth changes no result
+    if (b->nf && dd.depth > 8) dd.depth = 8;
     if (b->nf && !dd.rows) {
         const int h = d->nx / 220;
         dd.rows = h < 4 ? 4 : (h > 512 ? 512 : h);
