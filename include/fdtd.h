@@ -272,8 +272,8 @@ int fd2d_tm_run(const fd2d_tm_desc *d, int t0, int nsteps, void *stream);
 
 /* The launch plan fd2d_tm_run would use for this descriptor and nsteps:
  * number of launches and the chosen depth/rows/cols.  Defaults (measured on
- * MI355X): one column per lane; depth 10 from nx = 6144 on, else 8; rows =
- * nx/220 clamped to [16, 512], except 64 for 6144 <= nx < 12288; the
+ * MI355X): one column per lane; depth 10 from nx = 2048 on, else 8; rows =
+ * nx/128 for 4096 <= nx < 12288, else nx/170, clamped to [8, 512]; the
  * multi-wave form: depth 8, rows = nx/32 clamped to [32, 512].  Depths go up
  * to 12.  Host only.                                                       */
 int fd2d_tm_plan(const fd2d_tm_desc *d, int nsteps,

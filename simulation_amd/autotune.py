@@ -26,16 +26,16 @@ CACHE = os.environ.get("FDTD_TUNE_CACHE",
 
 def candidates(nx: int):
     """(depth, rows, cols, form) shapes tried for an nx-row grid: single-wave
-    strips at depths 10 and 8 with row blocks of nx/220 and nx/128 rows
-    (MI355X sweeps put the optimum there: 4096^2 at depth 8 and ~nx/220,
-    8192^2 at depth 10 and ~nx/128, 16384^2 at depth 10 and ~nx/220), two
-    columns per lane at the first of those, and the multi-wave form with
-    short blocks."""
-    rows = sorted({min(512, max(16, nx // d)) for d in (220, 128)})
-    for depth in (10, 8):
-        for r in rows:
-            yield depth, r, 1, 1
-    yield 8, rows[0], 2, 1
+    strips at depth 10 with row blocks of nx/170, nx/128 and nx/96 rows and
+    at depth 8 with nx/128 (MI355X sweeps put the optimum at depth 10 from
+    2048^2 on, rows ~nx/170 at 2048^2 and ~nx/128 from 4096^2 on), two
+    columns per lane at depth 8, and the multi-wave form with short blocks."""
+    def r(d):
+        return min(512, max(8, nx // d))
+    for rows in sorted({r(170), r(128), r(96)}):
+        yield 10, rows, 1, 1
+    yield 8, r(128), 1, 1
+    yield 8, r(128), 2, 1
     yield 8, 32, 1, 2
 
 

@@ -996,7 +996,7 @@ tm_tb_batch_kernel(const fd2d_grid *__restrict__ grids, const int *__restrict__ 
         if (P.nprobe)
             sp |= P.probe_i.v >= g.i0 && P.probe_i.v < g.i1 && P.probe_j.v >= js &&
                   P.probe_j.v < js + S;
-        if (!brow && !bcol && !sp) sweep<K, C, true, true, VEC, false, false, false, 0, true>(P, g, ring);
+        if (!brow && !bcol && !sp) sweep<K, C, true, true, VEC, false, false, false, 0, true, C == 1>(P, g, ring);
         else sweep<K, C, true, true, VEC, true, true, true, 0, true>(P, g, ring);
     }
 }

@@ -115,21 +115,22 @@ int mw_default_rows(int nx)
 }
 
 // Default depth and rows of the single-wave strips, from sweeps on MI355X
-// (lossy PML form, one column per lane; Mcell-updates/s at 2000 steps):
-//   4096^2:  depth 8, rows 18 (446k; depth 10 at rows 16-26: 362k-407k)
-//   8192^2:  depth 10, rows 64 (595k; rows 56/72: 556k/576k; depth 8 rows
-//            37/64: 561k/582k)
-//   16384^2: depth 10, rows 74 (618k; rows 100/128: 606k/588k; depth 8 rows
-//            74: 592k)
-// Deeper launches pay off from 8192^2 on (more steps per pass over HBM,
-// one carry set at 3 waves per SIMD); rows follow ~nx/220 except around
-// 8192^2, where blocks of 64 rows are best.
+// (lossy PML form, one column per lane; Mcell-updates/s at 1000-2000 steps):
+//   2048^2:  depth 10, rows 12 (295k; rows 16/24: 260k/206k; depth 8 rows 16:
+//            278k)
+//   4096^2:  depth 10, rows 32 (537k; rows 24/28/40: 518k/539k/541k; depth 8
+//            rows 16/32/48: 449k/494k/461k; depth 12: 482k)
+//   8192^2:  depth 10, rows 64 (657k; rows 56/72/80: 624k/645k/652k; depth 8
+//            rows 64: 586k; depth 12: 576k)
+//   16384^2: depth 10, rows 96 (691k; rows 64/128/160: 679k/688k/681k;
+//            depth 8 rows 96: 606k)
+// Depth 10 (two carry sets, 3 waves per SIMD) wins from 2048^2 on; rows
+// follow nx/128 for 4096 <= nx < 12288 and ~nx/170 outside.
 void tm_defaults(int nx, int *depth, int *rows)
 {
-    *depth = nx >= 6144 ? 10 : 8;
-    long h = nx / 220;
-    if (nx >= 6144 && nx < 12288) h = 64;
-    if (h < 16) h = 16;
+    *depth = nx >= 2048 ? 10 : 8;
+    long h = nx >= 4096 && nx < 12288 ? nx / 128 : nx / 170;
+    if (h < 8) h = 8;
     if (h > 512) h = 512;
     *rows = (int)h;
 }
