@@ -293,15 +293,20 @@ def sample(ez, hx, hy, nx: int, ny: int, dtype=np.float32) -> np.ndarray:
     right:  ez[t,ny-1-D], hn =  hx[t,ny-2-D],   ht = -hy[t,ny-1-D]
     ht is taken only at offsets -1 and 0 -- the two H points beside the cell
     (ht is staggered by half a cell along the edge), so a prediction reaches
-    one cell to either side; H in the last row or column reads as 0 (the H
-    update never runs there, so a window of a larger grid must not see it
-    either); tangential indices are clamped to the edge.  Then ez of grid
+    one cell to either side; H in the last row or column reads as +0,
+    whatever its sign (the H update never runs there, so a window of a
+    larger grid must not see it either); tangential indices are clamped to
+    the edge.  Then ez of grid
     depths 2, 1, 0 at offset 0 (targets and baseline, not network inputs)."""
     pos = positions(nx, ny)
     out = np.zeros((len(pos), NSAMP), dtype)
 
-    def H(a, i, j):
-        return dtype(0.0) if (i == nx - 1 or j == ny - 1) else a[i, j]
+    def H(a, i, j, sign=1):
+        # the last row or column reads as +0 whatever the sign; any other
+        # cell is negated as it is (a stored +0 gives -0)
+        if i == nx - 1 or j == ny - 1:
+            return dtype(0.0)
+        return -a[i, j] if sign < 0 else a[i, j]
 
     def ezc(e, D, t):
         i, j = cell(e, D, t, nx, ny)
@@ -315,11 +320,11 @@ def sample(ez, hx, hy, nx: int, ny: int, dtype=np.float32) -> np.ndarray:
                 if e == 0:
                     v = (H(hy, D, tt), H(hx, D, tt))
                 elif e == 1:
-                    v = (-H(hy, nx - 2 - D, tt), H(hx, nx - 1 - D, tt))
+                    v = (H(hy, nx - 2 - D, tt, -1), H(hx, nx - 1 - D, tt))
                 elif e == 2:
-                    v = (-H(hx, tt, D), -H(hy, tt, D))
+                    v = (H(hx, tt, D, -1), H(hy, tt, D, -1))
                 else:
-                    v = (H(hx, tt, ny - 2 - D), -H(hy, tt, ny - 1 - D))
+                    v = (H(hx, tt, ny - 2 - D), H(hy, tt, ny - 1 - D, -1))
                 out[p, sidx(0, d, o)] = ezc(e, D, tt)
                 out[p, sidx(1, d, o)] = v[0]
                 out[p, sidx(2, d, o)] = v[1] if o < 1 else 0.0

@@ -16,6 +16,19 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs a ROCm GPU and the built libfdtd_hip.so")
 
 
+def same_bits(a, b) -> bool:
+    """True when a and b hold the same values bit for bit: floating-point
+    arrays of one dtype are compared as their integer bit patterns (so +0 and
+    -0 differ, and a NaN equals the same NaN); anything else by value."""
+    a, b = np.asarray(a), np.asarray(b)
+    if a.shape != b.shape:
+        return False
+    if a.dtype == b.dtype and a.dtype.kind == "f":
+        u = np.dtype(f"u{a.dtype.itemsize}")
+        return bool(np.array_equal(np.ascontiguousarray(a).view(u), np.ascontiguousarray(b).view(u)))
+    return bool(np.array_equal(a, b))
+
+
 def golden(name: str):
     """(meta dict, {key: array}) of tests/golden/<name>.npz."""
     d = np.load(os.path.join(GOLDEN, name + ".npz"), allow_pickle=False)

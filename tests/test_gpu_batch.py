@@ -6,6 +6,7 @@ import ctypes as C
 import numpy as np
 import pytest
 import torch
+from conftest import same_bits
 
 pytestmark = pytest.mark.gpu
 
@@ -39,9 +40,9 @@ def test_batch_matches_single_grid_runs(gpu, depth, rows, cols):
         ref = s.state()
         got = b.grid_state(k)
         for f in FIELDS:
-            assert np.array_equal(got[f], ref[f]), (k, f)
+            assert same_bits(got[f], ref[f]), (k, f)
         if sp.probe:
-            assert np.array_equal(b.probe_trace(k), s.probe_traces()[0]), k
+            assert same_bits(b.probe_trace(k), s.probe_traces()[0]), k
 
 
 def test_batch_grid_matches_oracle(gpu, oracle):
@@ -61,8 +62,8 @@ def test_batch_grid_matches_oracle(gpu, oracle):
                                 [(si, sj)], w[None, :], [sp.probe])
         got = b.grid_state(k)
         for f in FIELDS:
-            assert np.array_equal(got[f], getattr(o, f)), (k, f)
-        assert np.array_equal(b.probe_trace(k), pr[0])
+            assert same_bits(got[f], getattr(o, f)), (k, f)
+        assert same_bits(b.probe_trace(k), pr[0])
 
 
 def test_batch_graph_replays_match_eager(gpu):
@@ -88,7 +89,7 @@ def test_batch_graph_replays_match_eager(gpu):
             for k in range(len(specs)):
                 a, b = ge.grid_state(k), gg.grid_state(k)
                 for f in FIELDS:
-                    assert np.array_equal(a[f], b[f]), (rep, k, f)
+                    assert same_bits(a[f], b[f]), (rep, k, f)
         gg.close()
 
 
@@ -131,8 +132,8 @@ def test_full_c3_batch_in_one_graph(gpu):
             ref = one.state()
             got = b.grid_state(k)
             for f in FIELDS:
-                assert np.array_equal(got[f], ref[f]), (k, sp.nx, f)
+                assert same_bits(got[f], ref[f]), (k, sp.nx, f)
             if sp.probe:
-                assert np.array_equal(b.probe_trace(k), one.probe_traces()[0]), k
+                assert same_bits(b.probe_trace(k), one.probe_traces()[0]), k
             del one
         b.close()

@@ -6,6 +6,7 @@ import pytest
 import torch
 
 from cases import dform_case
+from conftest import same_bits
 
 pytestmark = pytest.mark.gpu
 
@@ -36,10 +37,10 @@ def _check(c, s):
     e = c["exp"]
     torch.cuda.synchronize()
     for k in ("dx", "ix", "ex", "hy", "bc") + (("sx",) if c["debye"] else ()):
-        assert np.array_equal(getattr(s, k).cpu().numpy(), e["out_" + k]), k
+        assert same_bits(getattr(s, k).cpu().numpy(), e["out_" + k]), k
     if len(c["freq"]):
         for k in ("r_pt", "i_pt", "r_in", "i_in"):
-            assert np.array_equal(getattr(s.ft, k).cpu().numpy(), e["out_ft_" + k]), k
+            assert same_bits(getattr(s.ft, k).cpu().numpy(), e["out_ft_" + k]), k
 
 
 @pytest.mark.parametrize("name", CASES)
@@ -48,7 +49,7 @@ def test_dform_coefficients_bit_exact(gpu, name):
     _, md = _setup(c, gpu)
     for k in ("nax", "nbx", "ncx", "ndx"):
         if "out_md_" + k in c["exp"]:
-            assert np.array_equal(getattr(md, k).cpu().numpy(), c["exp"]["out_md_" + k]), k
+            assert same_bits(getattr(md, k).cpu().numpy(), c["exp"]["out_md_" + k]), k
 
 
 @pytest.mark.parametrize("name", CASES)
@@ -63,7 +64,7 @@ def test_dform_run_bit_exact(gpu, name, split):
         s.run(n)
     _check(c, s)
     torch.cuda.synchronize()
-    assert np.array_equal(s.probe_out.cpu().numpy(), c["exp"]["out_probe"])
+    assert same_bits(s.probe_out.cpu().numpy(), c["exp"]["out_probe"])
 
 
 @pytest.mark.parametrize("name", CASES)
@@ -102,7 +103,7 @@ def test_dform_run_per_step_path_bit_exact(gpu, name):
     s.run(c["ns"])
     _check(c, s)
     torch.cuda.synchronize()
-    assert np.array_equal(s.probe_out.cpu().numpy(), c["exp"]["out_probe"])
+    assert same_bits(s.probe_out.cpu().numpy(), c["exp"]["out_probe"])
 
 
 @pytest.mark.parametrize("nx,debye,nf,chunks", [

@@ -9,6 +9,7 @@ import pytest
 import torch
 
 from cases import FD1D_CASES, fd1d_case
+from conftest import same_bits
 
 pytestmark = pytest.mark.gpu
 
@@ -40,11 +41,11 @@ def test_fd1d_step_kernels_bit_exact(gpu, name):
         if c["probes"]:
             probe[:, s] = ex[list(c["probes"])].cpu().numpy()
     e = c["exp"]
-    assert np.array_equal(ex.cpu().numpy(), e["out_ex"])
-    assert np.array_equal(hy.cpu().numpy(), e["out_hy"])
-    assert np.array_equal(probe, e["out_probe"])
+    assert same_bits(ex.cpu().numpy(), e["out_ex"])
+    assert same_bits(hy.cpu().numpy(), e["out_hy"])
+    assert same_bits(probe, e["out_probe"])
     if c["abc"]:
-        assert np.array_equal(bc.cpu().numpy(), e["out_bc"])
+        assert same_bits(bc.cpu().numpy(), e["out_bc"])
 
 
 @pytest.mark.parametrize("name", FD1D_CASES)
@@ -68,14 +69,14 @@ def test_fd1d_single_launch_bit_exact(gpu, name):
         sh.append(hy.cpu().numpy())
         t = cut
     e = c["exp"]
-    assert np.array_equal(ex.cpu().numpy(), e["out_ex"])
-    assert np.array_equal(hy.cpu().numpy(), e["out_hy"])
-    assert np.array_equal(torch.cat(probes, 1).cpu().numpy(), e["out_probe"])
+    assert same_bits(ex.cpu().numpy(), e["out_ex"])
+    assert same_bits(hy.cpu().numpy(), e["out_hy"])
+    assert same_bits(torch.cat(probes, 1).cpu().numpy(), e["out_probe"])
     if c["abc"]:
-        assert np.array_equal(bc.cpu().numpy(), e["out_bc"])
+        assert same_bits(bc.cpu().numpy(), e["out_bc"])
     if c["snap_t"]:
-        assert np.array_equal(np.array(sx), e["out_snap_ex"])
-        assert np.array_equal(np.array(sh), e["out_snap_hy"])
+        assert same_bits(np.array(sx), e["out_snap_ex"])
+        assert same_bits(np.array(sh), e["out_snap_hy"])
 
 
 def test_fd1d_c1_simulate_matches_oracle_fp32_and_fp64(gpu, oracle):
@@ -89,8 +90,8 @@ def test_fd1d_c1_simulate_matches_oracle_fp32_and_fp64(gpu, oracle):
         ex = np.zeros(200, dt); hy = np.zeros(200, dt); bc = np.zeros(4, dt)
         w = oracle.waveform("gaussian", 0, 500, dt, center=40, spread=12.0)
         pr = oracle.fd1d_run(200, 0, 500, ex, hy, bc, None, None, 1, 100, w, (150,))
-        assert np.array_equal(out["ex"], ex) and np.array_equal(out["hy"], hy)
-        assert np.array_equal(out["probe"], pr)
+        assert same_bits(out["ex"], ex) and same_bits(out["hy"], hy)
+        assert same_bits(out["probe"], pr)
         res[dt] = out
     p32, p64 = res[np.float32]["probe"][0], res[np.float64]["probe"][0]
     rel = np.linalg.norm(p32 - p64) / np.linalg.norm(p64)
@@ -103,8 +104,8 @@ def test_fd1d_dielectric_device_matches_oracle(gpu, oracle):
     for dt in (np.float32, np.float64):
         ca, cb = fd1d.dielectric(512, 0.01 / 6e8, 4.0, 0.04, gpu, dt)
         rca, rcb = oracle.fd1d_dielectric(512, 0.01 / 6e8, 4.0, 0.04)
-        assert np.array_equal(ca.cpu().numpy(), rca.astype(dt))
-        assert np.array_equal(cb.cpu().numpy(), rcb.astype(dt))
+        assert same_bits(ca.cpu().numpy(), rca.astype(dt))
+        assert same_bits(cb.cpu().numpy(), rcb.astype(dt))
 
 
 def test_fd1d_hard_source_mirror_symmetry(gpu):
@@ -118,5 +119,5 @@ def test_fd1d_hard_source_mirror_symmetry(gpu):
     fd1d.run(nx, 300, ex, hy, bc, None, None, fd1d.SRC_HARD, kc, w)
     e, h = ex.cpu().numpy(), hy.cpu().numpy()
     m = np.arange(1, 190)
-    assert np.array_equal(e[kc - m], e[kc + m])
-    assert np.array_equal(h[kc + m], -h[kc - m - 1])
+    assert same_bits(e[kc - m], e[kc + m])
+    assert same_bits(h[kc + m], -h[kc - m - 1])

@@ -10,6 +10,7 @@ import pytest
 import torch
 
 from cases import FD2D_CASES, PML_KEYS, fd2d_case
+from conftest import same_bits
 
 pytestmark = pytest.mark.gpu
 
@@ -38,8 +39,8 @@ def _check(c, state, probe):
     e = c["exp"]
     for k in ("dz", "ez", "hx", "hy", "iz", "ihx", "ihy"):
         if "out_" + k in e and state.get(k) is not None:
-            assert np.array_equal(state[k], e["out_" + k]), k
-    assert np.array_equal(probe, e["out_probe"])
+            assert same_bits(state[k], e["out_" + k]), k
+    assert same_bits(probe, e["out_probe"])
 
 
 @pytest.mark.parametrize("name", F32_CASES)
@@ -83,7 +84,7 @@ def test_pmlparam_device_matches_oracle(gpu, oracle, nx, ny, npml):
     fd2d.pmlparam(nx, ny, npml, pml)
     ref = oracle.fd2d_pmlparam(nx, ny, npml)
     for k in PML_KEYS:
-        assert np.array_equal(getattr(pml, k).cpu().numpy(), ref[k].astype(np.float32)), k
+        assert same_bits(getattr(pml, k).cpu().numpy(), ref[k].astype(np.float32)), k
 
 
 @pytest.mark.parametrize("int_sub", [False, True])
@@ -97,8 +98,8 @@ def test_dielectric_device_matches_oracle(gpu, oracle, int_sub):
         cyl = cyl[:1]
     md = fd2d.dielectric(nx, ny, npml, 0, dt, 1, 0, gpu, cylinders=cyl, int_sub=int_sub)
     naz, nbz = oracle.fd2d_dielectric(nx, ny, npml, dt, cyl, int_sub=int_sub)
-    assert np.array_equal(md.naz.cpu().numpy(), naz.astype(np.float32))
-    assert np.array_equal(md.nbz.cpu().numpy(), nbz.astype(np.float32))
+    assert same_bits(md.naz.cpu().numpy(), naz.astype(np.float32))
+    assert same_bits(md.nbz.cpu().numpy(), nbz.astype(np.float32))
 
 
 def test_energy_matches_oracle(gpu, oracle):
@@ -132,8 +133,8 @@ def test_c2_shape_fused_matches_oracle(gpu, oracle, n, ns, depth):
     o = oracle.TMState(n, n, np.float32)
     pr = oracle.fd2d_tm_run(o, 0, ns, pml, naz, nbz, [cfg.sources[0][:2]], w[None, :], cfg.probes)
     for k in ("dz", "iz", "ez", "hx", "hy", "ihx", "ihy"):
-        assert np.array_equal(st[k], getattr(o, k)), k
-    assert np.array_equal(s.probe_traces(), pr)
+        assert same_bits(st[k], getattr(o, k)), k
+    assert same_bits(s.probe_traces(), pr)
     # FP64 oracle on FP64 coefficients (the PyCUDA set-up before narrowing)
     p64 = oracle.fd2d_pmlparam(n, n, cfg.npml)
     n64, b64 = oracle.fd2d_dielectric(n, n, cfg.npml, fd2d.time_step(), cfg.cylinders)
@@ -163,7 +164,7 @@ def test_fused_is_deterministic_and_matches_unfused(gpu):
     outs.append(s.state())
     for o in outs[1:]:
         for k in ("dz", "ez", "hx", "hy", "iz", "ihx", "ihy"):
-            assert np.array_equal(o[k], outs[0][k]), k
+            assert same_bits(o[k], outs[0][k]), k
 
 
 @pytest.mark.parametrize("depth,rows", [(8, 9), (8, 10), (5, 13), (5, 14)])
@@ -185,8 +186,8 @@ def test_interior_row_blocks_odd_and_even(gpu, depth, rows):
         b.step_unfused()
     sa, sb = a.state(), b.state()
     for k in ("dz", "ez", "hx", "hy", "iz", "ihx", "ihy"):
-        assert np.array_equal(sa[k], sb[k]), k
-    assert np.array_equal(a.probe_traces(), b.probe_traces())
+        assert same_bits(sa[k], sb[k]), k
+    assert same_bits(a.probe_traces(), b.probe_traces())
     assert np.abs(sa["ez"]).max() > 0.01
 
 
@@ -211,6 +212,6 @@ def test_many_sources_and_probes_tile_lists(gpu, rows):
         b.step_unfused()
     sa, sb = a.state(), b.state()
     for k in ("dz", "ez", "hx", "hy", "iz", "ihx", "ihy"):
-        assert np.array_equal(sa[k], sb[k]), k
-    assert np.array_equal(a.probe_traces(), b.probe_traces())
+        assert same_bits(sa[k], sb[k]), k
+    assert same_bits(a.probe_traces(), b.probe_traces())
     assert np.abs(sa["ez"]).max() > 0.1

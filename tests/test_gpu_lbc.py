@@ -12,6 +12,7 @@ import pytest
 import torch
 
 from oracle import lbc_oracle as L
+from conftest import same_bits
 
 pytestmark = pytest.mark.gpu
 
@@ -46,11 +47,11 @@ def test_sample_matches_oracle_bit_exact(gpu):
     F = [rng.standard_normal((53, 61)).astype(np.float32) for _ in range(3)]
     T = [torch.from_numpy(f).to(gpu) for f in F]
     got = learned.sample(*T).cpu().numpy()
-    assert np.array_equal(got, L.sample(*F, 53, 61))
+    assert same_bits(got, L.sample(*F, 53, 61))
     win = (5, 42, 7, 50)
     got = learned.sample(*T, window=win).cpu().numpy()
     want = L.sample(*[np.ascontiguousarray(f[5:42, 7:50]) for f in F], 37, 43)
-    assert np.array_equal(got, want)
+    assert same_bits(got, want)
 
 
 def test_forward_matches_oracle(gpu):
@@ -60,7 +61,7 @@ def test_forward_matches_oracle(gpu):
     y, s = net.forward(*_rows(X, gpu))
     y, s = y.cpu().numpy(), s.cpu().numpy()
     yo, c = L.forward(p, X, rounding=True)
-    assert np.array_equal(s, c["s"])
+    assert same_bits(s, c["s"])
     rel = np.linalg.norm(y - yo) / np.linalg.norm(yo)
     print(f"[lbc forward] rel L2 {rel:.2e}, max abs {np.abs(y - yo).max():.2e}", flush=True)
     assert rel < 2e-4
@@ -92,7 +93,7 @@ def test_gradient_matches_oracle(gpu):
     assert cos > 0.9995
     # deterministic: same blocks, same bits
     _, g2 = net.grad(src, offs, ts, tgt=torch.from_numpy(T).to(gpu), nblocks=7)
-    assert np.array_equal(g2.cpu().numpy().astype(np.float64), g)
+    assert same_bits(g2.cpu().numpy().astype(np.float64), g)
 
 
 def test_next_sample_targets(gpu):
@@ -138,7 +139,7 @@ def test_adam_step_matches_oracle(gpu):
     w = net.wbuf.cpu().numpy().view(np.uint16)
     W1 = L.unpack(got)["W1"]
     b = (w[:64 * 128].reshape(64, 128)[:, :108].astype(np.uint32) << 16).view(np.float32)
-    assert np.array_equal(b, L.bf16(W1))
+    assert same_bits(b, L.bf16(W1))
 
 
 def _next_rows(X, S1, T, dev):
@@ -198,7 +199,7 @@ def test_gram_and_ridge_fit_match_oracle(gpu):
     Z = np.zeros_like(X[:64])
     with pytest.raises(_lib.FdtdError, match="pivot 0"):
         net.fit_linear(net.gram(*_next_rows(Z, S1[:64] * 0, T[:64] * 0, gpu)), 0.0)
-    assert np.array_equal(net.numpy(), before)
+    assert same_bits(net.numpy(), before)
 
 
 def test_training_reduces_loss(gpu):
@@ -228,7 +229,7 @@ def test_training_reduces_loss(gpu):
     l1, _ = net.grad(src, offs, ts, tgt=tg)
     print(f"[lbc train] loss after the linear fit {l0:.3e} -> {l1:.3e}", flush=True)
     assert l1 < 0.1 * l0
-    assert np.array_equal(net.numpy()[L.P_L:], fitted)
+    assert same_bits(net.numpy()[L.P_L:], fitted)
 
 
 def _frame_operands(p):
@@ -266,14 +267,14 @@ def test_frame_operand_layout(gpu):
     nfrm = 64 * 26 * 8
     assert w.size == net.wbuf.numel() and w.size >= nfrm + 8
     got = (w[-nfrm - 8:-8].astype(np.uint32) << 16).view(np.float32)
-    assert np.array_equal(got, _frame_operands(p))
+    assert same_bits(got, _frame_operands(p))
     assert w[-8] == 1 and not w[-7:].any()
     src, offs, ts = _rows(X, gpu)
     net.train_step(src, offs, ts, 1e-2, tgt=torch.from_numpy(T).to(gpu), nblocks=3)
     fresh = learned.Net(net.numpy(), gpu)
     assert torch.equal(net.wbuf, fresh.wbuf)
     got = (net.wbuf.cpu().numpy().view(np.uint16)[-nfrm - 8:-8].astype(np.uint32) << 16).view(np.float32)
-    assert np.array_equal(got, _frame_operands(net.numpy()))
+    assert same_bits(got, _frame_operands(net.numpy()))
     # a zero output layer: the flag is clear, and an Adam step that moves W3
     # off zero raises it
     q = p.copy()

@@ -6,6 +6,7 @@ learned-boundary solver (history) and the 1-D D/E-form solver."""
 import numpy as np
 import pytest
 import torch
+from conftest import same_bits
 
 pytestmark = pytest.mark.gpu
 
@@ -21,7 +22,7 @@ def _same(a, b):
         if a[k] is None:
             assert b[k] is None, k
         else:
-            assert np.array_equal(a[k], b[k]), k
+            assert same_bits(a[k], b[k]), k
 
 
 def _roundtrip(make, first, then, tmp_path):

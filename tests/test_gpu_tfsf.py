@@ -11,6 +11,7 @@ import pytest
 import torch
 
 from cases import PML_KEYS, tfsf_case
+from conftest import same_bits
 
 pytestmark = pytest.mark.gpu
 
@@ -40,12 +41,12 @@ def _check(c, s):
     for k in ("dz", "iz", "ez", "hx", "hy", "ihx", "ihy"):
         if c["nbz"] is None and k == "iz":
             continue
-        assert np.array_equal(st[k], e["out_" + k]), k
+        assert same_bits(st[k], e["out_" + k]), k
     for k in ("ezi", "hxi", "bc"):
-        assert np.array_equal(getattr(s, k).cpu().numpy(), e["out_" + k]), k
+        assert same_bits(getattr(s, k).cpu().numpy(), e["out_" + k]), k
     if len(c["freq"]):
         for k in ("r_pt", "i_pt", "r_in", "i_in"):
-            assert np.array_equal(getattr(s.ft, k).cpu().numpy(), e["out_ft_" + k]), k
+            assert same_bits(getattr(s.ft, k).cpu().numpy(), e["out_ft_" + k]), k
 
 
 @pytest.mark.parametrize("fused,chunks", [(True, (160,)), (True, (1, 63, 96)), (False, (160,))])
@@ -99,7 +100,7 @@ def test_fused_matches_per_step(gpu, nf, depth, cols):
         b.run(n)
     sa, sb = a.state(), b.state()
     for k in ("dz", "iz", "ez", "hx", "hy", "ihx", "ihy"):
-        assert np.array_equal(sa[k], sb[k]), k
+        assert same_bits(sa[k], sb[k]), k
     for k in ("ezi", "hxi", "bc"):
         assert torch.equal(getattr(a, k), getattr(b, k)), k
     assert torch.equal(a.tm.probe_out, b.tm.probe_out)
@@ -130,7 +131,7 @@ def test_fused_wide_grid(gpu):
         b.run(n)
     sa, sb = a.state(), b.state()
     for k in ("dz", "iz", "ez", "hx", "hy", "ihx", "ihy"):
-        assert np.array_equal(sa[k], sb[k]), k
+        assert same_bits(sa[k], sb[k]), k
     for k in ("ezi", "hxi", "bc"):
         assert torch.equal(getattr(a, k), getattr(b, k)), k
     for k in ("r_pt", "i_pt", "r_in", "i_in"):

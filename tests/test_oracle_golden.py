@@ -5,6 +5,7 @@ import pytest
 
 from cases import (DFORM_CASES, FD1D_CASES, FD2D_CASES, TFSF_CASES, dform_case, fd1d_case,
                    fd2d_case, tfsf_case)
+from conftest import same_bits
 
 
 def _wave(O):
@@ -32,14 +33,14 @@ def test_fd1d_oracle_matches_reference(oracle, name):
         snaps_ex.append(ex.copy())
         snaps_hy.append(hy.copy())
     e = c["exp"]
-    assert np.array_equal(ex, e["out_ex"])
-    assert np.array_equal(hy, e["out_hy"])
-    assert np.array_equal(np.concatenate(probes, axis=1), e["out_probe"])
+    assert same_bits(ex, e["out_ex"])
+    assert same_bits(hy, e["out_hy"])
+    assert same_bits(np.concatenate(probes, axis=1), e["out_probe"])
     if c["abc"]:
-        assert np.array_equal(bc, e["out_bc"])
+        assert same_bits(bc, e["out_bc"])
     if c["snap_t"]:
-        assert np.array_equal(np.array(snaps_ex), e["out_snap_ex"])
-        assert np.array_equal(np.array(snaps_hy), e["out_snap_hy"])
+        assert same_bits(np.array(snaps_ex), e["out_snap_ex"])
+        assert same_bits(np.array(snaps_hy), e["out_snap_hy"])
     # the case is not trivially zero: the pulse is on the grid or passed the probe
     assert max(np.abs(e["out_ex"]).max(), np.abs(e["out_probe"]).max()) > 0.1
 
@@ -53,8 +54,8 @@ def test_fd2d_oracle_matches_reference(oracle, name):
     e = c["exp"]
     for k in ("dz", "ez", "hx", "hy", "iz", "ihx", "ihy"):
         if "out_" + k in e:
-            assert np.array_equal(getattr(st, k), e["out_" + k]), k
-    assert np.array_equal(pr, e["out_probe"])
+            assert same_bits(getattr(st, k), e["out_" + k]), k
+    assert same_bits(pr, e["out_probe"])
     assert np.abs(st.ez).max() > 1e-4
 
 
@@ -65,8 +66,8 @@ def test_dielectric_integer_subsampling_matches_reference(oracle):
     m, a = golden("fd2d_dielectric_c")
     cyl = [(m["nx"] / 2 - 1, m["ny"] / 2 - 1, m["rgrid"], m["epsr"], m["sigma"])]
     naz, nbz = oracle.fd2d_dielectric(m["nx"], m["ny"], m["npml"], m["dt"], cyl, int_sub=True)
-    assert np.array_equal(naz, a["out_naz"])
-    assert np.array_equal(nbz, a["out_nbz"])
+    assert same_bits(naz, a["out_naz"])
+    assert same_bits(nbz, a["out_nbz"])
     assert (a["out_nbz"] > 0).sum() > 100
 
 
@@ -83,7 +84,7 @@ def test_pmlparam_matches_reference_after_fp32_storage(oracle, npml):
     for k in oracle.PML_FIELDS:
         ref64 = a[f"out_fd2d_3_2_{npml}_{k}"]
         ref32 = a[f"out_test_3_2_{npml}_{k}"]
-        assert np.array_equal(p[k].astype(np.float32), ref64.astype(np.float32)), k
+        assert same_bits(p[k].astype(np.float32), ref64.astype(np.float32)), k
         ulp = np.spacing(np.abs(ref64))
         assert np.all(np.abs(p[k] - ref64) <= 2 * ulp), k
         assert np.allclose(p[k].astype(np.float32), ref32, rtol=2e-7, atol=0), k
@@ -107,10 +108,10 @@ def test_tfsf_oracle_matches_reference(oracle, name):
     for k, v in list(st.items()) + list(inc.items()):
         if c["nbz"] is None and k == "iz":
             continue
-        assert np.array_equal(v, e["out_" + k]), k
+        assert same_bits(v, e["out_" + k]), k
     for k, v in ft.items():
         if nf:
-            assert np.array_equal(v, e["out_ft_" + k]), k
+            assert same_bits(v, e["out_ft_" + k]), k
     # the plane wave crossed the box and the cylinder scattered it
     assert np.abs(e["out_ez"]).max() > 0.1
     if nf:
@@ -128,7 +129,7 @@ def test_dform_oracle_matches_reference(oracle, name):
     md = NO.dielectric_dform(nx, c["dt"], m["chi"], m["tau"], m["epsr"], m["sigma"], c["debye"], dt)
     e = c["exp"]
     for k, v in md.items():
-        assert np.array_equal(v, e["out_md_" + k]), k
+        assert same_bits(v, e["out_md_" + k]), k
     st = {k: np.zeros(nx, dt) for k in ("dx", "ix", "sx", "ex", "hy")}
     st["bc"] = np.zeros(4, dt)
     nf = len(c["freq"])
@@ -137,8 +138,8 @@ def test_dform_oracle_matches_reference(oracle, name):
     pr = NO.dform_run(st, md, c["ns"], c["wave"], c["freq"], c["dt"], ft, src_pos=c["src_pos"],
                       probes=c["probes"])
     for k in ("dx", "ix", "ex", "hy", "bc") + (("sx",) if c["debye"] else ()):
-        assert np.array_equal(st[k], e["out_" + k]), k
+        assert same_bits(st[k], e["out_" + k]), k
     for k, v in ft.items():
-        assert np.array_equal(v, e["out_ft_" + k]), k
-    assert np.array_equal(pr, e["out_probe"])
+        assert same_bits(v, e["out_ft_" + k]), k
+    assert same_bits(pr, e["out_probe"])
     assert np.abs(e["out_ex"]).max() > 0.1
