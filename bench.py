@@ -345,7 +345,7 @@ def bench_batch(a):
     effective = B_ALG * cells * K / (ev_ms * 1e-3) / 1e9
     # a launch reads and writes every state array once whatever its number of
     # fused steps, so launches of 7 and 8 steps share the record of depth 8
-    key = f"fd2d_batch_c3_seed0_d{D}_r{b.desc.rows or 64}_c{b.desc.cols}"
+    key = f"fd2d_batch_c3_seed0_d{D}_r{b.desc.rows or 128}_c{b.desc.cols}"
     rec = {
         "metric": METRIC, "value": round(cells * K / wall / 1e6, 1), "unit": "Mcell-updates/s",
         "n_gpus": 1, "steps": K, "warmup": a.warmup, "ms_per_step": round(wall * 1e3 / K, 4),
@@ -355,7 +355,7 @@ def bench_batch(a):
                                "cylinders, npml=8, hard Gaussian source, persistent launches"
                                + (", one captured HIP graph" if graph else ", eager launches"),
                    "cells": cells, "grids": len(specs), "steps_per_launch": sorted(set(depths)),
-                   "rows": b.desc.rows or 64,
+                   "rows": b.desc.rows or 128,
                    "cols": b.desc.cols, "waves": b.desc.waves, "units": b.desc.nunits,
                    "launches": L},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK,
@@ -512,8 +512,12 @@ def bench_tfsf(a):
         "config": {"workload": f"2-D TM plane wave {n}x{n} FP32, npml={npml} PML, TF/SF box, "
                                f"cylinder r={rgrid} eps_r=30 sigma=0.3, DFT at 3 frequencies "
                                f"(fd2d/clang/test_3_4.c)", "grid": [n, n], "npml": npml,
-                   "freq": list(freq), "depth": s.tm.depth or (4 if len(freq) else 8),
-                   "rows": s.tm.rows or (max(4, min(512, n // 220)) if len(freq) else max(16, min(512, n // 220))),
+                   "freq": list(freq),
+                   # with DFT planes the fused run defaults to depth 4 and
+                   # rows max(4, n/220); without, to the plain TM plan
+                   "depth": s.tm.depth or (4 if len(freq) else s.tm.plan(K)["depth"]),
+                   "rows": s.tm.rows or (max(4, min(512, n // 220)) if len(freq)
+                                         else s.tm.plan(K)["rows"]),
                    "cols": s.tm.cols or 1,
                    "per_step_kernels_value": round(per_step, 1)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK,

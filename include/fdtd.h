@@ -317,8 +317,8 @@ typedef struct fd2d_batch_desc {
 } fd2d_batch_desc;
 
 /* Number of work units (and so ints/4 of units_dev) a batch needs for the
- * given depth (0 = 8), rows per block (0 = 64, or the whole grid when it has
- * fewer rows) and columns per lane (0 = 1).  A unit is 4 ints: grid, strip,
+ * given depth (0 = 8), rows per block (0 = 128, or the whole grid when it
+ * has fewer rows) and columns per lane (0 = 1).  A unit is 4 ints: grid, strip,
  * first row, one past the last row.                                       */
 int fd2d_batch_units(const fd2d_grid *grids, int ngrids, int depth, int rows, int cols,
                      int *nunits);
@@ -327,7 +327,9 @@ int fd2d_batch_units(const fd2d_grid *grids, int ngrids, int depth, int rows, in
  * into the caller's device buffers (asynchronously on stream; the host
  * arrays may be released on return) and fill *out.  units_dev holds
  * 4*max_units ints, grids_dev ngrids records.  waves = 0 picks the number of
- * waves the device keeps resident.                                          */
+ * waves the device keeps resident.  The units are ordered for the waves of
+ * the persistent launch (wave w takes units w, w + waves, ...): sorted by
+ * estimated cost and dealt out round by round in alternating directions.   */
 int fd2d_batch_upload(const fd2d_grid *grids, int ngrids, int depth, int rows, int cols,
                       int waves, fd2d_grid *grids_dev, int *units_dev, int max_units,
                       fd2d_batch_desc *out, void *stream);

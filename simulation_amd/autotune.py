@@ -40,13 +40,13 @@ def candidates(nx: int):
 
 
 def batch_candidates():
-    """(depth, rows, cols, waves) shapes tried for a batch: depths 8 and 10
-    with 48-80 rows (on MI355X the 256-grid batch of sides 128-1024 peaks
-    there, all within a few per cent), and two columns per lane at depth 8."""
-    for depth in (8, 10):
-        for rows in (48, 64, 80):
-            yield depth, rows, 1, 0
-    yield 8, 64, 2, 0
+    """(depth, rows, cols, waves) shapes tried for a batch: depth 8 with
+    96-160 rows and depth 10 with 128 (on MI355X the 256-grid batch of sides
+    128-1024 peaks at depth 8, 128 rows), and two columns per lane."""
+    for rows in (96, 128, 160):
+        yield 8, rows, 1, 0
+    yield 10, 128, 1, 0
+    yield 8, 128, 2, 0
 
 
 def _device():
