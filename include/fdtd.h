@@ -586,8 +586,9 @@ typedef struct fd2d_tfsf_desc {
 /* With work, the run is fused: per chunk of up to 64 steps the incident grid
  * advances in one workgroup and records what the box needs per step into
  * work; the temporally blocked launches (d's partner set and launch shape,
- * as fd2d_tm_run) apply the TF/SF corrections and accumulate the DFT (lossy
- * PML form, nf <= 3), with the same bits as the per-step path.              */
+ * as fd2d_tm_run, at most 8 steps per launch) apply the TF/SF corrections
+ * and accumulate the DFT (PML form, lossy or not, nf <= 3), with the same
+ * bits as the per-step path.                                                */
 size_t fd2d_tfsf_work_elems(int ny, int npml);
 int fd2d_tfsf_run(const fd2d_tm_desc *d, const fd2d_tfsf_desc *b, int t0, int nsteps,
                   void *stream);

@@ -402,8 +402,17 @@ struct MwParams {
 template <int K, bool LOSSY, bool PML>
 __global__ void __launch_bounds__(NT) tm_mw_kernel(const MwParams M)
 {
-    __shared__ Xch<K> X;
-    __shared__ CoefRing R;
+    // the interior workgroups' exchange buffers and coefficient rings, or the
+    // frame workgroups' single-wave profile rings
+    __shared__ union {
+        struct {
+            Xch<K> X;
+            CoefRing R;
+        } in;
+        tb::NaRing<K, 1> fr[W];
+    } U;
+    Xch<K> &X = U.in.X;
+    CoefRing &R = U.in.R;
     const TbParams &P = M.P;
     const int w = (int)threadIdx.x / 64, lane = (int)threadIdx.x % 64;
     if ((int)blockIdx.x < M.nfw) {
@@ -423,11 +432,7 @@ __global__ void __launch_bounds__(NT) tm_mw_kernel(const MwParams M)
         g.j0 = g.vj0 - P.halo + lane;
         g.i0 = M.band[b][0] + y * M.frows;
         g.i1 = min(M.band[b][1], g.i0 + M.frows);
-        // the frame workgroups use no coefficient ring of their own: its LDS
-        // holds the waves' single-wave profile rings
-        static_assert(sizeof(CoefRing) >= W * sizeof(tb::NaRing<K, 1>), "frame rings");
-        tb::sweep<K, 1, LOSSY, PML, false, true, true, true, 0, true>(
-            P, g, reinterpret_cast<tb::NaRing<K, 1> *>(&R)[w]);
+        tb::sweep<K, 1, LOSSY, PML, false, true, true, true, 0, true>(P, g, U.fr[w]);
         return;
     }
     // interior super-tile, XCD-aware order

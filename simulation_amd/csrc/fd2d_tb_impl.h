@@ -102,6 +102,11 @@ struct Carry {
 template <int K, int C>
 struct alignas(16) NaRing {
     float v[2 * K][64][2 * C];
+    // probe values of the sweep's K levels, written out after the sweep (a
+    // global store inside the sweep loop, behind a branch, would turn every
+    // iteration's wait for the next row's loads into a wait for all
+    // outstanding memory operations)
+    float probe[FDTD_MAX_PROBES][K];
 };
 
 struct Geom {
@@ -347,7 +352,10 @@ __device__ __forceinline__ void iterate(const PR &P, const Geom &g, const ColCoe
         }
         for (int k = 0; SP && k < P.nsrc; ++k) {
             if (r == P.src_i[k]) {
-                const float v = ldg(P.src_wave + (size_t)k * P.wave_len + (t - 1));
+                // a scalar load (the same address in every lane): no vector
+                // load behind a branch in the loop, so the waits for the
+                // prefetched row stay counted
+                const float v = ldu(P.src_wave + (size_t)k * P.wave_len, t - 1);
 #pragma unroll
                 for (int c = 0; c < C; ++c)
                     if (g.j0 + c == P.src_j[k]) Dn.dz[c] = v;
@@ -355,7 +363,7 @@ __device__ __forceinline__ void iterate(const PR &P, const Geom &g, const ColCoe
         }
         // plane wave: the TF/SF correction of dz on the box's columns
         // (fd2d/pycuda/fd2d_3_4.py:164-170), after the D update
-        if (SP && P.tf_npml && r >= P.tf_npml - 1 && r <= nx - P.tf_npml) {
+        if (SP && NF > 0 && P.tf_npml && r >= P.tf_npml - 1 && r <= nx - P.tf_npml) {
             const float *iv = P.inc + (size_t)(t - P.inc_t0 - 1) * P.inc_stride;
 #pragma unroll
             for (int c = 0; c < C; ++c) {
@@ -393,9 +401,7 @@ __device__ __forceinline__ void iterate(const PR &P, const Geom &g, const ColCoe
 #pragma unroll
                     for (int c = 0; c < C; ++c) {
                         const int j = g.j0 + c;
-                        if (j == P.probe_j[pr] && j >= g.vj0 && j < g.vj1)
-                            *(__attribute__((address_space(1))) float *)(P.probe_out +
-                                (size_t)pr * P.probe_len + (t - 1)) = En[c];
+                        if (j == P.probe_j[pr] && j >= g.vj0 && j < g.vj1) ring.probe[pr][s] = En[c];
                     }
                 }
         }
@@ -452,7 +458,7 @@ __device__ __forceinline__ void iterate(const PR &P, const Geom &g, const ColCoe
         }
         // plane wave: TF/SF corrections of hx on the box's side columns and
         // of hy on its top and bottom rows (fd2d/pycuda/fd2d_3_4.py:211-226)
-        if (SP && P.tf_npml) {
+        if (SP && NF > 0 && P.tf_npml) {
             const int a = P.tf_npml - 1, b = ny - P.tf_npml, qb = nx - P.tf_npml;
             const float *iv = P.inc + (size_t)(t - P.inc_t0 - 1) * P.inc_stride + 2 - a;
             if (q >= a && q <= qb) {
@@ -650,6 +656,24 @@ __device__ __forceinline__ void sweep(const PR &P, const Geom &g, NaRing<K, C> &
             if constexpr (NF > 0) F0 = F1;
         }
     }
+    if constexpr (SP) {
+        // the recorded probe values: probe pr's K values if its cell is one
+        // this wave owns (row in [i0, i1), column in [vj0, vj1)); lane s
+        // writes level s.  The fences make the owning lane's LDS writes
+        // visible to the other lanes of the wave.
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const int lane = (int)(threadIdx.x & 63u);
+        const int js = g.j0 - lane * C;           // first column of the strip
+        for (int pr = 0; pr < P.nprobe; ++pr) {
+            const int pi = P.probe_i[pr], pj = P.probe_j[pr];
+            if (pi >= g.i0 && pi < g.i1 && pj >= g.vj0 && pj < g.vj1 && pj >= js &&
+                pj < js + 64 * C && lane < K)
+                *(__attribute__((address_space(1))) float *)(P.probe_out + (size_t)pr * P.probe_len +
+                                                             (P.t_first + lane - 1)) =
+                    ring.probe[pr][lane];
+        }
+    }
 }
 
 // PART TB_WHOLE: one launch over every tile -- the listed boundary tiles
@@ -660,7 +684,8 @@ __device__ __forceinline__ void sweep(const PR &P, const Geom &g, NaRing<K, C> &
 // listed others, as two launches that write disjoint cells and may run
 // concurrently.  A wave returns at once if its tile is not its part's.
 template <int K, int C, bool LOSSY, bool PML, bool VEC, int PART, int NF = 0>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(C == 1 && K <= 10 ? 3 : 1)))
+__global__ void __launch_bounds__(64)
+__attribute__((amdgpu_waves_per_eu(C == 1 && K <= 10 && NF == 0 ? 3 : 1)))
 tm_tb_kernel(const TbParams P)
 {
     constexpr int S = 64 * C;
@@ -861,11 +886,20 @@ int launch_kv(const TbParams &p_in, bool lossy, bool pml, hipStream_t s)
         else hipLaunchKernelGGL((tm_tb_kernel<K, C, false, false, VEC, PART>), grid, dim3(64), 0, s, p);
         return hip_check(hipGetLastError(), "fd2d_tm_run launch");
     }
-    if (p.nf) {
-        if (!(lossy && pml) || p.nf > TB_MAX_FREQ)
-            return fail("fused running DFT: lossy PML form and at most %d frequencies", TB_MAX_FREQ);
-        hipLaunchKernelGGL((tm_tb_kernel<K, C, true, true, VEC, TB_WHOLE, TB_MAX_FREQ>), grid, dim3(64),
-                           0, s, p);
+    if (p.nf || p.tf_npml) {
+        // plane-wave (TF/SF) launches, with or without DFT planes, run the
+        // instantiation that carries the planes (nf = 0: none are read or
+        // written); the others have no TF/SF code at all, so the waits of
+        // their source and probe tiles stay counted
+        if (!pml || p.nf > TB_MAX_FREQ)
+            return fail("fused plane wave / running DFT: PML form and at most %d frequencies",
+                        TB_MAX_FREQ);
+        if (lossy)
+            hipLaunchKernelGGL((tm_tb_kernel<K, C, true, true, VEC, TB_WHOLE, TB_MAX_FREQ>), grid,
+                               dim3(64), 0, s, p);
+        else
+            hipLaunchKernelGGL((tm_tb_kernel<K, C, false, true, VEC, TB_WHOLE, TB_MAX_FREQ>), grid,
+                               dim3(64), 0, s, p);
         return hip_check(hipGetLastError(), "fd2d_tm_run launch");
     }
     if (lossy && pml) hipLaunchKernelGGL((tm_tb_kernel<K, C, true, true, VEC, TB_WHOLE>), grid, dim3(64), 0, s, p);
@@ -932,7 +966,7 @@ struct BatchParams {
 // many independent grids (fd2d_grid records in device memory); each unit is
 // swept exactly as the single-grid kernels sweep it.  Lossy PML form only.
 template <int K, int C, bool VEC>
-__global__ void __launch_bounds__(64)
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(C == 1 && K <= 8 ? 4 : 1)))
 tm_tb_batch_kernel(const fd2d_grid *__restrict__ grids, const int *__restrict__ units, int nunits,
                    int parity, int t_first, int write_ez, int halo)
 {

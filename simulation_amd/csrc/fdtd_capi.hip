@@ -930,10 +930,15 @@ static int tfsf_fused(const fd2d_tm_desc *d, const fd2d_tfsf_desc *b, int t0, in
     fd2d_tm_desc dd = *d;
     dd.form = FDTD_TM_FORM_STRIP;          // TF/SF corrections and DFT: strip form
     if (b->nf && !dd.depth) dd.depth = 4;
-    // the DFT bodies keep their level loop unrolled up to depth 8 only (past
-    // it the twiddle table is indexed at run time and the launch parameters
-    // go to scratch memory); the depth changes no result
-    if (b->nf && dd.depth > 8) dd.depth = 8;
+    // the plane-wave launches run the instantiation that carries the DFT
+    // planes, whose level loop stays unrolled up to depth 8 only (past it the
+    // twiddle table is indexed at run time and the launch parameters go to
+    // scratch memory); the depth changes no result
+    if (!dd.depth) {
+        int h;
+        tm_defaults(d->nx, &dd.depth, &h);
+    }
+    if (dd.depth > 8) dd.depth = 8;
     if (b->nf && !dd.rows) {
         const int h = d->nx / 220;
         dd.rows = h < 4 ? 4 : (h > 512 ? 512 : h);
@@ -1030,8 +1035,8 @@ int fd2d_tfsf_run(const fd2d_tm_desc *d, const fd2d_tfsf_desc *b, int t0, int ns
         if (b->work_elems < fd2d_tfsf_work_elems(d->ny, b->npml))
             return fail("fd2d_tfsf_run: work has %zu floats, %zu needed", b->work_elems,
                         fd2d_tfsf_work_elems(d->ny, b->npml));
-        if (b->nf > TB_MAX_FREQ || (b->nf && !(pml && d->md.nbz)))
-            return fail("fd2d_tfsf_run: the fused path takes the lossy PML form and at most %d "
+        if (b->nf > TB_MAX_FREQ || !pml)
+            return fail("fd2d_tfsf_run: the fused path takes the PML form and at most %d "
                         "frequencies (pass work = NULL for the per-step path)", TB_MAX_FREQ);
         if (!d->dz2 || !d->hx2 || !d->hy2 || (d->md.nbz && !d->iz2) || (pml && (!d->ihx2 || !d->ihy2)))
             return fail("fd2d_tfsf_run: the fused path needs the partner state set");

@@ -218,9 +218,9 @@ class PlaneWaveSolver:
         self.dt = float(time_step() if dt is None else dt)
         self.ft = new_ftrans(len(self.freq), nx, ny, dev) if len(self.freq) else None
         # fused: the temporally blocked launches with the incident table in
-        # work (they take the DFT in the lossy PML form, up to 3 frequencies)
+        # work (the PML form, lossy or not, up to 3 DFT frequencies)
         nf = len(self.freq)
-        fused = fused and (nf == 0 or (nf <= 3 and pml is not None and md.nbz is not None))
+        fused = fused and nf <= 3 and pml is not None
         n = _lib.lib().fd2d_tfsf_work_elems(int(ny), int(npml)) if fused else 0
         self.work = torch.empty(n, dtype=torch.float32, device=dev) if n else None
 
