@@ -83,6 +83,9 @@ def parse():
                          "a persistent sweep of the unit list; -1: one wave per unit)")
     ap.add_argument("--no-graph", action="store_true",
                     help="batch: time the eager launches instead of one captured graph launch")
+    ap.add_argument("--static-medium", action="store_true",
+                    help="tm/large: let run calls reuse the free-space tile flags of earlier "
+                         "calls (the medium does not change during the bench)")
     ap.add_argument("--no-tune", action="store_true",
                     help="tm/large: skip the autotuner (default: it chooses depth/rows/cols/form "
                          "before the warm-up, unless --depth/--rows/--cols/--form fix them)")
@@ -739,6 +742,7 @@ def main():
     cfg.depth, cfg.rows, cfg.cols = a.depth, a.rows, a.cols
     cfg.form = {"auto": 0, "strip": 1, "multiwave": 2}[a.form]
     s = fd2d.build_solver(cfg, dev)          # loads every code object (fdtd_preload)
+    s.static_medium = a.static_medium
     tuned = None
     fixed = a.depth or a.rows or a.cols or a.form != "auto"
     if not (a.no_tune or fixed):
@@ -794,6 +798,7 @@ def main():
             "max_depth": plan["depth"], "rows": plan["rows"], "cols": plan["cols"],
             "form": a.form,
             "launches": plan["launches"],
+            "static_medium": a.static_medium,
             "autotuned": tuned,
         },
         "roofline": {

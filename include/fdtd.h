@@ -39,7 +39,7 @@
 extern "C" {
 #endif
 
-#define FDTD_ABI_VERSION 2
+#define FDTD_ABI_VERSION 3
 
 #define FDTD_OK 0
 #define FDTD_EINVAL (-1)
@@ -262,6 +262,19 @@ typedef struct fd2d_tm_desc {
     int depth, rows, cols;
     int band_rows[2], band_cols[2];
     int form;                     /* FDTD_TM_FORM_* */
+    /* Optional workspace of tile_elems >= fd2d_tm_tile_elems(d, nsteps)
+     * unsigned ints (NULL: none).  With it, the first fused launch of a
+     * call records which interior tiles read only naz = 1 and nbz = 0 (free
+     * space), and the call's later launches skip reading naz and nbz there
+     * (the values are the constants; every result bit is the same).
+     * tile_keep = 1 promises that naz and nbz hold what they held when the
+     * flags in tile_work were last recorded: a call with the same arrays,
+     * grid, rows, columns and a depth no deeper than the recording one then
+     * uses those flags from its first launch on.  tile_keep = 0: flags of
+     * earlier calls are never used.                                        */
+    unsigned *tile_work;
+    size_t tile_elems;
+    int tile_keep;
 } fd2d_tm_desc;
 
 #define FDTD_TM_FORM_AUTO 0
@@ -278,6 +291,11 @@ int fd2d_tm_run(const fd2d_tm_desc *d, int t0, int nsteps, void *stream);
  * to 12.  Host only.                                                       */
 int fd2d_tm_plan(const fd2d_tm_desc *d, int nsteps,
                  int *launches, int *depth, int *rows, int *cols);
+
+/* Elements of fd2d_tm_desc.tile_work the plan of (d, nsteps) uses: one per
+ * tile of the single-wave strip form (0 for the multi-wave form, which does
+ * not use it).  Host only.  No reference counterpart.                      */
+size_t fd2d_tm_tile_elems(const fd2d_tm_desc *d, int nsteps);
 
 /* ------------------------------------------------------ batched grids
  * Many independent TM grids of different sizes advanced together by one

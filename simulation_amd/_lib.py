@@ -62,6 +62,9 @@ class Fd2dTmDesc(C.Structure):
         ("depth", C.c_int), ("rows", C.c_int), ("cols", C.c_int),
         ("band_rows", C.c_int * 2), ("band_cols", C.c_int * 2),
         ("form", C.c_int),
+        ("tile_work", C.c_void_p),
+        ("tile_elems", C.c_size_t),
+        ("tile_keep", C.c_int),
     ]
 
 
@@ -174,6 +177,7 @@ SIGNATURES = {
                               _DP, _P]),
     "fd2d_tm_run": (_I, [C.POINTER(Fd2dTmDesc), _I, _I, _P]),
     "fd2d_tm_plan": (_I, [C.POINTER(Fd2dTmDesc), _I, _IP, _IP, _IP, _IP]),
+    "fd2d_tm_tile_elems": (C.c_size_t, [C.POINTER(Fd2dTmDesc), _I]),
     "fd2d_batch_units": (_I, [C.POINTER(Fd2dGrid), _I, _I, _I, _I, _IP]),
     "fd2d_batch_upload": (_I, [C.POINTER(Fd2dGrid), _I, _I, _I, _I, _I, _P, _P, _I,
                                C.POINTER(Fd2dBatchDesc), _P]),

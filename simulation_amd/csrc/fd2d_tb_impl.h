@@ -134,6 +134,10 @@ __device__ __forceinline__ float ldu(const float *base, int i)
 {
     return ((cfloat *)base)[__builtin_amdgcn_readfirstlane(i)];
 }
+__device__ __forceinline__ unsigned ldu(const unsigned *base, int i)
+{
+    return ((const __attribute__((address_space(4))) unsigned *)base)[__builtin_amdgcn_readfirstlane(i)];
+}
 __device__ __forceinline__ gfloat *at(const float *base, unsigned off)
 {
     const unsigned bytes = off << 2;                  // < 2^32: zero-extended 32-bit offset
@@ -213,18 +217,24 @@ __device__ __forceinline__ void st_rb(__amdgpu_buffer_rsrc_t rs, unsigned row, i
     }
 }
 
-template <int C, bool LOSSY, bool PML, bool VEC, bool CLAMP, class PR>
+// VAC: a free-space tile (every naz 1, nbz 0): neither array is read.
+template <int C, bool LOSSY, bool PML, bool VEC, bool CLAMP, bool VAC = false, class PR>
 __device__ __forceinline__ void load_row(const PR &P, int p, int j0, HRow<C> &H, DRow<C> &D)
 {
     const int pc = min(max(p, 0), P.nx - 1);     // rows outside the grid: any in-bounds row
     const unsigned row = (unsigned)pc * (unsigned)P.ny;
     ld<C, VEC, CLAMP>(P.dz_i, row, j0, P.ny, D.dz);
-    ld<C, VEC, CLAMP>(P.naz, row, j0, P.ny, D.na);
+    if (VAC) {
+#pragma unroll
+        for (int c = 0; c < C; ++c) { D.na[c] = 1.f; D.nb[c] = 0.f; }
+    } else {
+        ld<C, VEC, CLAMP>(P.naz, row, j0, P.ny, D.na);
+    }
     ld<C, VEC, CLAMP>(P.hx_i, row, j0, P.ny, H.hx);
     ld<C, VEC, CLAMP>(P.hy_i, row, j0, P.ny, H.hy);
     if (LOSSY) {
         ld<C, VEC, CLAMP>(P.iz_i, row, j0, P.ny, D.iz);
-        ld<C, VEC, CLAMP>(P.nbz, row, j0, P.ny, D.nb);
+        if (!VAC) ld<C, VEC, CLAMP>(P.nbz, row, j0, P.ny, D.nb);
     } else {
 #pragma unroll
         for (int c = 0; c < C; ++c) { D.iz[c] = 0.f; D.nb[c] = 0.f; }
@@ -265,19 +275,27 @@ __device__ __forceinline__ void col_coef(const PR &P, int j0, ColCoef<C> &cc)
 // BROW: rows may lie outside [1, nx-2] or outside the default band; BCOL: the
 // same for the strip's columns; SP: a source or probe lies in the rows and
 // columns the wave computes.
+// VAC: a free-space tile (naz 1, nbz 0 as constants, no profile ring);
+// REC: AND into vac whether the row's naz, nbz are exactly 1, 0.
 template <int K, int C, bool LOSSY, bool PML, bool VEC, bool BROW, bool BCOL, bool SP, int NF,
-          bool BS, bool PRO = false, class PR>
+          bool BS, bool PRO = false, bool VAC = false, bool REC = false, class PR>
 __device__ __forceinline__ void iterate(const PR &P, const Geom &g, const ColCoef<C> &cc,
                                         NaRing<K, C> &ring,
                                         int p, const HRow<C> &Hload, const DRow<C> &Dload,
                                         const Carry<C> (&Si)[K], Carry<C> (&So)[K],
                                         const FRow<C, NF> &Fload, const FRow<C, NF> (&SFi)[K],
-                                        FRow<C, NF> (&SFo)[K], int j = 0)
+                                        FRow<C, NF> (&SFo)[K], int j, bool &vac)
 {
     const int nx = P.nx, ny = P.ny;
     const unsigned lane = threadIdx.x & 63u;
     const unsigned slot = (unsigned)p % (unsigned)K;   // p >= 0
-    {
+    if constexpr (REC) {
+#pragma unroll
+        for (int c = 0; c < C; ++c)
+            vac = vac && __builtin_bit_cast(unsigned, Dload.na[c]) == 0x3F800000u &&
+                  (!LOSSY || __builtin_bit_cast(unsigned, Dload.nb[c]) == 0u);
+    }
+    if constexpr (!VAC) {
         // row p's profile values (level 0's E row) for the levels below
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
@@ -310,8 +328,8 @@ __device__ __forceinline__ void iterate(const PR &P, const Geom &g, const ColCoe
             for (int c = 0; c < C; ++c) {
                 Din.dz[c] = Si[s - 1].d.dz[c];
                 Din.iz[c] = Si[s - 1].d.iz[c];
-                Din.na[c] = rd[c];
-                Din.nb[c] = LOSSY ? rd[C + c] : 0.f;
+                Din.na[c] = VAC ? 1.f : rd[c];
+                Din.nb[c] = (LOSSY && !VAC) ? rd[C + c] : 0.f;
             }
             So[s - 1].d = Dpend;
             if constexpr (NF > 0) {
@@ -563,13 +581,16 @@ __device__ __forceinline__ void load_ft(const PR &P, int p, int j0, FRow<C, NF> 
     }
 }
 
+// Returns, with REC, whether every naz, nbz the sweep read is exactly 1, 0
+// (wave-uniform); otherwise false.
 template <int K, int C, bool LOSSY, bool PML, bool VEC, bool BROW, bool BCOL, bool SP, int NF = 0,
-          bool BS = false, bool PP = false, class PR>
-__device__ __forceinline__ void sweep(const PR &P, const Geom &g, NaRing<K, C> &ring)
+          bool BS = false, bool PP = false, bool VAC = false, bool REC = false, class PR>
+__device__ __forceinline__ bool sweep(const PR &P, const Geom &g, NaRing<K, C> &ring)
 {
+    bool vac = true;
     ColCoef<C> cc;
     col_coef<C, PML, BCOL>(P, g.j0, cc);
-    {
+    if constexpr (!VAC) {
         // rows above the sweep's first read zeros, as the carries start
         const unsigned lane = threadIdx.x & 63u;
 #pragma unroll
@@ -609,7 +630,7 @@ __device__ __forceinline__ void sweep(const PR &P, const Geom &g, NaRing<K, C> &
     HRow<C> H0, H1;
     DRow<C> D0, D1;
     FRow<C, NF> F0, F1;
-    load_row<C, LOSSY, PML, VEC, BCOL>(P, pbeg, g.j0, H0, D0);
+    load_row<C, LOSSY, PML, VEC, BCOL, VAC>(P, pbeg, g.j0, H0, D0);
     if constexpr (NF > 0) load_ft<C, NF, VEC, BCOL>(P, pbeg, g.j0, F0);
     // BS: the first row is needed at once; waiting for it outside the loop
     // keeps the loop head free of a wait that would also catch the stores of
@@ -624,33 +645,32 @@ __device__ __forceinline__ void sweep(const PR &P, const Geom &g, NaRing<K, C> &
         const int c0 = g.i0 - K;
 #pragma unroll 1
         for (; p < c0 + 2 * K - 1; ++p) {
-            load_row<C, LOSSY, PML, VEC, BCOL>(P, p + 1, g.j0, H1, D1);
-            iterate<K, C, LOSSY, PML, VEC, BROW, BCOL, SP, NF, BS, true>(
-                P, g, cc, ring, p, H0, D0, St[0], St[0], F0, SF[0], SF[0], p - c0);
+            load_row<C, LOSSY, PML, VEC, BCOL, VAC>(P, p + 1, g.j0, H1, D1);
+            iterate<K, C, LOSSY, PML, VEC, BROW, BCOL, SP, NF, BS, true, VAC, REC>(
+                P, g, cc, ring, p, H0, D0, St[0], St[0], F0, SF[0], SF[0], p - c0, vac);
             H0 = H1;
             D0 = D1;
         }
         // two rows per trip, so the row registers alternate as well
         for (; p < pend; p += 2) {
-            load_row<C, LOSSY, PML, VEC, BCOL>(P, p + 1, g.j0, H1, D1);
+            load_row<C, LOSSY, PML, VEC, BCOL, VAC>(P, p + 1, g.j0, H1, D1);
             if constexpr (NF > 0) load_ft<C, NF, VEC, BCOL>(P, p + 1, g.j0, F1);
-            iterate<K, C, LOSSY, PML, VEC, BROW, BCOL, SP, NF, BS>(P, g, cc, ring, p, H0, D0, St[0],
-                                                                   St[NS - 1], F0, SF[0], SF[NS - 1]);
-            load_row<C, LOSSY, PML, VEC, BCOL>(P, p + 2, g.j0, H0, D0);
+            iterate<K, C, LOSSY, PML, VEC, BROW, BCOL, SP, NF, BS, false, VAC, REC>(
+                P, g, cc, ring, p, H0, D0, St[0], St[NS - 1], F0, SF[0], SF[NS - 1], 0, vac);
+            load_row<C, LOSSY, PML, VEC, BCOL, VAC>(P, p + 2, g.j0, H0, D0);
             if constexpr (NF > 0) load_ft<C, NF, VEC, BCOL>(P, p + 2, g.j0, F0);
-            iterate<K, C, LOSSY, PML, VEC, BROW, BCOL, SP, NF, BS>(P, g, cc, ring, p + 1, H1, D1,
-                                                                   St[NS - 1], St[0], F1, SF[NS - 1],
-                                                                   SF[0]);
+            iterate<K, C, LOSSY, PML, VEC, BROW, BCOL, SP, NF, BS, false, VAC, REC>(
+                P, g, cc, ring, p + 1, H1, D1, St[NS - 1], St[0], F1, SF[NS - 1], SF[0], 0, vac);
         }
         if (p == pend)
-            iterate<K, C, LOSSY, PML, VEC, BROW, BCOL, SP, NF, BS>(P, g, cc, ring, p, H0, D0, St[0],
-                                                                   St[NS - 1], F0, SF[0], SF[NS - 1]);
+            iterate<K, C, LOSSY, PML, VEC, BROW, BCOL, SP, NF, BS, false, VAC, REC>(
+                P, g, cc, ring, p, H0, D0, St[0], St[NS - 1], F0, SF[0], SF[NS - 1], 0, vac);
     } else {
         for (int p = pbeg; p <= pend; ++p) {
-            load_row<C, LOSSY, PML, VEC, BCOL>(P, p + 1, g.j0, H1, D1);
+            load_row<C, LOSSY, PML, VEC, BCOL, VAC>(P, p + 1, g.j0, H1, D1);
             if constexpr (NF > 0) load_ft<C, NF, VEC, BCOL>(P, p + 1, g.j0, F1);
-            iterate<K, C, LOSSY, PML, VEC, BROW, BCOL, SP, NF, BS>(P, g, cc, ring, p, H0, D0, St[0], St[0],
-                                                                   F0, SF[0], SF[0]);
+            iterate<K, C, LOSSY, PML, VEC, BROW, BCOL, SP, NF, BS, false, VAC, REC>(
+                P, g, cc, ring, p, H0, D0, St[0], St[0], F0, SF[0], SF[0], 0, vac);
             H0 = H1;
             D0 = D1;
             if constexpr (NF > 0) F0 = F1;
@@ -674,6 +694,8 @@ __device__ __forceinline__ void sweep(const PR &P, const Geom &g, NaRing<K, C> &
                     ring.probe[pr][lane];
         }
     }
+    if constexpr (REC) return __builtin_amdgcn_ballot_w64(!vac) == 0;
+    return false;
 }
 
 // PART TB_WHOLE: one launch over every tile -- the listed boundary tiles
@@ -778,9 +800,31 @@ tm_tb_kernel(const TbParams P)
     }
     const bool interior = !brow && !bcol && !sp;
     __shared__ NaRing<K, C> ring;
+    // free-space tiles (tile index tid): recorded by a run call's first
+    // launch, read by the later ones.  The recording launch writes every
+    // tile's entry -- a tile that is not interior (at its depth) records "not
+    // free space" -- so a later launch of smaller depth, which may find more
+    // tiles interior, never reads an entry this recording did not write.
+    const unsigned tid = (unsigned)by * (unsigned)nsx + (unsigned)bx;
+    if constexpr (NF == 0 && PART == TB_WHOLE) {
+        if (P.vac_mode == 1 && !interior && bi < 0 && threadIdx.x == 0) P.tile_vac[tid] = P.vac_tag;
+    }
     if (interior) {
-        if (want_int)
-            sweep<K, C, LOSSY, PML, VEC, false, false, false, NF, NF == 0, NF == 0 && C == 1>(P, g, ring);
+        if (want_int) {
+            if constexpr (NF == 0 && PART == TB_WHOLE) {
+                if (P.vac_mode == 2 && ldu(P.tile_vac, (int)tid) == (P.vac_tag | 1u)) {
+                    sweep<K, C, LOSSY, PML, VEC, false, false, false, NF, true, C == 1, true>(P, g, ring);
+                } else {
+                    const bool v =
+                        sweep<K, C, LOSSY, PML, VEC, false, false, false, NF, true, C == 1, false, true>(
+                            P, g, ring);
+                    if (P.vac_mode == 1 && threadIdx.x == 0)
+                        P.tile_vac[tid] = P.vac_tag | (v ? 1u : 0u);
+                }
+            } else {
+                sweep<K, C, LOSSY, PML, VEC, false, false, false, NF, NF == 0, NF == 0 && C == 1>(P, g, ring);
+            }
+        }
     } else if (want_bnd) {
         if (!brow && !sp) sweep<K, C, LOSSY, PML, VEC, false, true, false, NF, NF == 0>(P, g, ring);
         else sweep<K, C, LOSSY, PML, VEC, true, true, true, NF, NF == 0>(P, g, ring);

@@ -9,7 +9,9 @@
 #include <cstdio>
 #include <cstring>
 #include <algorithm>
+#include <atomic>
 #include <climits>
+#include <mutex>
 #include <vector>
 
 #include "fdtd_internal.h"
@@ -595,6 +597,49 @@ int fd2d_tm_plan(const fd2d_tm_desc *d, int nsteps, int *launches, int *depth, i
     return 0;
 }
 
+// Free-space tile flags recorded per workspace (host side): the tag they were
+// written with and what they describe.
+struct TileRecord {
+    const unsigned *work;
+    const float *naz, *nbz;
+    int nx, ny, rows, cols, halo, depth;
+    unsigned tag;
+    bool same_tiles(const TileRecord &o) const
+    {
+        return naz == o.naz && nbz == o.nbz && nx == o.nx && ny == o.ny && rows == o.rows &&
+               cols == o.cols && halo == o.halo;
+    }
+};
+
+static std::mutex &tile_records_mutex()
+{
+    static std::mutex m;
+    return m;
+}
+
+static std::vector<TileRecord> &tile_records()
+{
+    static std::vector<TileRecord> v;
+    return v;
+}
+
+// Tiles of the single-wave strip form of a plan: strips x row blocks.
+static size_t tm_tiles(const fd2d_tm_desc *d, const Plan &pl)
+{
+    if (pl.mw) return 0;
+    const int w = 64 * pl.cols - 2 * pl.halo;
+    return (size_t)((d->ny + w - 1) / w) * (size_t)((d->nx + pl.rows - 1) / pl.rows);
+}
+
+size_t fd2d_tm_tile_elems(const fd2d_tm_desc *d, int nsteps)
+{
+    clear_error();
+    if (tm_validate(d, 0, 0)) return 0;
+    Plan pl;
+    if (plan_tm(d, nsteps, &pl)) return 0;
+    return tm_tiles(d, pl);
+}
+
 int fd2d_tm_run(const fd2d_tm_desc *d, int t0, int nsteps, void *stream)
 {
     clear_error();
@@ -636,9 +681,38 @@ int fd2d_tm_run(const fd2d_tm_desc *d, int t0, int nsteps, void *stream)
     float *A[6] = {d->dz, d->iz, d->hx, d->hy, d->ihx, d->ihy};
     float *B[6] = {d->dz2, d->iz2, d->hx2, d->hy2, d->ihx2, d->ihy2};
     const int L = pl.launches;
+    // free-space tiles: the first launch (the deepest: its tiles read every
+    // row a later launch's tiles read) records them, the others use them; a
+    // fresh tag per recording keeps flags of other calls from matching,
+    // unless the caller promises an unchanged medium (tile_keep) and the
+    // flags were recorded with this workspace, arrays and tile geometry at
+    // a depth at least this call's
+    const bool vac = d->tile_work && !pl.mw && L >= 2 && d->tile_elems >= tm_tiles(d, pl);
+    bool reuse = false;
+    if (vac) {
+        const int kmax = nfused / L + (nfused % L ? 1 : 0);
+        TileRecord now{d->tile_work, d->md.naz, d->md.nbz, d->nx, d->ny, pl.rows, pl.cols, pl.halo,
+                       kmax, 0u};
+        std::lock_guard<std::mutex> g(tile_records_mutex());
+        auto &recs = tile_records();
+        auto it = std::find_if(recs.begin(), recs.end(),
+                               [&](const TileRecord &r) { return r.work == d->tile_work; });
+        if (d->tile_keep && it != recs.end() && it->same_tiles(now) && it->depth >= kmax) {
+            reuse = true;
+            p.vac_tag = it->tag;
+        } else {
+            static std::atomic<unsigned> calls{0};
+            now.tag = (calls.fetch_add(1) + 1u) << 1;
+            p.vac_tag = now.tag;
+            if (it != recs.end()) *it = now;
+            else recs.push_back(now);
+        }
+        p.tile_vac = d->tile_work;
+    }
     int done = 0;
     for (int l = 0; l < L; ++l) {
         const int k = nfused / L + (l < nfused % L ? 1 : 0);
+        p.vac_mode = vac ? ((l == 0 && !reuse) ? 1 : 2) : 0;
         float **in = (l % 2 == 0) ? A : B;
         float **out = (l % 2 == 0) ? B : A;
         p.dz_i = in[0]; p.iz_i = in[1]; p.hx_i = in[2]; p.hy_i = in[3]; p.ihx_i = in[4]; p.ihy_i = in[5];

@@ -102,6 +102,14 @@ struct TbParams {
     // incident ez over the box columns npml-1 .. ny-npml
     int tf_npml, inc_t0, inc_stride;
     const float *inc;
+    // free-space tiles (fd2d_tm_desc.tile_work): vac_mode 0 off; 1 the
+    // interior tiles record in tile_vac[tile] whether every naz, nbz they
+    // read is 1, 0 (as vac_tag | 1, else vac_tag); 2 interior tiles whose
+    // entry reads vac_tag | 1 read neither array.  vac_tag changes with every
+    // run call (bit 0 clear), so a flag from another call never matches.
+    unsigned *tile_vac;
+    int vac_mode;
+    unsigned vac_tag;
     // running DFT of ez into nf <= TB_MAX_FREQ planes (0: none); tw_c/tw_s
     // [s][f]: twiddles of level s of the launch
     int nf;

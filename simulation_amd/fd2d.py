@@ -301,6 +301,12 @@ class TMSolver:
                           if probes else None)
         self.depth, self.rows, self.cols = depth, rows, cols
         self.form = int(form)
+        # free-space tiles skip reading naz/nbz after a run call's first launch;
+        # static_medium: the caller promises naz/nbz do not change between run
+        # calls, so the flags of an earlier call serve from the first launch
+        self.tile_flags = True
+        self.static_medium = False
+        self._tiles = None
         self.band_rows, self.band_cols = self.default_band()
         self.t = 0
         self._work = torch.empty(_lib.ENERGY_WORK, dtype=torch.float64, device=dev)
@@ -363,8 +369,17 @@ class TMSolver:
                     cols=out[3].value)
 
     def run(self, nsteps: int) -> None:
-        """Advance by nsteps (asynchronous on the current stream)."""
+        """Advance by nsteps (asynchronous on the current stream).  The
+        library records the free-space tiles of the plan in a workspace the
+        solver keeps (fd2d_tm_desc.tile_work; one int per tile)."""
         d = self.desc()
+        if self.tile_flags:
+            n = _lib.lib().fd2d_tm_tile_elems(C.byref(d), int(nsteps))
+            if n and (self._tiles is None or self._tiles.numel() < n):
+                self._tiles = torch.zeros(n, dtype=torch.int32, device=self.device)
+            if n:
+                d.tile_work, d.tile_elems = self._tiles.data_ptr(), self._tiles.numel()
+                d.tile_keep = 1 if self.static_medium else 0
         call("fd2d_tm_run", C.byref(d), int(self.t), int(nsteps), stream())
         self.t += int(nsteps)
 
@@ -477,6 +492,7 @@ def simulate(cfg: TMConfig, device=None, snapshot_fields=("ez", "hy"), out_dir=N
     probe traces as probes.csv (simulation_amd.store)."""
     from . import store
     s = build_solver(cfg, device)
+    s.static_medium = True          # the medium stays as built for the whole run
     snaps = {k: [] for k in snapshot_fields}
     snap_t = []
     seg = cfg.snap_every or cfg.ns

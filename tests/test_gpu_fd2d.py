@@ -215,3 +215,56 @@ def test_many_sources_and_probes_tile_lists(gpu, rows):
         assert same_bits(sa[k], sb[k]), k
     assert same_bits(a.probe_traces(), b.probe_traces())
     assert np.abs(sa["ez"]).max() > 0.1
+
+
+def test_free_space_tiles_same_bits_and_no_stale_flags(gpu):
+    """Runs that skip naz/nbz in free-space tiles (the tile workspace) give
+    the bits of runs that read them; a medium changed between two run calls
+    is seen by the second call (its flags are recorded afresh)."""
+    from simulation_amd import fd2d
+    n = 384
+    cfg = fd2d.c2_config(n, 200)
+    cfg.depth, cfg.rows = 6, 16
+    print("[fd2d] free-space tiles", flush=True)
+    a = fd2d.build_solver(cfg, gpu)
+    b = fd2d.build_solver(cfg, gpu)
+    b.tile_flags = False
+    for s in (a, b):
+        s.run(80)
+    assert a._tiles is not None and b._tiles is None
+    sa, sb = a.state(), b.state()
+    for k in ("dz", "iz", "ez", "hx", "hy", "ihx", "ihy"):
+        assert same_bits(sa[k], sb[k]), k
+    # a lossy patch appears in what was free space (the cylinder has radius
+    # 38 around (191, 191); the pulse reaches the patch within the 80 steps)
+    for s in (a, b):
+        s.md.naz[180:200, 232:252] = 0.5
+        s.md.nbz[180:200, 232:252] = 0.01
+        s.run(80)
+    sa, sb = a.state(), b.state()
+    for k in ("dz", "iz", "ez", "hx", "hy", "ihx", "ihy"):
+        assert same_bits(sa[k], sb[k]), k
+    assert same_bits(a.probe_traces(), b.probe_traces())
+    assert np.abs(sa["ez"][180:200, 232:252]).max() > 1e-6
+
+
+def test_free_space_flags_kept_across_calls(gpu):
+    """With static_medium the flags recorded by one call serve the next
+    calls (same arrays and tiles, depth no deeper); a deeper call records
+    afresh.  Same bits as a run that reads every naz/nbz."""
+    from simulation_amd import fd2d
+    n = 320
+    cfg = fd2d.c2_config(n, 300)
+    cfg.depth, cfg.rows = 6, 20
+    print("[fd2d] free-space flags across calls", flush=True)
+    a = fd2d.build_solver(cfg, gpu)
+    a.static_medium = True
+    b = fd2d.build_solver(cfg, gpu)
+    b.tile_flags = False
+    for chunk in (25, 60, 12, 31, 2, 70):     # launch depths 5, 6, 6, 6 or 5, 1, 6
+        a.run(chunk)
+        b.run(chunk)
+    sa, sb = a.state(), b.state()
+    for k in ("dz", "iz", "ez", "hx", "hy", "ihx", "ihy"):
+        assert same_bits(sa[k], sb[k]), k
+    assert same_bits(a.probe_traces(), b.probe_traces())
