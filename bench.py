@@ -83,15 +83,20 @@ def parse():
                          "a persistent sweep of the unit list; -1: one wave per unit)")
     ap.add_argument("--no-graph", action="store_true",
                     help="batch: time the eager launches instead of one captured graph launch")
-    ap.add_argument("--static-medium", action="store_true",
-                    help="tm/large: let run calls reuse the free-space tile flags of earlier "
-                         "calls (the medium does not change during the bench)")
+    ap.add_argument("--fresh-medium", action="store_true",
+                    help="tm/large: every run call records the free-space tile flags afresh "
+                         "(default: the medium is built once and never changes, as in the "
+                         "reference program, so calls reuse flags an earlier call of the same "
+                         "launch shape recorded)")
     ap.add_argument("--no-tune", action="store_true",
                     help="tm/large: skip the autotuner (default: it chooses depth/rows/cols/form "
                          "before the warm-up, unless --depth/--rows/--cols/--form fix them)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a plain 1 GiB device copy three times (rocprofv3 FETCH_SIZE/WRITE_SIZE "
                          "scale) and exit")
+    ap.add_argument("--traffic-last", type=int, default=0,
+                    help="record-traffic: only the last N dispatches of the kernel instance "
+                         "(the timed launches of an autotuned run)")
     ap.add_argument("--record-traffic", nargs=6, metavar=("KEY", "KERNEL", "FETCH_CSV", "WRITE_CSV",
                                                           "COPY_FETCH_CSV", "COPY_WRITE_CSV"),
                     help="host only: HBM bytes per launch of KERNEL from rocprofv3 --pmc "
@@ -221,11 +226,13 @@ def _counter_rows(path, name_part=None):
     return out
 
 
-def record_traffic(key, kernel, fetch_csv, write_csv, cfetch_csv, cwrite_csv) -> dict:
+def record_traffic(key, kernel, fetch_csv, write_csv, cfetch_csv, cwrite_csv, last=0) -> dict:
     """Bytes per launch of `kernel` (every dispatch whose name contains it;
     a launch of the split TM path is one interior + one boundary dispatch)
     from FETCH_SIZE and WRITE_SIZE (KiB per dispatch), each scaled by what
-    the counter reads for the largest copy dispatch of a 1 GiB copy."""
+    the counter reads for the largest copy dispatch of a 1 GiB copy.
+    last > 0: only the last `last` dispatches of each part (the timed
+    launches of a run whose autotuner dispatched the same instance first)."""
     cf = max(v for _, _, v in _counter_rows(cfetch_csv, "copy"))
     cw = max(v for _, _, v in _counter_rows(cwrite_csv, "copy"))
     sf, sw = 2.0 ** 30 / (cf * 1024), 2.0 ** 30 / (cw * 1024)
@@ -238,6 +245,16 @@ def record_traffic(key, kernel, fetch_csv, write_csv, cfetch_csv, cwrite_csv) ->
         count[n] = count.get(n, 0) + 1
     top = max(count.values())
     parts = sorted(n for n, c in count.items() if c == top)
+    if last:
+        if last > top:
+            raise SystemExit(f"record-traffic: {top} dispatches of the instance, {last} asked")
+
+        def tail(rows):
+            keep = []
+            for p in parts:
+                keep += sorted(r for r in rows if r[1] == p)[-last:]
+            return keep
+        fr, wr, top = tail(fr), tail(wr), last
 
     def per_launch(rows, scale):
         return sum(sum(v for _, n, v in rows if n == p) / top for p in parts) * 1024 * scale
@@ -708,7 +725,7 @@ def cpu_baseline_tfsf(n: int, budget_s: float, npml: int, rgrid: int, freq) -> d
 def main():
     a = parse()
     if a.record_traffic:
-        record_traffic(*a.record_traffic)
+        record_traffic(*a.record_traffic, last=a.traffic_last)
         return
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus N>1 is not supported "
@@ -742,7 +759,7 @@ def main():
     cfg.depth, cfg.rows, cfg.cols = a.depth, a.rows, a.cols
     cfg.form = {"auto": 0, "strip": 1, "multiwave": 2}[a.form]
     s = fd2d.build_solver(cfg, dev)          # loads every code object (fdtd_preload)
-    s.static_medium = a.static_medium
+    s.static_medium = not a.fresh_medium
     tuned = None
     fixed = a.depth or a.rows or a.cols or a.form != "auto"
     if not (a.no_tune or fixed):
@@ -798,7 +815,7 @@ def main():
             "max_depth": plan["depth"], "rows": plan["rows"], "cols": plan["cols"],
             "form": a.form,
             "launches": plan["launches"],
-            "static_medium": a.static_medium,
+            "static_medium": not a.fresh_medium,
             "autotuned": tuned,
         },
         "roofline": {

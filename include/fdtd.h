@@ -270,8 +270,10 @@ typedef struct fd2d_tm_desc {
      * tile_keep = 1 promises that naz and nbz hold what they held when the
      * flags in tile_work were last recorded: a call with the same arrays,
      * grid, rows, columns and a depth no deeper than the recording one then
-     * uses those flags from its first launch on.  tile_keep = 0: flags of
-     * earlier calls are never used.                                        */
+     * uses those flags from its first launch on (new arrays that reuse the
+     * memory of freed ones break that promise: pass tile_keep = 0 on the
+     * first call of a new medium).  tile_keep = 0: flags of earlier calls
+     * are never used.                                                      */
     unsigned *tile_work;
     size_t tile_elems;
     int tile_keep;

@@ -303,10 +303,16 @@ class TMSolver:
         self.form = int(form)
         # free-space tiles skip reading naz/nbz after a run call's first launch;
         # static_medium: the caller promises naz/nbz do not change between run
-        # calls, so the flags of an earlier call serve from the first launch
+        # calls, so the flags of an earlier call serve from the first launch.
+        # One workspace per tile geometry (depth, rows, cols) and per depth of
+        # the call's first launch (the launch that records reads the rows of
+        # that depth): calls of other lengths (a short warm-up, a tail) keep
+        # their own flags and leave the others' alone.  A workspace's first
+        # call always records, so flags of freed arrays whose memory a new
+        # medium reuses never match.
         self.tile_flags = True
         self.static_medium = False
-        self._tiles = None
+        self._tiles = {}
         self.band_rows, self.band_cols = self.default_band()
         self.t = 0
         self._work = torch.empty(_lib.ENERGY_WORK, dtype=torch.float64, device=dev)
@@ -373,13 +379,19 @@ class TMSolver:
         library records the free-space tiles of the plan in a workspace the
         solver keeps (fd2d_tm_desc.tile_work; one int per tile)."""
         d = self.desc()
-        if self.tile_flags:
+        if self.tile_flags and nsteps > 0:
             n = _lib.lib().fd2d_tm_tile_elems(C.byref(d), int(nsteps))
-            if n and (self._tiles is None or self._tiles.numel() < n):
-                self._tiles = torch.zeros(n, dtype=torch.int32, device=self.device)
             if n:
-                d.tile_work, d.tile_elems = self._tiles.data_ptr(), self._tiles.numel()
-                d.tile_keep = 1 if self.static_medium else 0
+                out = [C.c_int() for _ in range(4)]
+                call("fd2d_tm_plan", C.byref(d), int(nsteps), *[C.byref(o) for o in out])
+                L = max(out[0].value, 1)
+                key = (out[1].value, out[2].value, out[3].value, -(-int(nsteps) // L))
+                ws = self._tiles.get(key)
+                first = ws is None or ws.numel() < n
+                if first:
+                    ws = self._tiles[key] = torch.zeros(n, dtype=torch.int32, device=self.device)
+                d.tile_work, d.tile_elems = ws.data_ptr(), ws.numel()
+                d.tile_keep = 1 if self.static_medium and not first else 0
         call("fd2d_tm_run", C.byref(d), int(self.t), int(nsteps), stream())
         self.t += int(nsteps)
 

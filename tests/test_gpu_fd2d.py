@@ -231,7 +231,7 @@ def test_free_space_tiles_same_bits_and_no_stale_flags(gpu):
     b.tile_flags = False
     for s in (a, b):
         s.run(80)
-    assert a._tiles is not None and b._tiles is None
+    assert a._tiles and not b._tiles
     sa, sb = a.state(), b.state()
     for k in ("dz", "iz", "ez", "hx", "hy", "ihx", "ihy"):
         assert same_bits(sa[k], sb[k]), k
@@ -268,3 +268,40 @@ def test_free_space_flags_kept_across_calls(gpu):
     for k in ("dz", "iz", "ez", "hx", "hy", "ihx", "ihy"):
         assert same_bits(sa[k], sb[k]), k
     assert same_bits(a.probe_traces(), b.probe_traces())
+
+
+def test_free_space_flags_new_medium_in_freed_memory(gpu):
+    """A solver built after another was freed (its arrays, and so perhaps the
+    same memory, now hold another medium) records its own flags before it
+    keeps any: same bits as a run that reads every naz/nbz.  Calls of other
+    lengths in between (a warm-up of 5 steps) keep their own flags."""
+    import gc
+    import torch
+    from simulation_amd import fd2d
+    n = 288
+    cfg = fd2d.c2_config(n, 200)
+    cfg.depth, cfg.rows = 6, 12
+    print("[fd2d] free-space flags, new medium in freed memory", flush=True)
+    a = fd2d.build_solver(cfg, gpu)
+    a.static_medium = True
+    a.run(36)
+    a.run(36)
+    torch.cuda.synchronize()
+    del a
+    gc.collect()
+    cfg2 = fd2d.c2_config(n, 200)
+    cfg2.depth, cfg2.rows = 6, 12
+    cfg2.cylinders = [(110.0, 180.0, 20, 3.0, 0.03), (180.0, 100.0, 15, 2.0, 0.01)]
+    b = fd2d.build_solver(cfg2, gpu)
+    b.static_medium = True
+    c = fd2d.build_solver(cfg2, gpu)
+    c.tile_flags = False
+    for chunk in (36, 5, 36, 24):
+        b.run(chunk)
+        c.run(chunk)
+    assert len(b._tiles) == 2
+    sb, sc = b.state(), c.state()
+    for k in ("dz", "iz", "ez", "hx", "hy", "ihx", "ihy"):
+        assert same_bits(sb[k], sc[k]), k
+    assert same_bits(b.probe_traces(), c.probe_traces())
+    assert np.abs(sb["ez"][95:125, 165:195]).max() > 1e-6
