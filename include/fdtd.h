@@ -39,7 +39,7 @@
 extern "C" {
 #endif
 
-#define FDTD_ABI_VERSION 3
+#define FDTD_ABI_VERSION 4
 
 #define FDTD_OK 0
 #define FDTD_EINVAL (-1)
@@ -334,6 +334,16 @@ typedef struct fd2d_batch_desc {
                                      (INT_MAX: none), set by fd2d_batch_upload */
     int probe_len;                /* smallest probe_len over grids with a probe
                                      (INT_MAX: none), set by fd2d_batch_upload */
+    unsigned *unit_work;          /* optional workspace of nunits unsigned ints
+                                     (NULL: none; fd2d_batch_upload sets NULL).
+                                     With it, the first launch of a run (or of
+                                     each graph launch) records which interior
+                                     units read only naz = 1 and nbz = 0, and
+                                     the later launches skip reading naz and
+                                     nbz there; same results.  The flags are
+                                     recorded afresh by every run and graph
+                                     launch, so a medium may change between
+                                     them. */
 } fd2d_batch_desc;
 
 /* Number of work units (and so ints/4 of units_dev) a batch needs for the

@@ -137,6 +137,7 @@ class Fd2dBatchDesc(C.Structure):
         ("nunits", C.c_int), ("depth", C.c_int), ("rows", C.c_int), ("cols", C.c_int),
         ("waves", C.c_int), ("vec", C.c_int),
         ("wave_len", C.c_int), ("probe_len", C.c_int),
+        ("unit_work", C.c_void_p),
     ]
 
 

@@ -70,7 +70,7 @@ class BatchSolver:
     """Allocates and advances a batch of grids (lossy PML form)."""
 
     def __init__(self, specs, nsteps_max: int, depth: int = 0, rows: int = 0, cols: int = 0,
-                 waves: int = 0, device=None):
+                 waves: int = 0, device=None, unit_flags: bool = True):
         dev = device or require_gpu()
         _lib.preload()
         self.device = dev
@@ -78,6 +78,11 @@ class BatchSolver:
         self.nsteps_max = int(nsteps_max)
         self.t = 0
         self._keep = []
+        # free-space units: the first launch of each run records which
+        # interior units read only naz = 1, nbz = 0, later launches skip both
+        # arrays there (fd2d_batch_desc.unit_work, one int per unit)
+        self.unit_flags = bool(unit_flags)
+        self._unit_work = None
         n = len(self.specs)
         self.grids = (_lib.Fd2dGrid * n)()
         self.state = []
@@ -133,6 +138,10 @@ class BatchSolver:
         call("fd2d_batch_upload", self.grids, n, depth, rows, cols, waves,
              self.grids_dev.data_ptr(), self.units.data_ptr(), nunits.value,
              C.byref(self.desc), stream())
+        self._unit_work = None
+        if self.unit_flags:
+            self._unit_work = torch.zeros(nunits.value, dtype=torch.int32, device=self.device)
+            self.desc.unit_work = self._unit_work.data_ptr()
 
     def reset(self) -> None:
         """All fields and probe traces back to zero, step counter to 0."""

@@ -82,12 +82,12 @@ int launch_fd2d_tb(const TbParams &p, int depth, int cols, bool lossy, bool pml,
 
 int launch_fd2d_batch(const fd2d_grid *grids, const int *units, int nunits, int waves, int depth,
                       int cols, bool vec, int parity, int t_first, int write_ez, int halo,
-                      hipStream_t s)
+                      BatchVac vac, hipStream_t s)
 {
 #define FDTD_BATCH_CASE(k)                                                                   \
     case k:                                                                                  \
         return launch_fd2d_batch_depth<k>(grids, units, nunits, waves, cols, vec, parity,    \
-                                          t_first, write_ez, halo, s);
+                                          t_first, write_ez, halo, vac, s);
     switch (depth) {
         FDTD_BATCH_CASE(1) FDTD_BATCH_CASE(2) FDTD_BATCH_CASE(3) FDTD_BATCH_CASE(4)
         FDTD_BATCH_CASE(5) FDTD_BATCH_CASE(6) FDTD_BATCH_CASE(7) FDTD_BATCH_CASE(8)

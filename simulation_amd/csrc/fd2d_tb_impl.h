@@ -973,6 +973,24 @@ int launch_part(const TbParams &p, bool lossy, bool pml, int part, hipStream_t s
     }
 }
 
+// Whether every naz, nbz of rows [r0, r1] on the wave's columns (j0 .. j0+C-1
+// per lane, inside the grid) is exactly 1, 0: bit patterns, wave-uniform.
+template <int C, class PR>
+__device__ __forceinline__ bool free_space(const PR &P, int r0, int r1, int j0)
+{
+    bool ok = true;
+#pragma unroll 4
+    for (int r = r0; r <= r1; ++r) {
+        const unsigned row = (unsigned)r * (unsigned)P.ny + (unsigned)j0;
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+            ok &= __float_as_uint(ldg(P.naz + row + c)) == 0x3f800000u;
+            ok &= __float_as_uint(ldg(P.nbz + row + c)) == 0u;
+        }
+    }
+    return __builtin_amdgcn_ballot_w64(!ok) == 0;
+}
+
 // ----------------------------------------------------------- batched grids
 // A unit's parameters.  The sweep reads the same fields as from TbParams, but
 // a batch grid has at most one source and one probe, so the lists are single
@@ -1012,7 +1030,7 @@ struct BatchParams {
 template <int K, int C, bool VEC>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(C == 1 && K <= 8 ? 4 : 1)))
 tm_tb_batch_kernel(const fd2d_grid *__restrict__ grids, const int *__restrict__ units, int nunits,
-                   int parity, int t_first, int write_ez, int halo)
+                   int parity, int t_first, int write_ez, int halo, BatchVac vac)
 {
     constexpr int S = 64 * C;
     __shared__ NaRing<K, C> ring;
@@ -1074,17 +1092,35 @@ tm_tb_batch_kernel(const fd2d_grid *__restrict__ grids, const int *__restrict__ 
         if (P.nprobe)
             sp |= P.probe_i.v >= g.i0 && P.probe_i.v < g.i1 && P.probe_j.v >= js &&
                   P.probe_j.v < js + S;
-        if (!brow && !bcol && !sp) sweep<K, C, true, true, VEC, false, false, false, 0, true, C == 1>(P, g, ring);
-        else sweep<K, C, true, true, VEC, true, true, true, 0, true>(P, g, ring);
+        // free-space units: the first launch of a run call (its deepest)
+        // records every unit's entry, the later ones read it.  An interior
+        // unit of the recording launch first reads its rows of naz, nbz
+        // (those the sweep reads and one more below) and, if they are free
+        // space, sweeps without them at once; its traffic is that of a
+        // plain sweep.  (Recording inside the sweep, as tm_tb_kernel does,
+        // would take registers this kernel does not have at 4 waves per
+        // SIMD.)
+        if (!brow && !bcol && !sp) {
+            bool v = vac.mode == 2 && ldu(vac.work, u) == (vac.tag | 1u);
+            if (vac.mode == 1) {
+                v = free_space<C>(P, max(g.i0 - K, 0), min(g.i1 + K, P.nx - 1), g.j0);
+                if (threadIdx.x == 0) vac.work[u] = vac.tag | (v ? 1u : 0u);
+            }
+            if (v) sweep<K, C, true, true, VEC, false, false, false, 0, true, C == 1, true>(P, g, ring);
+            else sweep<K, C, true, true, VEC, false, false, false, 0, true, C == 1>(P, g, ring);
+        } else {
+            if (vac.mode == 1 && threadIdx.x == 0) vac.work[u] = vac.tag;
+            sweep<K, C, true, true, VEC, true, true, true, 0, true>(P, g, ring);
+        }
     }
 }
 
 template <int K, int C, bool VEC>
 int launch_batch_kv(const fd2d_grid *grids, const int *units, int nunits, int waves, int parity,
-                    int t_first, int write_ez, int halo, hipStream_t s)
+                    int t_first, int write_ez, int halo, BatchVac vac, hipStream_t s)
 {
     hipLaunchKernelGGL((tm_tb_batch_kernel<K, C, VEC>), dim3(waves), dim3(64), 0, s, grids, units,
-                       nunits, parity, t_first, write_ez, halo);
+                       nunits, parity, t_first, write_ez, halo, vac);
     return hip_check(hipGetLastError(), "fd2d_batch_run launch");
 }
 
@@ -1107,14 +1143,17 @@ int batch_resident_kv(int *waves)
 template <int K>
 int launch_fd2d_batch_depth(const fd2d_grid *grids, const int *units, int nunits, int waves,
                             int cols, bool vec, int parity, int t_first, int write_ez, int halo,
-                            hipStream_t s)
+                            BatchVac vac, hipStream_t s)
 {
     if (cols == 2 && vec)
-        return tb::launch_batch_kv<K, 2, true>(grids, units, nunits, waves, parity, t_first, write_ez, halo, s);
+        return tb::launch_batch_kv<K, 2, true>(grids, units, nunits, waves, parity, t_first, write_ez, halo,
+                                               vac, s);
     if (cols == 2)
-        return tb::launch_batch_kv<K, 2, false>(grids, units, nunits, waves, parity, t_first, write_ez, halo, s);
+        return tb::launch_batch_kv<K, 2, false>(grids, units, nunits, waves, parity, t_first, write_ez,
+                                                halo, vac, s);
     if (cols == 1)
-        return tb::launch_batch_kv<K, 1, false>(grids, units, nunits, waves, parity, t_first, write_ez, halo, s);
+        return tb::launch_batch_kv<K, 1, false>(grids, units, nunits, waves, parity, t_first, write_ez,
+                                                halo, vac, s);
     return fail("columns per lane %d not built", cols);
 }
 

@@ -271,11 +271,20 @@ int batch_launches(const fd2d_batch_desc *b, int t0, int nsteps, int max_wave_le
     if (L > nsteps)
         return fail("fd2d_batch_run: %d steps cannot be split into an even number of launches", nsteps);
     const int halo = ((D + b->cols - 1) / b->cols) * b->cols;
+    // free-space units: the first launch (the deepest: its units read every
+    // row a later launch's units read) records every unit's entry with a tag
+    // fresh to the call (bit 0 clear), the later launches use them
+    BatchVac vac{b->unit_work, 0, 0u};
+    if (b->unit_work) {
+        static std::atomic<unsigned> calls{0};
+        vac.tag = (calls.fetch_add(1) + 1u) << 1;
+    }
     int done = 0;
     for (int l = 0; l < L; ++l) {
         const int k = nsteps / L + (l < nsteps % L ? 1 : 0);
+        vac.mode = b->unit_work ? (l == 0 ? 1 : 2) : 0;
         int rc = launch_fd2d_batch(b->grids_dev, b->units_dev, b->nunits, b->waves, k, b->cols,
-                                   b->vec != 0, l % 2, t0 + done + 1, l == L - 1, halo, s);
+                                   b->vec != 0, l % 2, t0 + done + 1, l == L - 1, halo, vac, s);
         if (rc) return rc;
         done += k;
     }
@@ -1404,6 +1413,7 @@ int fd2d_batch_upload(const fd2d_grid *grids, int ngrids, int depth, int rows, i
     out->vec = vec ? 1 : 0;
     out->wave_len = min_wave;
     out->probe_len = min_probe;
+    out->unit_work = nullptr;
     return 0;
 }
 

@@ -269,17 +269,26 @@ int launch_fd2d_mw_depth(const TbParams &p, bool lossy, bool pml, int rows, int 
 template <int K>
 int preload_mw_depth();
 
+// Free-space units of a batched launch: work[u] for unit u, as
+// TbParams::tile_vac for tiles (mode 0: none, 1: record, 2: use; entries
+// tag | 1 mark units that read only naz = 1, nbz = 0).
+struct BatchVac {
+    unsigned *work;
+    int mode;
+    unsigned tag;
+};
+
 // Batched grids (lossy PML form): persistent launch over a work list of
 // 4-int units {grid, strip, first row, one past the last row}; parity 1
 // reads the partner sets.
 int launch_fd2d_batch(const fd2d_grid *grids, const int *units, int nunits, int waves, int depth,
                       int cols, bool vec, int parity, int t_first, int write_ez, int halo,
-                      hipStream_t s);
+                      BatchVac vac, hipStream_t s);
 int fd2d_batch_resident(int depth, int cols, bool vec, int *waves);
 template <int K>
 int launch_fd2d_batch_depth(const fd2d_grid *grids, const int *units, int nunits, int waves,
                             int cols, bool vec, int parity, int t_first, int write_ez, int halo,
-                            hipStream_t s);
+                            BatchVac vac, hipStream_t s);
 template <int K>
 int fd2d_batch_resident_depth(int cols, bool vec, int *waves);
 

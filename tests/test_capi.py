@@ -30,7 +30,7 @@ def test_header_symbols_are_exported(L):
 
 def test_abi_version_and_host_waveforms(L, oracle):
     lib = L.lib()
-    assert lib.fdtd_abi_version() == 3
+    assert lib.fdtd_abi_version() == 4
     for t in (1, 7, 20, 40, 333):
         assert lib.fdtd_gaussian_f32(t, 20, 6.0) == oracle.gaussian(t, 20, 6.0)
         assert lib.fdtd_gaussian_f64(t, 40, 12.0) == oracle.gaussian(t, 40, 12.0, "float64")

@@ -137,3 +137,35 @@ def test_full_c3_batch_in_one_graph(gpu):
                 assert same_bits(b.probe_trace(k), one.probe_traces()[0]), k
             del one
         b.close()
+
+
+def test_batch_free_space_units_same_bits(gpu):
+    """Batches that skip naz/nbz in free-space units (fd2d_batch_desc.unit_work)
+    give the bits of batches that read them, also after a medium changes
+    between two runs (every run records its units afresh)."""
+    from simulation_amd.batch import BatchSolver
+    specs = _specs()
+    print("[batch] free-space units", flush=True)
+    a = BatchSolver(specs, 80, depth=6, rows=20, cols=1, device=gpu)
+    b = BatchSolver(specs, 80, depth=6, rows=20, cols=1, device=gpu, unit_flags=False)
+    assert a._unit_work is not None and b._unit_work is None
+
+    def same():
+        for k in range(len(specs)):
+            ga, gb = a.grid_state(k), b.grid_state(k)
+            for f in FIELDS:
+                assert same_bits(ga[f], gb[f]), (k, f)
+            if specs[k].probe:
+                assert same_bits(a.probe_trace(k), b.probe_trace(k)), k
+    for x in (a, b):
+        x.run(40)
+    same()
+    assert int((a._unit_work & 1).sum()) > 0          # some units were free space
+    # a lossy patch appears in the free space of grid 3 (333 x 141, no
+    # cylinders; source at (20, 70)), inside units recorded as free space
+    for x in (a, b):
+        x.state[3]["md"].naz[40:60, 60:80] = 0.5
+        x.state[3]["md"].nbz[40:60, 60:80] = 0.01
+        x.run(40)
+    same()
+    assert np.abs(a.grid_state(3)["ez"][40:60, 60:80]).max() > 1e-6
