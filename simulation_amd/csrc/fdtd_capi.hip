@@ -713,8 +713,14 @@ int fd2d_tm_run(const fd2d_tm_desc *d, int t0, int nsteps, void *stream)
             static std::atomic<unsigned> calls{0};
             now.tag = (calls.fetch_add(1) + 1u) << 1;
             p.vac_tag = now.tag;
-            if (it != recs.end()) *it = now;
-            else recs.push_back(now);
+            if (it != recs.end()) {
+                *it = now;
+            } else {
+                // a bounded list: a workspace whose record was dropped only
+                // records again on its next call
+                if (recs.size() >= 256) recs.erase(recs.begin());
+                recs.push_back(now);
+            }
         }
         p.tile_vac = d->tile_work;
     }
