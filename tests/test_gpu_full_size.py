@@ -2,8 +2,9 @@
 the whole grid bit for bit against the FP32 oracle (every state array and
 every probe trace), and the FP32 drift against the FP64 oracle.
 
-The 4096^2 configuration (2000 steps) runs in every GPU run: the oracle
-takes about 10 s (FP32) and 16 s (FP64) on 16 host threads.  The 16384^2
+The 4096^2 configuration (2000 steps) and the 8192^2 bench workload (1000
+steps, FP32 only) run in every GPU run: the oracle takes about 10 s and 16 s
+(4096^2, FP32 and FP64) and 20 s (8192^2) on 16 host threads.  The 16384^2
 configuration (5000 steps) needs about 40 times that, so its tests run only
 when FDTD_FULL_SIZE=1 is set.  The oracle runs print a line every 250
 steps."""
@@ -95,6 +96,16 @@ def test_c2_full_grid_full_steps(gpu, oracle):
     st, traces, pml, naz, nbz = _gpu_run(cfg, gpu)
     _check_fp32(oracle, cfg, st, traces, pml, naz, nbz, "C2")
     _check_fp64(oracle, cfg, st, traces, "C2")
+
+
+def test_headline_8192_full_grid(gpu, oracle):
+    """The bench's 8192^2 workload (npml=8, cylinder r=400, hard Gaussian at
+    the centre, four probes) for 1000 steps: the whole grid bit for bit
+    against the FP32 oracle (about 20 s of oracle time)."""
+    from simulation_amd import fd2d
+    cfg = fd2d.c2_config(8192, 1000)
+    st, traces, pml, naz, nbz = _gpu_run(cfg, gpu)
+    _check_fp32(oracle, cfg, st, traces, pml, naz, nbz, "8192^2")
 
 
 @full_only
