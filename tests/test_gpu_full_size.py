@@ -125,3 +125,75 @@ def test_c5_full_grid_full_steps_fp64(gpu, oracle):
     cfg = fd2d.c5_config(16384, 5000)
     st, traces, _, _, _ = _gpu_run(cfg, gpu)
     _check_fp64(oracle, cfg, st, traces, "C5")
+
+
+def _c4_case(n=2048, ns=600):
+    """2048^2 learned-boundary grid (no PML), a lossy cylinder and a hard
+    Gaussian 160 cells below the top edge: the pulse reaches the learned
+    frame after about 320 steps and works against it for the rest."""
+    cyl = [(400.0, 1200.5, 100.0, 4.0, 0.02)]
+    return n, ns, cyl, (160, 1100)
+
+
+def _c4_runs(gpu, params, n, ns, cyl, src):
+    from simulation_amd import fd1d, fd2d, learned
+    dt = fd2d.time_step()
+    md = fd2d.dielectric(n, n, 0, 0, dt, 1.0, 0.0, gpu, cylinders=cyl)
+    w = fd1d.waveform("gaussian", ns, 0, np.float32, center=20, spread=6.0)
+    s = learned.LearnedTMSolver(n, n, learned.Net(params, gpu), md, [(*src, w)], [], nsteps_max=ns,
+                                device=gpu)
+    print(f"[full-size] C4 {n}^2 learned boundary, {ns} steps on the GPU", flush=True)
+    s.run(ns)
+    return s.state(), md.naz.cpu().numpy(), md.nbz.cpu().numpy(), w
+
+
+def _c4_oracle(params, naz, nbz, ns, src, w, dtype, label, chunk=100):
+    from oracle import lbc_oracle as L
+    n = naz.shape[0]
+    st = {k: np.zeros((n, n), dtype) for k in ("dz", "iz", "ez", "hx", "hy")}
+    hist, t0 = None, time.time()
+    for a in range(0, ns, chunk):
+        k = min(chunk, ns - a)
+        hist = L.lbc_run(st, naz, nbz, params, k, hist, [src], [w], t0=a)
+        print(f"[full-size] {label}: {a + k}/{ns} steps, {time.time() - t0:.0f} s", flush=True)
+    return st
+
+
+@full_only
+def test_c4_full_grid_zero_network_bit_exact(gpu):
+    """With a zero network every prediction is the two-step boundary in FP32
+    on both sides: the whole 2048^2 learned-boundary path (blocked launch,
+    frame kernel, history ring) bit for bit against the oracle's step."""
+    from oracle import lbc_oracle as L
+    n, ns, cyl, src = _c4_case()
+    params = np.zeros(L.NPARAM, np.float32)
+    st, naz, nbz, w = _c4_runs(gpu, params, n, ns, cyl, src)
+    o = _c4_oracle(params, naz, nbz, ns, src, w, np.float32, "C4 zero network FP32 oracle")
+    for k in ("dz", "iz", "ez", "hx", "hy"):
+        assert same_bits(st[k], o[k]), k
+    assert np.abs(o["ez"][:8]).max() > 1e-6          # the pulse reached the top frame
+    print("[full-size] C4 zero network: every state array bit-exact against the FP32 oracle",
+          flush=True)
+
+
+@full_only
+def test_c4_full_grid_shipped_network(gpu, oracle):
+    """The shipped network at 2048^2 against the FP32 oracle step and the
+    FP64 evaluation of the same scheme (FP64 coefficients, samples,
+    prediction and updates)."""
+    from simulation_amd import fd2d, learned
+    n, ns, cyl, src = _c4_case()
+    params = np.load(learned.WEIGHTS, allow_pickle=False)["params"].astype(np.float32)
+    st, naz, nbz, w = _c4_runs(gpu, params, n, ns, cyl, src)
+    o = _c4_oracle(params, naz, nbz, ns, src, w, np.float32, "C4 shipped network FP32 oracle")
+    r32 = {k: float(np.linalg.norm(st[k] - o[k]) / np.linalg.norm(o[k])) for k in ("ez", "hx", "hy")}
+    exact = all(same_bits(st[k], o[k]) for k in ("dz", "iz", "ez", "hx", "hy"))
+    print(f"[full-size] C4 shipped network vs FP32 oracle: bit-exact={exact}, rel-L2 {r32}", flush=True)
+    del o
+    n64, b64 = oracle.fd2d_dielectric(n, n, 0, fd2d.time_step(), cyl)
+    w64 = oracle.waveform("gaussian", 0, ns, np.float64, center=20, spread=6.0)
+    o64 = _c4_oracle(params, n64, b64, ns, src, w64, np.float64, "C4 shipped network FP64 oracle")
+    r64 = {k: float(np.linalg.norm(st[k] - o64[k]) / np.linalg.norm(o64[k])) for k in ("ez", "hx", "hy")}
+    print(f"[full-size] C4 shipped network FP32 vs FP64 rel-L2 {r64}", flush=True)
+    assert max(r32.values()) < 1e-4
+    assert max(r64.values()) < 1e-3
