@@ -26,14 +26,16 @@ CACHE = os.environ.get("FDTD_TUNE_CACHE",
 
 def candidates(nx: int):
     """(depth, rows, cols, form) shapes tried for an nx-row grid: single-wave
-    strips at depth 10 with row blocks of nx/170, nx/128 and nx/96 rows and
-    at depth 8 with nx/128 (MI355X sweeps put the optimum at depth 10 from
-    2048^2 on, rows ~nx/170 at 2048^2 and ~nx/128 from 4096^2 on), two
-    columns per lane at depth 8, and the multi-wave form with short blocks."""
+    strips at depth 10 (and from 4096 rows on depth 11) with row blocks of
+    nx/170, nx/128 and nx/96 rows and at depth 8 with nx/128 (MI355X sweeps
+    put the optimum at depth 10 at 2048^2 and depth 11 from 4096^2 on, rows
+    ~nx/170 at 2048^2 and ~nx/128 from 4096^2 on), two columns per lane at
+    depth 8, and the multi-wave form with short blocks."""
     def r(d):
         return min(512, max(8, nx // d))
-    for rows in sorted({r(170), r(128), r(96)}):
-        yield 10, rows, 1, 1
+    for depth in ((10, 11) if nx >= 4096 else (10,)):
+        for rows in sorted({r(170), r(128), r(96)}):
+            yield depth, rows, 1, 1
     yield 8, r(128), 1, 1
     yield 8, r(128), 2, 1
     yield 8, 32, 1, 2
@@ -99,15 +101,18 @@ def tune(solver: fd2d.TMSolver, steps: int = 80, use_cache: bool = True, configs
         saved = {k: getattr(solver, k).clone() for k in fd2d.STATE if getattr(solver, k) is not None}
         probe_saved = None if solver.probe_out is None else solver.probe_out.clone()
         t_saved = solver.t
-        n = steps if not solver.probes else min(steps, solver.probe_len)
-        n -= n % 2
+        base = steps if not solver.probes else min(steps, solver.probe_len)
         results = []
         for cand in (configs or candidates(solver.nx)):
             depth, rows, cols = cand[:3]
             form = cand[3] if len(cand) > 3 else 0
             solver.depth, solver.rows, solver.cols, solver.form = depth, rows, cols, form
+            # whole pairs of full-depth launches where the run is long enough
+            # (a long run's launches are nearly all full depth), else the
+            # run's own length
+            n = base - base % (2 * depth) if base >= 2 * depth else base - base % 2
 
-            def go():
+            def go(n=n):
                 solver.t = 0
                 solver.run(n)
             dt = _time(go)

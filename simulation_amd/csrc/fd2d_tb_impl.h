@@ -707,7 +707,7 @@ __device__ __forceinline__ bool sweep(const PR &P, const Geom &g, NaRing<K, C> &
 // concurrently.  A wave returns at once if its tile is not its part's.
 template <int K, int C, bool LOSSY, bool PML, bool VEC, int PART, int NF = 0>
 __global__ void __launch_bounds__(64)
-__attribute__((amdgpu_waves_per_eu(C == 1 && K <= 10 && NF == 0 ? 3 : 1)))
+__attribute__((amdgpu_waves_per_eu(C == 1 && K <= 11 && NF == 0 ? 3 : 1)))
 tm_tb_kernel(const TbParams P)
 {
     constexpr int S = 64 * C;

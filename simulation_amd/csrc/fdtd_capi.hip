@@ -118,19 +118,23 @@ int mw_default_rows(int nx)
 
 // Default depth and rows of the single-wave strips, from sweeps on MI355X
 // (lossy PML form, one column per lane; Mcell-updates/s at 1000-2000 steps):
-//   2048^2:  depth 10, rows 12 (295k; rows 16/24: 260k/206k; depth 8 rows 16:
-//            278k)
-//   4096^2:  depth 10, rows 32 (537k; rows 24/28/40: 518k/539k/541k; depth 8
-//            rows 16/32/48: 449k/494k/461k; depth 12: 482k)
-//   8192^2:  depth 10, rows 64 (657k; rows 56/72/80: 624k/645k/652k; depth 8
-//            rows 64: 586k; depth 12: 576k)
-//   16384^2: depth 10, rows 96 (691k; rows 64/128/160: 679k/688k/681k;
-//            depth 8 rows 96: 606k)
-// Depth 10 (two carry sets, 3 waves per SIMD) wins from 2048^2 on; rows
-// follow nx/128 for 4096 <= nx < 12288 and ~nx/170 outside.
+//   2048^2:  depth 10, rows 12 (355k; rows 16/24: 260k/206k; depth 8 rows 16:
+//            278k; depth 11 rows 12/16: 342k/305k)
+//   4096^2:  depth 11, rows 32 (621k; rows 40: 626k; depth 10 rows 32/42:
+//            592k/595k; depth 8 rows 16/32/48: 449k/494k/461k)
+//   8192^2:  depth 11, rows 64 (737k-757k; rows 56/72/85/96: 699k/732k/724k/
+//            725k; depth 10 rows 64: 698k-715k; depth 8 rows 64: 586k;
+//            depth 12: 576k)
+//   16384^2: depth 11, rows 96 (815k; rows 128: 817k; depth 10 rows 96:
+//            774k; depth 8 rows 96: 606k)
+// Depths 10 and 11 keep two carry sets in 3 waves per SIMD (depth 12 needs
+// 192 VGPRs: 2 waves); depth 11 moves the arrays once per 11 steps instead
+// of 10 and wins from 4096^2 on, where its one more halo column and warm-up
+// row per side cost less.  Rows follow nx/128 for 4096 <= nx < 12288 and
+// ~nx/170 outside.
 void tm_defaults(int nx, int *depth, int *rows)
 {
-    *depth = nx >= 2048 ? 10 : 8;
+    *depth = nx >= 4096 ? 11 : (nx >= 2048 ? 10 : 8);
     long h = nx >= 4096 && nx < 12288 ? nx / 128 : nx / 170;
     if (h < 8) h = 8;
     if (h > 512) h = 512;
@@ -155,7 +159,8 @@ int plan_tm(const fd2d_tm_desc *d, int nsteps, Plan *pl)
     const int mw = d->form == FDTD_TM_FORM_MULTIWAVE;
     int H = d->rows ? d->rows : (mw ? mw_default_rows(d->nx) : (int)h);
     if (H < 1) return fail("fd2d_tm: rows=%d", H);
-    // strip halo: the deepest launch's cone, rounded up to whole lanes
+    // strip halo: the deepest launch's cone, rounded up to whole lanes (set
+    // again below from the launches of this call)
     pl->depth = D; pl->cols = C; pl->rows = H; pl->halo = ((D + C - 1) / C) * C;
     pl->mw = mw; pl->frows = mw ? (int)h : H;
     pl->launches = 0; pl->tail_unfused = 0;
@@ -167,6 +172,12 @@ int plan_tm(const fd2d_tm_desc *d, int nsteps, Plan *pl)
         else { pl->tail_unfused = 1; L = (nsteps - 1 + D - 1) / D; if (L % 2) L += 1; }
     }
     pl->launches = L;
+    // the strip halo of this call: its deepest launch's cone (a 20-step call
+    // at depth 11 runs two launches of 10 steps, which need 10 halo columns,
+    // not 11)
+    const int nfused = nsteps - pl->tail_unfused;
+    const int kmax = nfused / L + (nfused % L ? 1 : 0);
+    pl->halo = ((kmax + C - 1) / C) * C;
     return 0;
 }
 
@@ -1209,7 +1220,10 @@ int fd2d_lbc_run(const fd2d_tm_desc *d, const fd2d_lbc_desc *b, int t0, int nste
     TbParams p{};
     p.nx = d->nx; p.ny = d->ny;
     p.naz = d->md.naz; p.nbz = d->md.nbz;
-    p.halo = pl.halo; p.rows = pl.rows;
+    // this run's launches fuse up to pl.depth steps (the plan above was made
+    // for 16 steps only to fix depth, rows and columns)
+    p.halo = ((pl.depth + pl.cols - 1) / pl.cols) * pl.cols;
+    p.rows = pl.rows;
     // interior part: waves whose reads and writes stay DZ or more cells from
     // every edge.  The frame kernel reads less than DZ = 3K+6 cells deep and
     // writes less than K+3 deep, so the interior part of launch l+1 needs
