@@ -464,8 +464,9 @@ int lbc_train_grad(int n, const float *src, const long long *offs, long long tst
 /* Time stepping with the learned boundary.  The TM descriptor gives the grid
  * without a PML (pml fields NULL; md.nbz optional: lossy or lossless), the
  * partner state set, sources and probes (all at least fd2d_lbc_margin()
- * cells from every edge) and the launch shape of the blocked kernel (depth
- * at most 8).  Each launch of k <= depth steps is the blocked launch with PEC
+ * cells from every edge, and 3 depth + 6 when that is more: the frame
+ * kernel's deepest region) and the launch shape of the blocked kernel
+ * (depth at most 10, default 8).  Each launch of k <= depth steps is the blocked launch with PEC
  * edges followed by a frame kernel that recomputes the outer k+3 cells with
  * the network applied every step (the blocked launch is exact deeper in).
  * hist, hist2: [lbc_hist_elems(nx, ny)] floats each (npos x 4 x 30), the last four boundary

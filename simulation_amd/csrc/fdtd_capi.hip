@@ -1181,7 +1181,7 @@ size_t lbc_hist_elems(int nx, int ny)
 size_t lbc_work_bytes(int nx, int ny)
 {
     if (nx < 64 || ny < 64) return 0;
-    return lbc_frame_table_offset(nx, ny, 9);
+    return lbc_frame_table_offset(nx, ny, LBC_MAX_DEPTH + 1);
 }
 
 int fd2d_lbc_margin(void) { return FDTD_LBC_MARGIN; }
@@ -1195,7 +1195,17 @@ int fd2d_lbc_run(const fd2d_tm_desc *d, const fd2d_lbc_desc *b, int t0, int nste
     if (!b || !b->params || !b->wbuf || !b->hist || !b->hist2 || !b->work)
         return fail("fd2d_lbc_run: params, wbuf, hist, hist2 and work are required");
     if (d->nx < 64 || d->ny < 64) return fail("fd2d_lbc_run: grid %dx%d below 64x64", d->nx, d->ny);
-    const int M = FDTD_LBC_MARGIN;
+    Plan pl;
+    fd2d_tm_desc ds = *d;
+    ds.form = FDTD_TM_FORM_STRIP;         // the split launches are single-wave strips
+    if (!ds.depth) ds.depth = 8;          // and fuse at most LBC_MAX_DEPTH steps (the frame's tables)
+    rc = plan_tm(&ds, 2 * LBC_MAX_DEPTH, &pl);
+    if (rc) return rc;
+    if (pl.depth > LBC_MAX_DEPTH)
+        return fail("fd2d_lbc_run: depth %d above %d", pl.depth, LBC_MAX_DEPTH);
+    // sources and probes outside the frame kernel's regions (which apply no
+    // source and record no probe): 3K+6 deep at the corner chunks
+    const int M = std::max(FDTD_LBC_MARGIN, 3 * pl.depth + 6);
     for (int k = 0; k < d->nsrc; ++k)
         if (d->src_i[k] < M || d->src_i[k] >= d->nx - M || d->src_j[k] < M || d->src_j[k] >= d->ny - M)
             return fail("fd2d_lbc_run: source %d within %d cells of an edge", k, M);
@@ -1205,13 +1215,6 @@ int fd2d_lbc_run(const fd2d_tm_desc *d, const fd2d_lbc_desc *b, int t0, int nste
             return fail("fd2d_lbc_run: probe %d within %d cells of an edge", k, M);
     if (nsteps == 0) return 0;
     if (nsteps < 2) return fail("fd2d_lbc_run: nsteps=%d (at least 2: launches come in pairs)", nsteps);
-    Plan pl;
-    fd2d_tm_desc ds = *d;
-    ds.form = FDTD_TM_FORM_STRIP;         // the split launches are single-wave strips
-    if (!ds.depth) ds.depth = 8;          // and fuse at most 8 steps (the frame's tables)
-    rc = plan_tm(&ds, 2 * 8, &pl);
-    if (rc) return rc;
-    if (pl.depth > 8) return fail("fd2d_lbc_run: depth %d above 8", pl.depth);
     const bool lossy = d->md.nbz != nullptr;
     if (!d->dz2 || !d->hx2 || !d->hy2 || (lossy && !d->iz2))
         return fail("fd2d_lbc_run: the partner state set (dz2, iz2, hx2, hy2) is required");
@@ -1221,7 +1224,7 @@ int fd2d_lbc_run(const fd2d_tm_desc *d, const fd2d_lbc_desc *b, int t0, int nste
     p.nx = d->nx; p.ny = d->ny;
     p.naz = d->md.naz; p.nbz = d->md.nbz;
     // this run's launches fuse up to pl.depth steps (the plan above was made
-    // for 16 steps only to fix depth, rows and columns)
+    // only to fix depth, rows and columns)
     p.halo = ((pl.depth + pl.cols - 1) / pl.cols) * pl.cols;
     p.rows = pl.rows;
     // interior part: waves whose reads and writes stay DZ or more cells from

@@ -70,6 +70,7 @@ struct CylList {
 // Arguments of one temporally blocked launch (fd2d_tb.hip).
 constexpr int TB_MAX_DEPTH = 12;    // temporally blocked launches: steps per launch
 constexpr int MW_MAX_DEPTH = 8;     // multi-wave form: steps per launch
+constexpr int LBC_MAX_DEPTH = 10;   // learned boundary: steps per launch (frame tables, LDS)
 constexpr int TB_MAX_FREQ = 3;      // fused running DFT: frequencies
 constexpr int TB_MAX_LIST_RB = 64;  // boundary launch: whole row blocks listed
 constexpr int TB_MAX_LIST_SX = 64;  // boundary launch: whole strips listed

@@ -49,14 +49,15 @@ namespace {
 
 constexpr int CH = 32;                        // chunk length along the edge (at 2048^2:
                                               // 256 workgroups, one per CU)
-constexpr int KMAX = 8;
+constexpr int KMAX = 10;                      // steps per launch
 constexpr int NT = 512;                       // threads per workgroup (8 waves)
 // region cells: corner chunks (3K+6) x (CH + 2K + 1), others at most
-// (2K+6) x (CH + 3K + 2) (30 x 49 and 22 x 58 at K = 8)
+// (2K+6) x (CH + 3K + 2) (30 x 49 and 22 x 58 at K = 8, 36 x 53 and
+// 26 x 64 at K = 10)
 constexpr int RMAX_C = (3 * KMAX + 6) * (CH + 2 * KMAX + 1);
 constexpr int RMAX_E = (2 * KMAX + 6) * (CH + 3 * KMAX + 2);
 constexpr int RMAX = ((RMAX_C > RMAX_E ? RMAX_C : RMAX_E) + 3) / 4 * 4;   // keeps the ring 16-byte aligned
-constexpr int PMAX = 128;                     // positions per workgroup
+constexpr int PMAX = 136;                     // positions per workgroup (130 at K = 10)
 
 struct FrameSmem {
     float dz[RMAX], iz[RMAX], ez[RMAX], hx[RMAX], hy[RMAX], na[RMAX], nb[RMAX];
@@ -72,10 +73,12 @@ struct FrameSmem {
 };
 
 static_assert(sizeof(FrameSmem) + 64 <= 160 * 1024, "frame kernel LDS above 160 KiB");
+static_assert(CH + 3 * KMAX + 2 + 2 * (3 * KMAX + 3) <= PMAX && PMAX % 8 == 0,
+              "positions of a frame workgroup above PMAX");
 
 // A workgroup's positions and sample sources depend only on the grid shape
 // and the depth: built once (frame_setup_kernel) into global memory.
-struct FrameTab {
+struct alignas(16) FrameTab {
     int np, pad[3];
     int nact[KMAX + 4];     // nact[l]: positions needed at level l (a prefix)
     int pos[PMAX], own[PMAX];
@@ -179,7 +182,8 @@ __device__ inline void block_geo(int b, int nx, int ny, int K, Geo &G, int &kind
 }
 
 // One workgroup per frame workgroup: its position list (thread 0; at most
-// CH + 3K + 2 + 2 (3K + 3) = 112 positions) and sample-source table.
+// CH + 3K + 2 + 2 (3K + 3) positions: 112 at K = 8, 130 at K = 10) and
+// sample-source table.
 __global__ void __launch_bounds__(NT) frame_setup_kernel(int nx, int ny, int K, FrameTab *tabs)
 {
     __shared__ FrameTab S;
@@ -192,7 +196,7 @@ __global__ void __launch_bounds__(NT) frame_setup_kernel(int nx, int ny, int K, 
     block_geo(blockIdx.x, nx, ny, K, G, kind, q);
     const int RH = G.RH, RW = G.RW;
     // ---------------------------------------------------------- positions
-    // (thread 0 builds the list; at most CH + 3K + 2 + 2 (3K + 3) = 112)
+    // (thread 0 builds the list; at most CH + 3K + 2 + 2 (3K + 3) <= PMAX)
     if (tid == 0) {
         int n = 0;
         auto add = [&](int e, int t0, int t1, int w0, int w1) {

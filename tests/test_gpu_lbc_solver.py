@@ -52,13 +52,15 @@ def _gpu_run(nx, ny, md, params, runs, src, wave, probe, dev, **kw):
     return s
 
 
-@pytest.mark.parametrize("depth,rows,runs", [(8, 0, (60, 30)), (3, 16, (67, 8)), (8, 24, (2, 82))])
+@pytest.mark.parametrize("depth,rows,runs", [(8, 0, (60, 30)), (3, 16, (67, 8)), (8, 24, (2, 82)),
+                                             (10, 0, (60, 30)), (9, 20, (61, 30))])
 def test_zero_network_bit_exact(gpu, depth, rows, runs):
     nx, ny = 96, 112
     print(f"[lbc solver] zero net depth={depth} rows={rows} runs={runs}", flush=True)
     p = np.zeros(L.NPARAM, np.float32)
     md = _medium(nx, ny, gpu)
-    src, probe = (34, 40), (40, 40)
+    # at least max(32, 3 depth + 6) cells from every edge
+    src, probe = ((34, 40), (40, 40)) if depth <= 8 else ((40, 42), (46, 42))
     wave = _wave(sum(runs))
     g = _gpu_run(nx, ny, md, p, runs, src, wave, probe, gpu, depth=depth, rows=rows)
     st, hist = _oracle_run(nx, ny, md, p, runs, src, wave)
@@ -104,6 +106,15 @@ def test_lbc_run_rejects_bad_setups(gpu):
     s = learned.LearnedTMSolver(nx, ny, net, md, [(48, 48, _wave(10))], [], nsteps_max=10, device=gpu)
     with pytest.raises(_lib.FdtdError, match="at least 2"):
         s.run(1)
+    # depth 10: the frame's corner regions reach 36 cells deep
+    s = learned.LearnedTMSolver(nx, ny, net, md, [(34, 48, _wave(10))], [], nsteps_max=10, depth=10,
+                                device=gpu)
+    with pytest.raises(_lib.FdtdError, match="within 36"):
+        s.run(4)
+    s = learned.LearnedTMSolver(nx, ny, net, md, [(48, 48, _wave(10))], [], nsteps_max=10, depth=11,
+                                device=gpu)
+    with pytest.raises(_lib.FdtdError, match="above 10"):
+        s.run(4)
 
 
 def test_trained_network_gain(gpu):
