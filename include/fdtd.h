@@ -264,16 +264,19 @@ typedef struct fd2d_tm_desc {
     int form;                     /* FDTD_TM_FORM_* */
     /* Optional workspace of tile_elems >= fd2d_tm_tile_elems(d, nsteps)
      * unsigned ints (NULL: none).  With it, the first fused launch of a
-     * call records which interior tiles read only naz = 1 and nbz = 0 (free
-     * space), and the call's later launches skip reading naz and nbz there
-     * (the values are the constants; every result bit is the same).
+     * call records which interior tiles read only naz = 1, nbz = +0 and
+     * iz = +0 (free space at rest), and the call's later launches neither
+     * read naz, nbz, iz there nor store iz (it stays +0 in both state sets:
+     * iz + 0 * ez is +0; every result bit is the same as without the
+     * workspace, for finite fields).
      * tile_keep = 1 promises that naz and nbz hold what they held when the
-     * flags in tile_work were last recorded: a call with the same arrays,
+     * flags in tile_work were last recorded, and that iz and iz2 were
+     * written by nothing but those runs since: a call with the same arrays,
      * grid, rows, columns and a depth no deeper than the recording one then
      * uses those flags from its first launch on (new arrays that reuse the
      * memory of freed ones break that promise: pass tile_keep = 0 on the
-     * first call of a new medium).  tile_keep = 0: flags of earlier calls
-     * are never used.                                                      */
+     * first call of a new medium or state).  tile_keep = 0: flags of
+     * earlier calls are never used.                                        */
     unsigned *tile_work;
     size_t tile_elems;
     int tile_keep;

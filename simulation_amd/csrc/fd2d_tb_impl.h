@@ -218,7 +218,9 @@ __device__ __forceinline__ void st_rb(__amdgpu_buffer_rsrc_t rs, unsigned row, i
 }
 
 // VAC: a free-space tile (every naz 1, nbz 0): neither array is read.
-template <int C, bool LOSSY, bool PML, bool VEC, bool CLAMP, bool VAC = false, class PR>
+// ZIZ: and every iz it reads is +0, so iz is not read either.
+template <int C, bool LOSSY, bool PML, bool VEC, bool CLAMP, bool VAC = false, bool ZIZ = false,
+          class PR>
 __device__ __forceinline__ void load_row(const PR &P, int p, int j0, HRow<C> &H, DRow<C> &D)
 {
     const int pc = min(max(p, 0), P.nx - 1);     // rows outside the grid: any in-bounds row
@@ -233,7 +235,12 @@ __device__ __forceinline__ void load_row(const PR &P, int p, int j0, HRow<C> &H,
     ld<C, VEC, CLAMP>(P.hx_i, row, j0, P.ny, H.hx);
     ld<C, VEC, CLAMP>(P.hy_i, row, j0, P.ny, H.hy);
     if (LOSSY) {
-        ld<C, VEC, CLAMP>(P.iz_i, row, j0, P.ny, D.iz);
+        if (ZIZ) {
+#pragma unroll
+            for (int c = 0; c < C; ++c) D.iz[c] = 0.f;
+        } else {
+            ld<C, VEC, CLAMP>(P.iz_i, row, j0, P.ny, D.iz);
+        }
         if (!VAC) ld<C, VEC, CLAMP>(P.nbz, row, j0, P.ny, D.nb);
     } else {
 #pragma unroll
@@ -276,9 +283,11 @@ __device__ __forceinline__ void col_coef(const PR &P, int j0, ColCoef<C> &cc)
 // same for the strip's columns; SP: a source or probe lies in the rows and
 // columns the wave computes.
 // VAC: a free-space tile (naz 1, nbz 0 as constants, no profile ring);
-// REC: AND into vac whether the row's naz, nbz are exactly 1, 0.
+// REC: AND into vac whether the row's naz, nbz, iz are exactly 1, +0, +0;
+// ZIZ (with VAC): every iz of the tile is +0 in both state sets, so it is
+// neither read nor stored (in free space iz + 0 * ez stays +0).
 template <int K, int C, bool LOSSY, bool PML, bool VEC, bool BROW, bool BCOL, bool SP, int NF,
-          bool BS, bool PRO = false, bool VAC = false, bool REC = false, class PR>
+          bool BS, bool PRO = false, bool VAC = false, bool REC = false, bool ZIZ = false, class PR>
 __device__ __forceinline__ void iterate(const PR &P, const Geom &g, const ColCoef<C> &cc,
                                         NaRing<K, C> &ring,
                                         int p, const HRow<C> &Hload, const DRow<C> &Dload,
@@ -293,7 +302,8 @@ __device__ __forceinline__ void iterate(const PR &P, const Geom &g, const ColCoe
 #pragma unroll
         for (int c = 0; c < C; ++c)
             vac = vac && __builtin_bit_cast(unsigned, Dload.na[c]) == 0x3F800000u &&
-                  (!LOSSY || __builtin_bit_cast(unsigned, Dload.nb[c]) == 0u);
+                  (!LOSSY || (__builtin_bit_cast(unsigned, Dload.nb[c]) == 0u &&
+                              __builtin_bit_cast(unsigned, Dload.iz[c]) == 0u));
     }
     if constexpr (!VAC) {
         // row p's profile values (level 0's E row) for the levels below
@@ -327,7 +337,7 @@ __device__ __forceinline__ void iterate(const PR &P, const Geom &g, const ColCoe
 #pragma unroll
             for (int c = 0; c < C; ++c) {
                 Din.dz[c] = Si[s - 1].d.dz[c];
-                Din.iz[c] = Si[s - 1].d.iz[c];
+                Din.iz[c] = ZIZ ? 0.f : Si[s - 1].d.iz[c];
                 Din.na[c] = VAC ? 1.f : rd[c];
                 Din.nb[c] = (LOSSY && !VAC) ? rd[C + c] : 0.f;
             }
@@ -521,7 +531,7 @@ __device__ __forceinline__ void iterate(const PR &P, const Geom &g, const ColCoe
                 const bool okr = r >= g.i0 && r < g.i1 && r >= sw && r < P.nx - sw;
                 const unsigned rowr = (unsigned)r * (unsigned)ny;
                 st_rb<C>(out_rsrc(P.dz_o, nb), rowr, g.j0, c0, c1, okr, Dn.dz);
-                if (LOSSY) st_rb<C>(out_rsrc(P.iz_o, nb), rowr, g.j0, c0, c1, okr, Dn.iz);
+                if (LOSSY && !ZIZ) st_rb<C>(out_rsrc(P.iz_o, nb), rowr, g.j0, c0, c1, okr, Dn.iz);
                 st_rb<C>(out_rsrc(P.ez_o, P.ez_o ? nb : 0u), rowr, g.j0, c0, c1, okr, En);
                 const bool okq = q >= g.i0 && q < g.i1 && q >= sw && q < P.nx - sw;
                 const unsigned rowq = (unsigned)q * (unsigned)ny;
@@ -535,7 +545,7 @@ __device__ __forceinline__ void iterate(const PR &P, const Geom &g, const ColCoe
                 if (r >= g.i0 && r < g.i1 && r >= sw && r < P.nx - sw) {
                     const unsigned row = (unsigned)r * (unsigned)ny;
                     st<C, VEC>(P.dz_o, row, g.j0, c0, c1, Dn.dz);
-                    if (LOSSY) st<C, VEC>(P.iz_o, row, g.j0, c0, c1, Dn.iz);
+                    if (LOSSY && !ZIZ) st<C, VEC>(P.iz_o, row, g.j0, c0, c1, Dn.iz);
                     if (P.ez_o) st<C, VEC>(P.ez_o, row, g.j0, c0, c1, En);
                     if constexpr (NF > 0) {
                         const size_t plane = (size_t)P.nx * (size_t)ny;
@@ -581,10 +591,11 @@ __device__ __forceinline__ void load_ft(const PR &P, int p, int j0, FRow<C, NF> 
     }
 }
 
-// Returns, with REC, whether every naz, nbz the sweep read is exactly 1, 0
-// (wave-uniform); otherwise false.
+// Returns, with REC, whether every naz, nbz, iz the sweep read is exactly 1,
+// +0, +0 (wave-uniform); otherwise false.
 template <int K, int C, bool LOSSY, bool PML, bool VEC, bool BROW, bool BCOL, bool SP, int NF = 0,
-          bool BS = false, bool PP = false, bool VAC = false, bool REC = false, class PR>
+          bool BS = false, bool PP = false, bool VAC = false, bool REC = false, bool ZIZ = false,
+          class PR>
 __device__ __forceinline__ bool sweep(const PR &P, const Geom &g, NaRing<K, C> &ring)
 {
     bool vac = true;
@@ -630,7 +641,7 @@ __device__ __forceinline__ bool sweep(const PR &P, const Geom &g, NaRing<K, C> &
     HRow<C> H0, H1;
     DRow<C> D0, D1;
     FRow<C, NF> F0, F1;
-    load_row<C, LOSSY, PML, VEC, BCOL, VAC>(P, pbeg, g.j0, H0, D0);
+    load_row<C, LOSSY, PML, VEC, BCOL, VAC, ZIZ>(P, pbeg, g.j0, H0, D0);
     if constexpr (NF > 0) load_ft<C, NF, VEC, BCOL>(P, pbeg, g.j0, F0);
     // BS: the first row is needed at once; waiting for it outside the loop
     // keeps the loop head free of a wait that would also catch the stores of
@@ -645,31 +656,31 @@ __device__ __forceinline__ bool sweep(const PR &P, const Geom &g, NaRing<K, C> &
         const int c0 = g.i0 - K;
 #pragma unroll 1
         for (; p < c0 + 2 * K - 1; ++p) {
-            load_row<C, LOSSY, PML, VEC, BCOL, VAC>(P, p + 1, g.j0, H1, D1);
-            iterate<K, C, LOSSY, PML, VEC, BROW, BCOL, SP, NF, BS, true, VAC, REC>(
+            load_row<C, LOSSY, PML, VEC, BCOL, VAC, ZIZ>(P, p + 1, g.j0, H1, D1);
+            iterate<K, C, LOSSY, PML, VEC, BROW, BCOL, SP, NF, BS, true, VAC, REC, ZIZ>(
                 P, g, cc, ring, p, H0, D0, St[0], St[0], F0, SF[0], SF[0], p - c0, vac);
             H0 = H1;
             D0 = D1;
         }
         // two rows per trip, so the row registers alternate as well
         for (; p < pend; p += 2) {
-            load_row<C, LOSSY, PML, VEC, BCOL, VAC>(P, p + 1, g.j0, H1, D1);
+            load_row<C, LOSSY, PML, VEC, BCOL, VAC, ZIZ>(P, p + 1, g.j0, H1, D1);
             if constexpr (NF > 0) load_ft<C, NF, VEC, BCOL>(P, p + 1, g.j0, F1);
-            iterate<K, C, LOSSY, PML, VEC, BROW, BCOL, SP, NF, BS, false, VAC, REC>(
+            iterate<K, C, LOSSY, PML, VEC, BROW, BCOL, SP, NF, BS, false, VAC, REC, ZIZ>(
                 P, g, cc, ring, p, H0, D0, St[0], St[NS - 1], F0, SF[0], SF[NS - 1], 0, vac);
-            load_row<C, LOSSY, PML, VEC, BCOL, VAC>(P, p + 2, g.j0, H0, D0);
+            load_row<C, LOSSY, PML, VEC, BCOL, VAC, ZIZ>(P, p + 2, g.j0, H0, D0);
             if constexpr (NF > 0) load_ft<C, NF, VEC, BCOL>(P, p + 2, g.j0, F0);
-            iterate<K, C, LOSSY, PML, VEC, BROW, BCOL, SP, NF, BS, false, VAC, REC>(
+            iterate<K, C, LOSSY, PML, VEC, BROW, BCOL, SP, NF, BS, false, VAC, REC, ZIZ>(
                 P, g, cc, ring, p + 1, H1, D1, St[NS - 1], St[0], F1, SF[NS - 1], SF[0], 0, vac);
         }
         if (p == pend)
-            iterate<K, C, LOSSY, PML, VEC, BROW, BCOL, SP, NF, BS, false, VAC, REC>(
+            iterate<K, C, LOSSY, PML, VEC, BROW, BCOL, SP, NF, BS, false, VAC, REC, ZIZ>(
                 P, g, cc, ring, p, H0, D0, St[0], St[NS - 1], F0, SF[0], SF[NS - 1], 0, vac);
     } else {
         for (int p = pbeg; p <= pend; ++p) {
-            load_row<C, LOSSY, PML, VEC, BCOL, VAC>(P, p + 1, g.j0, H1, D1);
+            load_row<C, LOSSY, PML, VEC, BCOL, VAC, ZIZ>(P, p + 1, g.j0, H1, D1);
             if constexpr (NF > 0) load_ft<C, NF, VEC, BCOL>(P, p + 1, g.j0, F1);
-            iterate<K, C, LOSSY, PML, VEC, BROW, BCOL, SP, NF, BS, false, VAC, REC>(
+            iterate<K, C, LOSSY, PML, VEC, BROW, BCOL, SP, NF, BS, false, VAC, REC, ZIZ>(
                 P, g, cc, ring, p, H0, D0, St[0], St[0], F0, SF[0], SF[0], 0, vac);
             H0 = H1;
             D0 = D1;
@@ -813,7 +824,10 @@ tm_tb_kernel(const TbParams P)
         if (want_int) {
             if constexpr (NF == 0 && PART == TB_WHOLE) {
                 if (P.vac_mode == 2 && ldu(P.tile_vac, (int)tid) == (P.vac_tag | 1u)) {
-                    sweep<K, C, LOSSY, PML, VEC, false, false, false, NF, true, C == 1, true>(P, g, ring);
+                    // iz is +0 there in both state sets: the recording launch
+                    // read +0 and wrote +0, and iz + 0 * ez stays +0
+                    sweep<K, C, LOSSY, PML, VEC, false, false, false, NF, true, C == 1, true, false,
+                          LOSSY>(P, g, ring);
                 } else {
                     const bool v =
                         sweep<K, C, LOSSY, PML, VEC, false, false, false, NF, true, C == 1, false, true>(
@@ -1106,6 +1120,9 @@ tm_tb_batch_kernel(const fd2d_grid *__restrict__ grids, const int *__restrict__ 
                 v = free_space<C>(P, max(g.i0 - K, 0), min(g.i1 + K, P.nx - 1), g.j0);
                 if (threadIdx.x == 0) vac.work[u] = vac.tag | (v ? 1u : 0u);
             }
+            // (iz is read and written as in any unit: a third interior body
+            // for free-space units without iz, as tm_tb_kernel has, spills
+            // 60 bytes per lane here and costs more than it saves)
             if (v) sweep<K, C, true, true, VEC, false, false, false, 0, true, C == 1, true>(P, g, ring);
             else sweep<K, C, true, true, VEC, false, false, false, 0, true, C == 1>(P, g, ring);
         } else {

@@ -621,13 +621,13 @@ int fd2d_tm_plan(const fd2d_tm_desc *d, int nsteps, int *launches, int *depth, i
 // written with and what they describe.
 struct TileRecord {
     const unsigned *work;
-    const float *naz, *nbz;
+    const float *naz, *nbz, *iz, *iz2;
     int nx, ny, rows, cols, halo, depth;
     unsigned tag;
     bool same_tiles(const TileRecord &o) const
     {
-        return naz == o.naz && nbz == o.nbz && nx == o.nx && ny == o.ny && rows == o.rows &&
-               cols == o.cols && halo == o.halo;
+        return naz == o.naz && nbz == o.nbz && iz == o.iz && iz2 == o.iz2 && nx == o.nx &&
+               ny == o.ny && rows == o.rows && cols == o.cols && halo == o.halo;
     }
 };
 
@@ -711,8 +711,8 @@ int fd2d_tm_run(const fd2d_tm_desc *d, int t0, int nsteps, void *stream)
     bool reuse = false;
     if (vac) {
         const int kmax = nfused / L + (nfused % L ? 1 : 0);
-        TileRecord now{d->tile_work, d->md.naz, d->md.nbz, d->nx, d->ny, pl.rows, pl.cols, pl.halo,
-                       kmax, 0u};
+        TileRecord now{d->tile_work, d->md.naz, d->md.nbz, d->iz, d->iz2, d->nx, d->ny, pl.rows,
+                       pl.cols, pl.halo, kmax, 0u};
         std::lock_guard<std::mutex> g(tile_records_mutex());
         auto &recs = tile_records();
         auto it = std::find_if(recs.begin(), recs.end(),

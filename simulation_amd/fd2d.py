@@ -303,7 +303,8 @@ class TMSolver:
         self.form = int(form)
         # free-space tiles skip reading naz/nbz after a run call's first launch;
         # static_medium: the caller promises naz/nbz do not change between run
-        # calls, so the flags of an earlier call serve from the first launch.
+        # calls and iz is written only by the runs (forget_tiles otherwise),
+        # so the flags of an earlier call serve from the first launch.
         # One workspace per tile geometry (depth, rows, cols) and per depth of
         # the call's first launch (the launch that records reads the rows of
         # that depth): calls of other lengths (a short warm-up, a tail) keep
@@ -373,6 +374,13 @@ class TMSolver:
         call("fd2d_tm_plan", C.byref(d), int(nsteps), *[C.byref(o) for o in out])
         return dict(launches=out[0].value, depth=out[1].value, rows=out[2].value,
                     cols=out[3].value)
+
+    def forget_tiles(self) -> None:
+        """Drop the free-space flags: the next run records afresh.  Call it
+        after writing iz (or the medium) other than through runs; with
+        static_medium the flags otherwise assume iz is +0 wherever they mark
+        free space."""
+        self._tiles = {}
 
     def run(self, nsteps: int) -> None:
         """Advance by nsteps (asynchronous on the current stream).  The

@@ -137,6 +137,11 @@ def load_checkpoint(path: str, solver) -> int:
             if tuple(a.shape) != tuple(v.shape):
                 raise ValueError(f"{path}: {k} has shape {a.shape}, the solver {tuple(v.shape)}")
             v.copy_(torch.from_numpy(a).to(v.device))
+    # a restored state may hold another run's iz: free-space flags are
+    # recorded afresh by the next run
+    tm = getattr(solver, "tm", solver)
+    if hasattr(tm, "forget_tiles"):
+        tm.forget_tiles()
     _step_owner(solver).t = int(meta["t"])
     torch.cuda.synchronize()
     return int(meta["t"])

@@ -305,3 +305,35 @@ def test_free_space_flags_new_medium_in_freed_memory(gpu):
         assert same_bits(sb[k], sc[k]), k
     assert same_bits(b.probe_traces(), c.probe_traces())
     assert np.abs(sb["ez"][95:125, 165:195]).max() > 1e-6
+
+
+def test_free_space_flags_after_foreign_state(gpu, tmp_path):
+    """A state loaded from another medium's run (nonzero iz where this medium
+    is free space) into a solver that keeps its free-space flags across
+    calls: the load drops the flags, so the next run records afresh and
+    gives the bits of a solver without flags."""
+    from simulation_amd import fd2d, store
+    n = 320
+    cfg = fd2d.c2_config(n, 400)
+    cfg.depth, cfg.rows = 6, 20
+    print("[fd2d] free-space flags after a foreign state", flush=True)
+    a = fd2d.build_solver(cfg, gpu)
+    a.static_medium = True
+    a.run(60)                                     # records; flags kept
+    other = fd2d.c2_config(n, 400)
+    other.depth, other.rows = 6, 20
+    other.cylinders = [(100.0, 220.0, 30, 3.0, 0.04)]
+    other.sources = [(110, 200, 20, 6.0)]
+    c = fd2d.build_solver(other, gpu)
+    c.run(80)
+    assert np.abs(c.state()["iz"][80:120, 200:240]).max() > 0   # iz where a's medium is free
+    path = str(tmp_path / "foreign.npz")
+    store.save_checkpoint(path, c)
+    b = fd2d.build_solver(cfg, gpu)
+    b.tile_flags = False
+    for s in (a, b):
+        store.load_checkpoint(path, s)
+        s.run(60)
+    sa, sb = a.state(), b.state()
+    for k in ("dz", "iz", "ez", "hx", "hy", "ihx", "ihy"):
+        assert same_bits(sa[k], sb[k]), k
