@@ -131,7 +131,10 @@ int mw_default_rows(int nx)
 // 192 VGPRs: 2 waves); depth 11 moves the arrays once per 11 steps instead
 // of 10 and wins from 4096^2 on, where its one more halo column and warm-up
 // row per side cost less.  Rows follow nx/128 for 4096 <= nx < 12288 and
-// ~nx/170 outside.
+// ~nx/170 outside.  With the free-space tiles that move no iz (fewer bytes
+// per launch) the two depths are close below 16384^2: 8192^2 depth 10/11
+// rows 64: 852k/839k, rows 85: 831k/841k; 4096^2 rows 32: 679k/684k;
+// 16384^2 rows 96: 855k/886k (the autotuner picks per grid).
 void tm_defaults(int nx, int *depth, int *rows)
 {
     *depth = nx >= 4096 ? 11 : (nx >= 2048 ? 10 : 8);
