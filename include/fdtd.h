@@ -475,8 +475,10 @@ int lbc_train_grad(int n, const float *src, const long long *offs, long long tst
  * hist, hist2: [lbc_hist_elems(nx, ny)] floats each (npos x 4 x 30), the last four boundary
  * samples of every position (zero for fields at rest); the run leaves them
  * current in hist.  work: lbc_work_bytes(nx, ny) bytes of device memory for
- * the frame kernel's geometry tables (rewritten by every call; one run at a
- * time per work buffer).  nsteps >= 2 (an even number of launches).        */
+ * the frame kernel's geometry tables and the interior tiles' free-space
+ * flags (recorded by the first launch of every call, as with
+ * fd2d_tm_desc.tile_work; rewritten by every call; one run at a time per
+ * work buffer).  nsteps >= 2 (an even number of launches).                 */
 #define FDTD_LBC_MARGIN 32
 typedef struct fd2d_lbc_desc {
     const float *params;        /* lbc_param_count() FP32 parameters         */

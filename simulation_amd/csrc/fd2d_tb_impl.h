@@ -817,12 +817,12 @@ tm_tb_kernel(const TbParams P)
     // free space" -- so a later launch of smaller depth, which may find more
     // tiles interior, never reads an entry this recording did not write.
     const unsigned tid = (unsigned)by * (unsigned)nsx + (unsigned)bx;
-    if constexpr (NF == 0 && PART == TB_WHOLE) {
+    if constexpr (NF == 0 && PART != TB_BOUNDARY) {
         if (P.vac_mode == 1 && !interior && bi < 0 && threadIdx.x == 0) P.tile_vac[tid] = P.vac_tag;
     }
     if (interior) {
         if (want_int) {
-            if constexpr (NF == 0 && PART == TB_WHOLE) {
+            if constexpr (NF == 0 && PART != TB_BOUNDARY) {
                 if (P.vac_mode == 2 && ldu(P.tile_vac, (int)tid) == (P.vac_tag | 1u)) {
                     // iz is +0 there in both state sets: the recording launch
                     // read +0 and wrote +0, and iz + 0 * ez stays +0

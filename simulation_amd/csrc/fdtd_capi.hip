@@ -1181,10 +1181,21 @@ size_t lbc_hist_elems(int nx, int ny)
     return (size_t)lbc::npos(nx, ny) * lbc::HIST * lbc::NS;
 }
 
+// The learned run's work buffer: the frame kernel's geometry tables, then
+// the free-space flags of the interior part's tiles (one unsigned per tile;
+// room for the smallest tiles a plan allows: strips of 8 columns, blocks of
+// one row).
+static size_t lbc_tile_offset(int nx, int ny)
+{
+    return (lbc_frame_table_offset(nx, ny, LBC_MAX_DEPTH + 1) + 255) / 256 * 256;
+}
+
+static size_t lbc_tile_capacity(int nx, int ny) { return (size_t)nx * (size_t)((ny + 7) / 8); }
+
 size_t lbc_work_bytes(int nx, int ny)
 {
     if (nx < 64 || ny < 64) return 0;
-    return lbc_frame_table_offset(nx, ny, LBC_MAX_DEPTH + 1);
+    return lbc_tile_offset(nx, ny) + lbc_tile_capacity(nx, ny) * sizeof(unsigned);
 }
 
 int fd2d_lbc_margin(void) { return FDTD_LBC_MARGIN; }
@@ -1268,11 +1279,24 @@ int fd2d_lbc_run(const fd2d_tm_desc *d, const fd2d_lbc_desc *b, int t0, int nste
     if (!rc) rc = launch_lbc_frame_setup(d->nx, d->ny, kq, tabq, ax->s2);
     if (!rc && kr) rc = launch_lbc_frame_setup(d->nx, d->ny, kq + 1, tabr, ax->s2);
     if (rc) return rc;
+    // free-space tiles of the interior part, as in fd2d_tm_run: the first
+    // launch (the deepest) records them with a tag fresh to the call, the
+    // later ones run them without naz, nbz, iz (iz is +0 in both sets there)
+    {
+        const int w = 64 * pl.cols - 2 * p.halo;
+        const size_t tiles = (size_t)((d->ny + w - 1) / w) * (size_t)((d->nx + pl.rows - 1) / pl.rows);
+        if (tiles <= lbc_tile_capacity(d->nx, d->ny)) {
+            static std::atomic<unsigned> calls{0};
+            p.tile_vac = reinterpret_cast<unsigned *>(wk + lbc_tile_offset(d->nx, d->ny));
+            p.vac_tag = (calls.fetch_add(1) + 1u) << 1;
+        }
+    }
     int done = 0;
     for (int l = 0; l < L; ++l) {
         const int k = kq + (l < kr ? 1 : 0);
         float **in = (l % 2 == 0) ? A : B;
         float **out = (l % 2 == 0) ? B : A;
+        p.vac_mode = p.tile_vac ? (l == 0 ? 1 : 2) : 0;
         p.dz_i = in[0]; p.iz_i = in[1]; p.hx_i = in[2]; p.hy_i = in[3];
         p.ihx_i = p.ihy_i = nullptr;
         p.dz_o = out[0]; p.iz_o = out[1]; p.hx_o = out[2]; p.hy_o = out[3];
