@@ -217,9 +217,22 @@ __device__ __forceinline__ void st_rb(__amdgpu_buffer_rsrc_t rs, unsigned row, i
     }
 }
 
+// Loads through a buffer resource: offsets at or past its size read +0
+// without touching memory.
+template <int C>
+__device__ __forceinline__ void ld_rb(__amdgpu_buffer_rsrc_t rs, unsigned row, int j0, float (&v)[C])
+{
+#pragma unroll
+    for (int c = 0; c < C; ++c)
+        v[c] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(
+                                             rs, (row + (unsigned)(j0 + c)) << 2, 0, 0));
+}
+
 // VAC: a free-space tile (every naz 1, nbz 0): neither array is read.
-// ZIZ: and every iz it reads is +0, so iz is not read either.
-template <int C, bool LOSSY, bool PML, bool VEC, bool CLAMP, bool VAC = false, bool ZIZ = false,
+// ZIZ 1: and every iz it reads is +0, so iz is not read either; ZIZ 2: iz
+// through buffer resources of P.iz_bytes bytes (0: reads +0, stores
+// dropped), so one body serves both.
+template <int C, bool LOSSY, bool PML, bool VEC, bool CLAMP, bool VAC = false, int ZIZ = 0,
           class PR>
 __device__ __forceinline__ void load_row(const PR &P, int p, int j0, HRow<C> &H, DRow<C> &D)
 {
@@ -235,9 +248,11 @@ __device__ __forceinline__ void load_row(const PR &P, int p, int j0, HRow<C> &H,
     ld<C, VEC, CLAMP>(P.hx_i, row, j0, P.ny, H.hx);
     ld<C, VEC, CLAMP>(P.hy_i, row, j0, P.ny, H.hy);
     if (LOSSY) {
-        if (ZIZ) {
+        if constexpr (ZIZ == 1) {
 #pragma unroll
             for (int c = 0; c < C; ++c) D.iz[c] = 0.f;
+        } else if constexpr (ZIZ == 2) {
+            ld_rb<C>(out_rsrc(P.iz_i, P.iz_bytes), (unsigned)pc * (unsigned)P.ny, j0, D.iz);
         } else {
             ld<C, VEC, CLAMP>(P.iz_i, row, j0, P.ny, D.iz);
         }
@@ -284,10 +299,11 @@ __device__ __forceinline__ void col_coef(const PR &P, int j0, ColCoef<C> &cc)
 // columns the wave computes.
 // VAC: a free-space tile (naz 1, nbz 0 as constants, no profile ring);
 // REC: AND into vac whether the row's naz, nbz, iz are exactly 1, +0, +0;
-// ZIZ (with VAC): every iz of the tile is +0 in both state sets, so it is
-// neither read nor stored (in free space iz + 0 * ez stays +0).
+// ZIZ 1 (with VAC): every iz of the tile is +0 in both state sets, so it is
+// neither read nor stored (in free space iz + 0 * ez stays +0); ZIZ 2: iz
+// read and stored through buffer resources of P.iz_bytes bytes.
 template <int K, int C, bool LOSSY, bool PML, bool VEC, bool BROW, bool BCOL, bool SP, int NF,
-          bool BS, bool PRO = false, bool VAC = false, bool REC = false, bool ZIZ = false, class PR>
+          bool BS, bool PRO = false, bool VAC = false, bool REC = false, int ZIZ = 0, class PR>
 __device__ __forceinline__ void iterate(const PR &P, const Geom &g, const ColCoef<C> &cc,
                                         NaRing<K, C> &ring,
                                         int p, const HRow<C> &Hload, const DRow<C> &Dload,
@@ -337,7 +353,7 @@ __device__ __forceinline__ void iterate(const PR &P, const Geom &g, const ColCoe
 #pragma unroll
             for (int c = 0; c < C; ++c) {
                 Din.dz[c] = Si[s - 1].d.dz[c];
-                Din.iz[c] = ZIZ ? 0.f : Si[s - 1].d.iz[c];
+                Din.iz[c] = ZIZ == 1 ? 0.f : Si[s - 1].d.iz[c];
                 Din.na[c] = VAC ? 1.f : rd[c];
                 Din.nb[c] = (LOSSY && !VAC) ? rd[C + c] : 0.f;
             }
@@ -531,7 +547,10 @@ __device__ __forceinline__ void iterate(const PR &P, const Geom &g, const ColCoe
                 const bool okr = r >= g.i0 && r < g.i1 && r >= sw && r < P.nx - sw;
                 const unsigned rowr = (unsigned)r * (unsigned)ny;
                 st_rb<C>(out_rsrc(P.dz_o, nb), rowr, g.j0, c0, c1, okr, Dn.dz);
-                if (LOSSY && !ZIZ) st_rb<C>(out_rsrc(P.iz_o, nb), rowr, g.j0, c0, c1, okr, Dn.iz);
+                if constexpr (LOSSY && ZIZ == 2)
+                    st_rb<C>(out_rsrc(P.iz_o, P.iz_bytes), rowr, g.j0, c0, c1, okr, Dn.iz);
+                else if constexpr (LOSSY && ZIZ == 0)
+                    st_rb<C>(out_rsrc(P.iz_o, nb), rowr, g.j0, c0, c1, okr, Dn.iz);
                 st_rb<C>(out_rsrc(P.ez_o, P.ez_o ? nb : 0u), rowr, g.j0, c0, c1, okr, En);
                 const bool okq = q >= g.i0 && q < g.i1 && q >= sw && q < P.nx - sw;
                 const unsigned rowq = (unsigned)q * (unsigned)ny;
@@ -545,7 +564,8 @@ __device__ __forceinline__ void iterate(const PR &P, const Geom &g, const ColCoe
                 if (r >= g.i0 && r < g.i1 && r >= sw && r < P.nx - sw) {
                     const unsigned row = (unsigned)r * (unsigned)ny;
                     st<C, VEC>(P.dz_o, row, g.j0, c0, c1, Dn.dz);
-                    if (LOSSY && !ZIZ) st<C, VEC>(P.iz_o, row, g.j0, c0, c1, Dn.iz);
+                    static_assert(ZIZ != 2, "iz through buffer resources: buffer-store sweeps only");
+                    if (LOSSY && ZIZ == 0) st<C, VEC>(P.iz_o, row, g.j0, c0, c1, Dn.iz);
                     if (P.ez_o) st<C, VEC>(P.ez_o, row, g.j0, c0, c1, En);
                     if constexpr (NF > 0) {
                         const size_t plane = (size_t)P.nx * (size_t)ny;
@@ -594,7 +614,7 @@ __device__ __forceinline__ void load_ft(const PR &P, int p, int j0, FRow<C, NF> 
 // Returns, with REC, whether every naz, nbz, iz the sweep read is exactly 1,
 // +0, +0 (wave-uniform); otherwise false.
 template <int K, int C, bool LOSSY, bool PML, bool VEC, bool BROW, bool BCOL, bool SP, int NF = 0,
-          bool BS = false, bool PP = false, bool VAC = false, bool REC = false, bool ZIZ = false,
+          bool BS = false, bool PP = false, bool VAC = false, bool REC = false, int ZIZ = 0,
           class PR>
 __device__ __forceinline__ bool sweep(const PR &P, const Geom &g, NaRing<K, C> &ring)
 {
@@ -827,7 +847,7 @@ tm_tb_kernel(const TbParams P)
                     // iz is +0 there in both state sets: the recording launch
                     // read +0 and wrote +0, and iz + 0 * ez stays +0
                     sweep<K, C, LOSSY, PML, VEC, false, false, false, NF, true, C == 1, true, false,
-                          LOSSY>(P, g, ring);
+                          LOSSY ? 1 : 0>(P, g, ring);
                 } else {
                     const bool v =
                         sweep<K, C, LOSSY, PML, VEC, false, false, false, NF, true, C == 1, false, true>(
@@ -987,8 +1007,9 @@ int launch_part(const TbParams &p, bool lossy, bool pml, int part, hipStream_t s
     }
 }
 
-// Whether every naz, nbz of rows [r0, r1] on the wave's columns (j0 .. j0+C-1
-// per lane, inside the grid) is exactly 1, 0: bit patterns, wave-uniform.
+// Whether every naz, nbz, iz (input set) of rows [r0, r1] on the wave's
+// columns (j0 .. j0+C-1 per lane, inside the grid) is exactly 1, +0, +0: bit
+// patterns, wave-uniform.
 template <int C, class PR>
 __device__ __forceinline__ bool free_space(const PR &P, int r0, int r1, int j0)
 {
@@ -1000,6 +1021,7 @@ __device__ __forceinline__ bool free_space(const PR &P, int r0, int r1, int j0)
         for (int c = 0; c < C; ++c) {
             ok &= __float_as_uint(ldg(P.naz + row + c)) == 0x3f800000u;
             ok &= __float_as_uint(ldg(P.nbz + row + c)) == 0u;
+            ok &= __float_as_uint(ldg(P.iz_i + row + c)) == 0u;
         }
     }
     return __builtin_amdgcn_ballot_w64(!ok) == 0;
@@ -1032,6 +1054,7 @@ struct BatchParams {
     One probe_i, probe_j;
     float *probe_out;
     int probe_len;
+    unsigned iz_bytes;      // iz of free-space units: 0 (reads +0, no stores) or the grid's bytes
     static constexpr int skip_w = 0;
     static constexpr int tf_npml = 0, inc_t0 = 0, inc_stride = 0;
     static constexpr const float *inc = nullptr;
@@ -1116,14 +1139,17 @@ tm_tb_batch_kernel(const fd2d_grid *__restrict__ grids, const int *__restrict__ 
         // SIMD.)
         if (!brow && !bcol && !sp) {
             bool v = vac.mode == 2 && ldu(vac.work, u) == (vac.tag | 1u);
+            // the later launches move no iz in free-space units (+0 in both
+            // sets: the recording launch read +0 and writes +0); iz goes
+            // through buffer resources of size 0 there, so the recording
+            // launch and the later ones share one body (a second free-space
+            // body would spill)
+            P.iz_bytes = v ? 0u : (unsigned)P.nx * (unsigned)P.ny * 4u;
             if (vac.mode == 1) {
                 v = free_space<C>(P, max(g.i0 - K, 0), min(g.i1 + K, P.nx - 1), g.j0);
                 if (threadIdx.x == 0) vac.work[u] = vac.tag | (v ? 1u : 0u);
             }
-            // (iz is read and written as in any unit: a third interior body
-            // for free-space units without iz, as tm_tb_kernel has, spills
-            // 60 bytes per lane here and costs more than it saves)
-            if (v) sweep<K, C, true, true, VEC, false, false, false, 0, true, C == 1, true>(P, g, ring);
+            if (v) sweep<K, C, true, true, VEC, false, false, false, 0, true, C == 1, true, false, 2>(P, g, ring);
             else sweep<K, C, true, true, VEC, false, false, false, 0, true, C == 1>(P, g, ring);
         } else {
             if (vac.mode == 1 && threadIdx.x == 0) vac.work[u] = vac.tag;
