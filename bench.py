@@ -728,7 +728,7 @@ def main():
         record_traffic(*a.record_traffic, last=a.traffic_last)
         return
     if a.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; --gpus N>1 is not supported "
+        print(json.dumps({"error": "the bench runs on one GPU only; --gpus N>1 is not supported "
                                    "(the multi-GPU slab decomposition is a design, not code)"}))
         sys.exit(2)
     if a.copy_calibration:
