@@ -25,6 +25,9 @@ from . import _lib
 from ._dev import fptr, ptr, require_gpu, stream
 from ._lib import call
 
+# state tensors a TM descriptor points to
+_DESC_TENSORS = ("dz", "iz", "ez", "hx", "hy", "ihx", "ihy", "dz2", "iz2", "hx2", "hy2", "ihx2", "ihy2")
+
 pmlayer = namedtuple("pmlayer", ("fx1", "fx2", "fx3", "fy1", "fy2", "fy3",
                                  "gx2", "gx3", "gy2", "gy3"))
 medium = namedtuple("medium", ("naz", "nbz"))
@@ -314,6 +317,7 @@ class TMSolver:
         self.tile_flags = True
         self.static_medium = False
         self._tiles = {}
+        self._dkey, self._dcache, self._tcache = None, None, {}
         self.band_rows, self.band_cols = self.default_band()
         self.t = 0
         self._work = torch.empty(_lib.ENERGY_WORK, dtype=torch.float64, device=dev)
@@ -375,6 +379,20 @@ class TMSolver:
         return dict(launches=out[0].value, depth=out[1].value, rows=out[2].value,
                     cols=out[3].value)
 
+    def _desc_key(self) -> tuple:
+        """Everything desc() reads (tensors by object and address): a run
+        call reuses the last descriptor while this is unchanged."""
+        def t(x):
+            return None if x is None else (id(x), x.data_ptr())
+        pm = self.pml
+        return (self.nx, self.ny,
+                tuple(t(getattr(self, k)) for k in _DESC_TENSORS),
+                t(self.md.naz), t(self.md.nbz), t(self.src_wave), t(self.probe_out),
+                None if pm is None else tuple(t(getattr(pm, k)) for k in pmlayer._fields),
+                tuple(self.sources), self.wave_len, tuple(self.probes), self.probe_len,
+                self.depth, self.rows, self.cols, tuple(self.band_rows), tuple(self.band_cols),
+                self.form)
+
     def forget_tiles(self) -> None:
         """Drop the free-space flags: the next run records afresh.  Call it
         after writing iz (or the medium) other than through runs; with
@@ -386,14 +404,27 @@ class TMSolver:
         """Advance by nsteps (asynchronous on the current stream).  The
         library records the free-space tiles of the plan in a workspace the
         solver keeps (fd2d_tm_desc.tile_work; one int per tile)."""
-        d = self.desc()
+        # the descriptor and the tile geometry of a call are rebuilt only
+        # when something they are made of changed (tens of microseconds of
+        # host time per call otherwise)
+        dkey = self._desc_key()
+        if dkey != self._dkey:
+            self._dkey, self._dcache, self._tcache = dkey, self.desc(), {}
+        d = self._dcache
+        d.tile_work, d.tile_elems, d.tile_keep = None, 0, 0
         if self.tile_flags and nsteps > 0:
-            n = _lib.lib().fd2d_tm_tile_elems(C.byref(d), int(nsteps))
+            geo = self._tcache.get(int(nsteps))
+            if geo is None:
+                n = _lib.lib().fd2d_tm_tile_elems(C.byref(d), int(nsteps))
+                key = None
+                if n:
+                    out = [C.c_int() for _ in range(4)]
+                    call("fd2d_tm_plan", C.byref(d), int(nsteps), *[C.byref(o) for o in out])
+                    L = max(out[0].value, 1)
+                    key = (out[1].value, out[2].value, out[3].value, -(-int(nsteps) // L))
+                geo = self._tcache[int(nsteps)] = (n, key)
+            n, key = geo
             if n:
-                out = [C.c_int() for _ in range(4)]
-                call("fd2d_tm_plan", C.byref(d), int(nsteps), *[C.byref(o) for o in out])
-                L = max(out[0].value, 1)
-                key = (out[1].value, out[2].value, out[3].value, -(-int(nsteps) // L))
                 ws = self._tiles.get(key)
                 first = ws is None or ws.numel() < n
                 if first:
