@@ -860,6 +860,17 @@ tm_tb_kernel(const TbParams P)
             }
         }
     } else if (want_bnd) {
+        if constexpr (PART == TB_BOUNDARY && NF == 0) {
+            // a tile of the boundary part inside the part's own band runs
+            // the interior body (the learned boundary's part 1: tiles near
+            // the frame but clear of the grid's edge rows and columns)
+            const bool ebrow = !(g.i0 - 2 * K >= P.erlo && g.i1 - 1 + K < P.erhi);
+            const bool ebcol = !(js >= P.eclo && js + S <= P.echi);
+            if (!ebrow && !ebcol && !sp) {
+                sweep<K, C, LOSSY, PML, VEC, false, false, false, NF, true, C == 1>(P, g, ring);
+                return;
+            }
+        }
         if (!brow && !sp) sweep<K, C, LOSSY, PML, VEC, false, true, false, NF, NF == 0>(P, g, ring);
         else sweep<K, C, LOSSY, PML, VEC, true, true, true, NF, NF == 0>(P, g, ring);
     }

@@ -1247,6 +1247,9 @@ int fd2d_lbc_run(const fd2d_tm_desc *d, const fd2d_lbc_desc *b, int t0, int nste
     // neither the frame of launch l nor conflicts with it.
     const int DZ = 3 * pl.depth + 6;
     p.rlo = DZ; p.rhi = d->nx - DZ; p.clo = DZ; p.chi = d->ny - DZ;
+    // no PML: boundary-part tiles clear of the edge rows and columns take
+    // the interior body
+    p.erlo = 1; p.erhi = d->nx - 1; p.eclo = 1; p.echi = d->ny - 1;
     p.nsrc = d->nsrc;
     for (int k = 0; k < d->nsrc; ++k) { p.src_i[k] = d->src_i[k]; p.src_j[k] = d->src_j[k]; }
     p.src_wave = d->src_wave; p.wave_len = d->wave_len;

@@ -87,6 +87,11 @@ struct TbParams {
     // interior band: rows [rlo, rhi) and columns [clo, chi) lie inside
     // [1, n-2] and hold the default profile values (empty when unknown)
     int rlo, rhi, clo, chi;
+    // the boundary part's (TB_BOUNDARY) own band: its tiles inside it need
+    // neither the edge rows and columns nor profile values (the learned
+    // boundary's partition band is wider than that); empty: every tile of
+    // the part runs the general bodies
+    int erlo, erhi, eclo, echi;
     int t_first;                   // time index of the first fused step
     int nsrc;
     int src_i[FDTD_MAX_SOURCES], src_j[FDTD_MAX_SOURCES];
